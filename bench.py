@@ -1,0 +1,221 @@
+"""LETKF analysis benchmark on one MI355X.
+
+Workload: the k=64 clustered-obs cycle -- grid 360x180x32 (ps + u, v, T:
+n = 97 per column), k = 64, 500,000 obs rows (70% on 40 great-circle swaths,
+20% at 1,000 station columns, 10% uniform), GC localisation c = 300 km
+(support 600 km), one local analysis per column: N_A = 64,800.
+
+A step is one analysis cycle with every input already in HBM: the obs-site
+index build (set_obs_device) plus the analysis (analyze_device: selection,
+obs-space products, FP64 eigensolve + transform, ensemble update).  The
+mixed-precision form (BF16 hi/lo split products, FP32 accumulation, FP64
+eigensolve) is the headline; the plain FP64 form is timed beside it.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dida-thhpca_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector and FP64 matrix peak
+PEAK_BF16_TFLOPS = 2500.0   # dense bf16 MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--mixed", type=int, default=2, help="precision of the headline mode (1 fp32, 2 bf16x3)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=16, help="every Nth column for the CPU baseline")
+    ap.add_argument("--parity-cols", type=int, default=48)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the path shards by latitude bands, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from dida_letkf import synth, engine
+    from dida_letkf.config import PREC_FP64
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    inst = synth.generate(args.config, seed=args.seed)
+    cfg = inst.cfg
+    log(f"generated {args.config}: grid {cfg.nlat}x{cfg.nlon}x{cfg.nz} n={cfg.n} k={cfg.k} "
+        f"obs rows={inst.obs.n_rows} sites={inst.obs.n_sites} in {time.time() - t0:.1f}s")
+    N_A = cfg.n_analyses()
+    o = inst.obs
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d = dict(xb=T(inst.xb), axb=T(inst.axb), lat=T(o.site_lat), lon=T(o.site_lon),
+             rp=T(o.site_row_ptr.astype(np.int32)), yb=T(o.yb), dep=T(o.dep), rinv=T(o.rinv),
+             vco=T(o.vcoord))
+    stream = torch.cuda.current_stream()
+
+    def make_engine(prec):
+        import copy
+        c = copy.copy(cfg)
+        c.precision = prec
+        eng = engine.LetkfEngine(c)
+        eng.set_stream(stream.cuda_stream)
+        eng.set_grid(inst.grid_lat, inst.grid_lon)
+        return eng
+
+    def step(eng, xa, axa):
+        eng.set_obs_device(o.n_sites, d["lat"].data_ptr(), d["lon"].data_ptr(), d["rp"].data_ptr(),
+                           o.n_rows, d["yb"].data_ptr(), d["dep"].data_ptr(), d["rinv"].data_ptr(),
+                           d["vco"].data_ptr())
+        return eng.analyze_device(d["xb"].data_ptr(), d["axb"].data_ptr(), xa.data_ptr(),
+                                  axa.data_ptr())
+
+    def run_mode(prec):
+        eng = make_engine(prec)
+        xa = torch.zeros_like(d["xb"])
+        axa = torch.zeros_like(d["axb"])
+        for _ in range(args.warmup):
+            step(eng, xa, axa)
+        torch.cuda.synchronize()
+        k_ms = dict(select=0.0, products=0.0, transform=0.0)
+        ts = time.perf_counter()
+        flagged = 0
+        for _ in range(args.steps):
+            flagged += step(eng, xa, axa)
+            st = eng.stats
+            k_ms["select"] += st.ms_select
+            k_ms["products"] += st.ms_products
+            k_ms["transform"] += st.ms_transform
+        torch.cuda.synchronize()
+        el = time.perf_counter() - ts
+        stats = eng.stats.as_dict()
+        eng.close()
+        for kk in k_ms:
+            k_ms[kk] /= max(args.steps, 1)
+        return dict(elapsed=el, ms_step=1e3 * el / args.steps, value=N_A * args.steps / el,
+                    kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa)
+
+    res = {}
+    modes = [args.mixed] + ([] if args.no_fp64 else [PREC_FP64])
+    for prec in modes:
+        r = run_mode(prec)
+        res[prec] = r
+        log(f"precision {prec}: {r['value']:.4g} analyses/s, {r['ms_step']:.2f} ms/step, "
+            f"kernels {r['kernel_ms']}, rows {r['stats']['total_rows']} max {r['stats']['max_rows']}")
+
+    # ---- accuracy: FP64 vs oracle on sampled columns; mixed vs FP64 increments
+    import orc
+    acc = {}
+    rng = np.random.default_rng(1)
+    cols = np.sort(rng.choice(N_A, min(args.parity_cols, N_A), replace=False)).astype(np.int32)
+    ref = orc.letkf_compute(inst, columns=cols, nthreads=min(16, os.cpu_count() or 1))
+    ref_full = ref["xa"] + ref["axa"][..., None]
+    lon_i, lat_i = cols // cfg.nlat, cols % cfg.nlat
+    if PREC_FP64 in res:
+        g64 = (res[PREC_FP64]["xa"] + res[PREC_FP64]["axa"][..., None]).cpu().numpy()
+        diff = np.abs(g64[lon_i, lat_i] - ref_full[lon_i, lat_i]).max()
+        acc["fp64_vs_oracle_max_rel"] = float(diff / np.abs(ref_full[lon_i, lat_i]).max())
+        gm = (res[args.mixed]["xa"] + res[args.mixed]["axa"][..., None])
+        g64t = res[PREC_FP64]["xa"] + res[PREC_FP64]["axa"][..., None]
+        xbf = d["xb"] + d["axb"][..., None]
+        per_var = {}
+        vs = cfg.var_start() - 1
+        names = ["ps", "u", "v", "T"][: cfg.n_var]
+        for v in range(cfg.n_var):
+            nl = 1 if cfg.var_lev()[v] == 1 else cfg.nz
+            sl = slice(int(vs[v]), int(vs[v]) + nl)
+            num = torch.sqrt(torch.mean((gm[:, :, sl] - g64t[:, :, sl]) ** 2))
+            den = torch.sqrt(torch.mean((g64t[:, :, sl] - xbf[:, :, sl]) ** 2))
+            per_var[names[v]] = float(num / den)
+        acc["mixed_increment_rms_rel_err"] = per_var
+    log(f"accuracy: {acc}")
+
+    # ---- CPU baseline: the oracle (port of the reference loop) on a sample
+    cpu = None
+    if not args.no_cpu:
+        ncore = min(16, os.cpu_count() or 1)
+        samp = np.arange(0, N_A, args.cpu_sample, dtype=np.int32)
+        tc = time.perf_counter()
+        orc.letkf_compute(inst, columns=samp, nthreads=ncore)
+        tcpu = time.perf_counter() - tc
+        cpu = dict(value=float(samp.size / tcpu), unit="local analyses/s", cores=ncore, kind="port",
+                   sample=f"every {args.cpu_sample}th column of the same cycle ({samp.size} analyses, "
+                          f"{tcpu:.1f} s wall); brute-force gen_yb_dis over all obs as on one rank, "
+                          f"own Householder+QL eigensolver, FP64")
+        log(f"cpu baseline: {cpu}")
+
+    # ---- roofline of the dominant kernel (largest average duration)
+    head = res[args.mixed]
+    km = head["kernel_ms"]
+    dom = max(("products", "transform"), key=lambda x: km[x])
+    k = cfg.k
+    st = head["stats"]
+    m_col = cfg.n
+    if dom == "transform":
+        # textbook symmetric-QR eigensolve with vectors 9k^3 (Golub & Van Loan
+        # Sec. 8.3) + T = V f V^T 2k^3 + overline_w 4k^2 + update 2k^2 m
+        flops = N_A * (9 * k**3 + 2 * k**3 + 4 * k**2 + 2 * k**2 * m_col)
+        peak = PEAK_FP64_TFLOPS
+    else:
+        flops = k * (k + 1) * st["total_rows"] + 2 * k * st["total_rows"]
+        peak = PEAK_BF16_TFLOPS if args.mixed == 2 else PEAK_FP64_TFLOPS * 2
+    achieved = flops / (km[dom] * 1e-3) / 1e12
+    roof = dict(bound="mfma", kernel=dom, achieved=round(achieved, 4), peak=peak, unit="TFLOP/s",
+                frac=round(achieved / peak, 5), traffic=None)
+
+    line = {
+        "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
+        "value": round(head["value"], 2),
+        "unit": "local analyses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+f64" if args.mixed == 2 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: 360x180x32 grid, n=97, k=64, 500k clustered obs, "
+                               f"c={cfg.local_dist:g} km, N_A={N_A} analyses/cycle",
+                   "n_analyses": N_A, "k": k, "obs_rows": o.n_rows,
+                   "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
+                   "parallelism": "1 GPU"},
+        "kernel_ms": head["kernel_ms"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+    }
+    if PREC_FP64 in res:
+        line["fp64_value"] = round(res[PREC_FP64]["value"], 2)
+        line["fp64_ms_per_step"] = round(res[PREC_FP64]["ms_step"], 3)
+        line["fp64_kernel_ms"] = res[PREC_FP64]["kernel_ms"]
+        line["mixed_over_fp64"] = round(head["value"] / res[PREC_FP64]["value"], 3)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,98 @@
+// letkf_device.h -- device-side helpers shared by the LETKF kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dida {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// pro_info.F90:148-161: pi = atan(1.0)*4 evaluated in real(4), and the
+// real(4) literal a = 6371.393, both promoted to real(8).
+constexpr double kPiF32 = 3.1415927410125732;
+constexpr double kEarthF32 = 6371.39306640625;
+// letkf_yb.F90:273-278 real(4) literals 1.666667 and 0.6666667
+constexpr double kC1666667 = 1.66666698455810546875;
+constexpr double kC0666667 = 0.666666686534881591796875;
+
+// Gaspari-Cohn, letkf_yb.F90:262-286, in the Fortran operation order
+// without contraction.
+__device__ __forceinline__ double schr2(double ri) {
+#pragma clang fp contract(off)
+  double rci;
+  if (ri > 2.0) {
+    rci = 0.0;
+  } else if (ri > 1.0) {
+    double rj = ri / 12.0 - 0.5;
+    rj = rj * ri + 0.625;
+    rj = rj * ri + kC1666667;
+    rci = (rj * ri - 5.0) * ri + 4.0 - kC0666667 / ri;
+  } else if (ri > 0.0) {
+    double rk = ri * ri;
+    double rj = (-0.25 * ri + 0.5) * ri + 0.625;
+    rci = (rj * ri - kC1666667) * rk + 1.0;
+  } else {
+    rci = 1.0;
+  }
+  return rci;
+}
+
+// letkf_yb.F90:287-302
+__device__ __forceinline__ double schr(double ri) { return ri > 2.0 ? 0.0 : 1.0; }
+
+__device__ __forceinline__ double gc_weight(double ri, int gc_filter) {
+  return gc_filter ? schr2(ri) : schr(ri);
+}
+
+// latlon_dist_r8 with the per-point factors precomputed:
+// c1 = cos(x1/180*pi), s1 = sin(x1/180*pi), l1 = y1/180*pi (same for 2).
+__device__ __forceinline__ double latlon_dist_pre(double c1, double s1, double l1, double c2,
+                                                  double s2, double l2) {
+#pragma clang fp contract(off)
+  double c3 = cos(l1 - l2);
+  double v = c1 * c2 * c3 + s1 * s2;
+  v = (v > 1.0) ? 1.0 : v;   // min(v, 1.0); NaN passes through like Fortran MIN
+  return kEarthF32 * acos(v);
+}
+
+__device__ __forceinline__ double deg2rad_ref(double x) {
+#pragma clang fp contract(off)
+  return x / 180.0 * kPiF32;
+}
+
+// Butterfly sums: every lane ends with the bitwise-same value (IEEE addition
+// is commutative, so partners add the same two operands).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// exclusive prefix sum of an int over the 64 lanes
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
+}  // namespace dida

@@ -1,0 +1,803 @@
+// letkf_engine.hip -- host runtime behind the C ABI (include/dida_letkf.h).
+//
+// Owns device state per handle: grid, the binned obs-site index, obs rows in
+// the storage precision of the configured mode, and per-batch work buffers.
+// One analysis call runs, per batch of horizontal columns:
+//   select(count) -> exclusive scan -> select(fill) -> products -> transform
+// on the handle's stream.  No host synchronisation happens inside a batch
+// except to read the batch's total row count (to size the row list).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dida_letkf.h"
+#include "letkf_kernels.h"
+
+using namespace dida;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return set_error(e_ == hipErrorOutOfMemory ? DIDA_ERR_NOMEM : DIDA_ERR_HIP,          \
+                       "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,    \
+                       __LINE__);                                                          \
+  } while (0)
+
+#define RC_TRY(expr)            \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ < 0) return rc_;    \
+  } while (0)
+
+// Growable device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes && p) return DIDA_OK;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_error(DIDA_ERR_NOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return DIDA_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+__global__ void k_f64_to_f32(long long n, const double* __restrict__ in, float* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+__global__ void k_iota(long long n, int* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int)i;
+}
+
+int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
+
+}  // namespace
+
+struct dida_letkf_handle {
+  dida_letkf_config cfg;
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int n = 0, nlat_c = 0, nlon_c = 0, nzc = 1, kp = 32;
+  long long n_h = 0, n_an = 0;
+  std::vector<double> hyai, hybi;
+  bool have_grid = false, have_obs = false;
+  // grid
+  DevBuf glat, glon;
+  // raw obs staging (device)
+  DevBuf o_lat, o_lon, o_rowptr, o_yb64, o_dep, o_rinv64, o_vco;
+  // obs index and rows
+  int n_sites = 0, n_rows = 0;
+  int nb_lat = 0, nb_lon = 0;
+  double bdeg_lat = 0, bdeg_lon = 0, theta_deg = 0;
+  DevBuf key_in, key_out, val_in, val_out, hist, bin_start, sort_tmp;
+  DevBuf s_cos, s_sin, s_lonr, s_row0, s_nrow;
+  DevBuf yb32, rinv32;
+  DevBuf d_hyai, d_hybi;
+  // work
+  DevBuf cnt, off, scan_tmp, row_idx, row_gc, Abuf, bbuf, status, trans, wout;
+  DevBuf x_xb, x_axb, x_xa, x_axa;   // host-entry staging
+  std::vector<int> nobsl_host;
+  long long max_batch = 0;
+};
+
+extern "C" {
+
+const char* dida_letkf_last_error(void) { return g_last_error.c_str(); }
+int dida_letkf_abi_version(void) { return DIDA_LETKF_ABI_VERSION; }
+
+void dida_letkf_config_default(dida_letkf_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->k = 20;
+  c->nlat = 1;
+  c->nlon = 1;
+  c->nz = 1;
+  c->n2d = 1;
+  c->n3d = 0;
+  c->halo = 0;
+  c->cube_n = 1;
+  c->cube_nz = 32;
+  c->local_dist = 0.0;
+  c->vert_dist = 0.0;
+  c->inflation_factor = 1.0;
+  c->obs_polar = 91.0;
+  c->gc_filter = 0;
+  c->gc_filter_vertical = 0;
+  c->precision = DIDA_PREC_FP64;
+  c->vcoord_mode = DIDA_VCOORD_LEVEL;
+  c->ps_index = 0;
+  c->p0 = 1.0e5;
+  c->ptop = 0.0;
+}
+
+long long dida_letkf_n_analyses(const dida_letkf_config* c) {
+  if (!c || c->cube_n < 1 || c->nlat < 1 || c->nlon < 1) return -1;
+  long long nh = (long long)((c->nlat + c->cube_n - 1) / c->cube_n) *
+                 ((c->nlon + c->cube_n - 1) / c->cube_n);
+  if (c->vert_dist > 0.0) {
+    int cz = c->cube_nz > 0 ? c->cube_nz : c->nz;
+    nh *= (c->nz + cz - 1) / cz;
+  }
+  return nh;
+}
+
+int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
+  if (!cfg || !out) return set_error(DIDA_ERR_ARG, "null argument");
+  *out = nullptr;
+  const dida_letkf_config& c = *cfg;
+  if (c.k < 2 || c.k > 128) return set_error(DIDA_ERR_ARG, "k=%d outside 2..128", c.k);
+  if (c.k > 64) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: transform kernel supports k<=64", c.k);
+  if (c.nlat < 1 || c.nlon < 1 || c.nz < 1 || c.n2d < 0 || c.n3d < 0 || c.n2d + c.n3d < 1)
+    return set_error(DIDA_ERR_ARG, "bad grid/variable sizes");
+  if (c.n2d + c.n3d > kMaxVars) return set_error(DIDA_ERR_ARG, "more than %d variables", kMaxVars);
+  if (c.halo < 0 || c.cube_n < 1) return set_error(DIDA_ERR_ARG, "bad halo/cube_n");
+  if (!(c.local_dist > 0.0)) return set_error(DIDA_ERR_ARG, "local_dist must be > 0");
+  if (!(c.inflation_factor > 0.0)) return set_error(DIDA_ERR_ARG, "inflation_factor must be > 0");
+  if (c.precision < DIDA_PREC_FP64 || c.precision > DIDA_PREC_BF16X3)
+    return set_error(DIDA_ERR_ARG, "bad precision %d", c.precision);
+  if (c.vert_dist > 0.0) {
+    if (c.vcoord_mode == DIDA_VCOORD_LNP && (!c.hyai || !c.hybi))
+      return set_error(DIDA_ERR_ARG, "LNP vertical mode needs hyai/hybi");
+    if (c.vcoord_mode != DIDA_VCOORD_LEVEL && c.vcoord_mode != DIDA_VCOORD_LNP)
+      return set_error(DIDA_ERR_ARG, "bad vcoord_mode");
+    if (c.vcoord_mode == DIDA_VCOORD_LNP && (c.ps_index < 0 || c.ps_index >= c.n2d + c.n3d * c.nz))
+      return set_error(DIDA_ERR_ARG, "bad ps_index");
+  }
+  dida_letkf_handle* h = new (std::nothrow) dida_letkf_handle();
+  if (!h) return set_error(DIDA_ERR_NOMEM, "host allocation failed");
+  h->cfg = c;
+  h->cfg.hyai = nullptr;
+  h->cfg.hybi = nullptr;
+  if (c.vert_dist > 0.0 && c.vcoord_mode == DIDA_VCOORD_LNP) {
+    h->hyai.assign(c.hyai, c.hyai + c.nz + 1);
+    h->hybi.assign(c.hybi, c.hybi + c.nz + 1);
+  }
+  if (h->cfg.cube_nz < 1) h->cfg.cube_nz = c.nz;
+  h->n = c.n2d + c.n3d * c.nz;
+  h->nlat_c = (c.nlat + c.cube_n - 1) / c.cube_n;
+  h->nlon_c = (c.nlon + c.cube_n - 1) / c.cube_n;
+  h->nzc = c.vert_dist > 0.0 ? (c.nz + h->cfg.cube_nz - 1) / h->cfg.cube_nz : 1;
+  h->n_h = (long long)h->nlat_c * h->nlon_c;
+  h->n_an = h->n_h * h->nzc;
+  h->kp = kp_of(c.k);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return set_error(DIDA_ERR_HIP, "HIP init failed: %s", hipGetErrorString(e));
+  }
+  h->stream = h->own_stream;
+  // obs-space support angle 2c/a (radians -> degrees) with margin for the
+  // real(4) pi scaling of the reference's distance
+  const double a = 6371.39306640625;
+  h->theta_deg = (2.0 * c.local_dist / a) * (180.0 / M_PI) * (1.0 + 1e-6) + 1e-2;
+  double bd = std::min(10.0, std::max(0.5, h->theta_deg / 2.0));
+  h->nb_lat = (int)std::ceil(180.0 / bd);
+  h->nb_lon = (int)std::ceil(360.0 / bd);
+  h->bdeg_lat = 180.0 / h->nb_lat;
+  h->bdeg_lon = 360.0 / h->nb_lon;
+  // columns per batch: bound the products buffer to ~4 GiB
+  long long per_col = (long long)h->nzc * h->kp * (h->kp + 1) * 8;
+  long long mb = std::max<long long>(1, (4LL << 30) / per_col);
+  if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
+  h->max_batch = std::min<long long>(mb, h->n_h);
+  *out = h;
+  return DIDA_OK;
+}
+
+int dida_letkf_destroy(dida_letkf_handle* h) {
+  if (!h) return DIDA_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  DevBuf* bufs[] = {&h->glat, &h->glon, &h->o_lat, &h->o_lon, &h->o_rowptr, &h->o_yb64,
+                    &h->o_dep, &h->o_rinv64, &h->o_vco, &h->key_in, &h->key_out, &h->val_in,
+                    &h->val_out, &h->hist, &h->bin_start, &h->sort_tmp, &h->s_cos, &h->s_sin,
+                    &h->s_lonr, &h->s_row0, &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai,
+                    &h->d_hybi, &h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc,
+                    &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
+                    &h->x_xa, &h->x_axa};
+  for (DevBuf* b : bufs) b->release();
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+  delete h;
+  return DIDA_OK;
+}
+
+int dida_letkf_set_stream(dida_letkf_handle* h, void* s) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  h->stream = s ? (hipStream_t)s : h->own_stream;
+  return DIDA_OK;
+}
+
+int dida_letkf_set_grid(dida_letkf_handle* h, const float* lat, const float* lon) {
+  if (!h || !lat || !lon) return set_error(DIDA_ERR_ARG, "null argument");
+  const auto& c = h->cfg;
+  RC_TRY(h->glat.ensure(sizeof(float) * c.nlat));
+  RC_TRY(h->glon.ensure(sizeof(float) * c.nlon));
+  HIP_TRY(hipMemcpyAsync(h->glat.p, lat, sizeof(float) * c.nlat, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->glon.p, lon, sizeof(float) * c.nlon, hipMemcpyHostToDevice, h->stream));
+  if (!h->hyai.empty()) {
+    RC_TRY(h->d_hyai.ensure(sizeof(double) * h->hyai.size()));
+    RC_TRY(h->d_hybi.ensure(sizeof(double) * h->hybi.size()));
+    HIP_TRY(hipMemcpyAsync(h->d_hyai.p, h->hyai.data(), sizeof(double) * h->hyai.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_hybi.p, h->hybi.data(), sizeof(double) * h->hybi.size(),
+                           hipMemcpyHostToDevice, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->have_grid = true;
+  return DIDA_OK;
+}
+
+// Builds the obs index from device arrays (raw layout as the C ABI).
+static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
+                           const double* lon, const int* row_ptr, int n_rows, const double* yb,
+                           const double* dep, const double* rinv, const double* vco) {
+  const auto& c = h->cfg;
+  hipStream_t st = h->stream;
+  h->n_sites = n_sites;
+  h->n_rows = n_rows;
+  const int nbins = h->nb_lat * h->nb_lon;
+  RC_TRY(h->key_in.ensure(sizeof(int) * std::max(n_sites, 1)));
+  RC_TRY(h->key_out.ensure(sizeof(int) * std::max(n_sites, 1)));
+  RC_TRY(h->val_in.ensure(sizeof(int) * std::max(n_sites, 1)));
+  RC_TRY(h->val_out.ensure(sizeof(int) * std::max(n_sites, 1)));
+  RC_TRY(h->hist.ensure(sizeof(int) * (nbins + 1)));
+  RC_TRY(h->bin_start.ensure(sizeof(int) * (nbins + 1)));
+  HIP_TRY(hipMemsetAsync(h->hist.p, 0, sizeof(int) * (nbins + 1), st));
+  HIP_TRY(launch_site_keys(st, n_sites, lat, lon, c.obs_polar, h->nb_lat, h->nb_lon, h->bdeg_lat,
+                           h->bdeg_lon, h->key_in.as<int>(), h->val_in.as<int>(),
+                           h->hist.as<int>()));
+  // stable sort of site indices by bin key (ties keep input order)
+  int end_bit = 1;
+  while ((1 << end_bit) <= nbins) ++end_bit;
+  size_t tmp_bytes = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->key_in.as<int>(),
+                                             h->key_out.as<int>(), h->val_in.as<int>(),
+                                             h->val_out.as<int>(), n_sites, 0, end_bit, st));
+  size_t scan_bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->hist.as<int>(),
+                                           h->bin_start.as<int>(), nbins + 1, st));
+  RC_TRY(h->sort_tmp.ensure(std::max(tmp_bytes, scan_bytes)));
+  if (n_sites > 0)
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(h->sort_tmp.p, tmp_bytes, h->key_in.as<int>(),
+                                               h->key_out.as<int>(), h->val_in.as<int>(),
+                                               h->val_out.as<int>(), n_sites, 0, end_bit, st));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(h->sort_tmp.p, scan_bytes, h->hist.as<int>(),
+                                           h->bin_start.as<int>(), nbins + 1, st));
+  const size_t ns = std::max(n_sites, 1);
+  RC_TRY(h->s_cos.ensure(sizeof(double) * ns));
+  RC_TRY(h->s_sin.ensure(sizeof(double) * ns));
+  RC_TRY(h->s_lonr.ensure(sizeof(double) * ns));
+  RC_TRY(h->s_row0.ensure(sizeof(int) * ns));
+  RC_TRY(h->s_nrow.ensure(sizeof(int) * ns));
+  HIP_TRY(launch_site_gather(st, n_sites, h->val_out.as<int>(), lat, lon, row_ptr,
+                             h->s_cos.as<double>(), h->s_sin.as<double>(), h->s_lonr.as<double>(),
+                             h->s_row0.as<int>(), h->s_nrow.as<int>()));
+  // rows in the storage precision of the mode
+  const long long nr = std::max(n_rows, 1);
+  RC_TRY(h->rinv32.ensure(sizeof(float) * nr));
+  if (n_rows > 0)
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n_rows + 255) / 256)), dim3(256), 0, st,
+                       (long long)n_rows, rinv, h->rinv32.as<float>());
+  if (c.precision != DIDA_PREC_FP64) {
+    const long long ny = (long long)n_rows * c.k;
+    RC_TRY(h->yb32.ensure(sizeof(float) * std::max(ny, 1LL)));
+    if (ny > 0)
+      hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, st, ny,
+                         yb, h->yb32.as<float>());
+  }
+  HIP_TRY(hipGetLastError());
+  h->have_obs = true;
+  return DIDA_OK;
+}
+
+static int stage_obs(dida_letkf_handle* h, int n_sites, const double* lat, const double* lon,
+                     const int* row_ptr, int n_rows, const double* yb, const double* dep,
+                     const double* rinv, const double* vco, hipMemcpyKind kind) {
+  const auto& c = h->cfg;
+  hipStream_t st = h->stream;
+  const size_t ns = std::max(n_sites, 1), nr = std::max(n_rows, 1);
+  RC_TRY(h->o_lat.ensure(sizeof(double) * ns));
+  RC_TRY(h->o_lon.ensure(sizeof(double) * ns));
+  RC_TRY(h->o_yb64.ensure(sizeof(double) * nr * c.k));
+  RC_TRY(h->o_dep.ensure(sizeof(double) * nr));
+  RC_TRY(h->o_rinv64.ensure(sizeof(double) * nr));
+  HIP_TRY(hipMemcpyAsync(h->o_lat.p, lat, sizeof(double) * n_sites, kind, st));
+  HIP_TRY(hipMemcpyAsync(h->o_lon.p, lon, sizeof(double) * n_sites, kind, st));
+  HIP_TRY(hipMemcpyAsync(h->o_yb64.p, yb, sizeof(double) * (size_t)n_rows * c.k, kind, st));
+  HIP_TRY(hipMemcpyAsync(h->o_dep.p, dep, sizeof(double) * n_rows, kind, st));
+  HIP_TRY(hipMemcpyAsync(h->o_rinv64.p, rinv, sizeof(double) * n_rows, kind, st));
+  const int* d_rowptr = nullptr;
+  if (row_ptr) {
+    RC_TRY(h->o_rowptr.ensure(sizeof(int) * (ns + 1)));
+    HIP_TRY(hipMemcpyAsync(h->o_rowptr.p, row_ptr, sizeof(int) * (n_sites + 1), kind, st));
+    d_rowptr = h->o_rowptr.as<int>();
+  }
+  if (vco) {
+    RC_TRY(h->o_vco.ensure(sizeof(double) * nr));
+    HIP_TRY(hipMemcpyAsync(h->o_vco.p, vco, sizeof(double) * n_rows, kind, st));
+  } else if (c.vert_dist > 0.0) {
+    return set_error(DIDA_ERR_ARG, "vertical localisation needs vcoord");
+  }
+  return build_obs_index(h, n_sites, h->o_lat.as<double>(), h->o_lon.as<double>(), d_rowptr,
+                         n_rows, h->o_yb64.as<double>(), h->o_dep.as<double>(),
+                         h->o_rinv64.as<double>(), vco ? h->o_vco.as<double>() : nullptr);
+}
+
+static int check_obs_args(dida_letkf_handle* h, int n_sites, const double* lat, const double* lon,
+                          const int* row_ptr, int n_rows, const double* yb, const double* dep,
+                          const double* rinv, bool host) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  if (n_sites < 0 || n_rows < 0) return set_error(DIDA_ERR_ARG, "negative sizes");
+  if (n_sites > 0 && (!lat || !lon)) return set_error(DIDA_ERR_ARG, "null site arrays");
+  if (n_rows > 0 && (!yb || !dep || !rinv)) return set_error(DIDA_ERR_ARG, "null row arrays");
+  if (!row_ptr && n_rows != n_sites)
+    return set_error(DIDA_ERR_ARG, "one row per site needs n_rows == n_sites");
+  if (row_ptr && host) {
+    if (row_ptr[0] != 0 || row_ptr[n_sites] != n_rows)
+      return set_error(DIDA_ERR_ARG, "site_row_ptr must run from 0 to n_rows");
+    for (int s = 0; s < n_sites; ++s)
+      if (row_ptr[s + 1] < row_ptr[s]) return set_error(DIDA_ERR_ARG, "site_row_ptr decreasing");
+  }
+  return DIDA_OK;
+}
+
+int dida_letkf_set_obs(dida_letkf_handle* h, int n_sites, const double* lat, const double* lon,
+                       const int* row_ptr, int n_rows, const double* yb, const double* dep,
+                       const double* rinv, const double* vco) {
+  RC_TRY(check_obs_args(h, n_sites, lat, lon, row_ptr, n_rows, yb, dep, rinv, true));
+  RC_TRY(stage_obs(h, n_sites, lat, lon, row_ptr, n_rows, yb, dep, rinv, vco,
+                   hipMemcpyHostToDevice));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DIDA_OK;
+}
+
+int dida_letkf_set_obs_device(dida_letkf_handle* h, int n_sites, const double* lat,
+                              const double* lon, const int* row_ptr, int n_rows,
+                              const double* yb, const double* dep, const double* rinv,
+                              const double* vco) {
+  RC_TRY(check_obs_args(h, n_sites, lat, lon, row_ptr, n_rows, yb, dep, rinv, false));
+  return stage_obs(h, n_sites, lat, lon, row_ptr, n_rows, yb, dep, rinv, vco,
+                   hipMemcpyDeviceToDevice);
+}
+
+}  // extern "C"
+
+// Shared analysis driver.  xb etc. are device pointers.
+static int run_analysis(dida_letkf_handle* h, const double* xb, const double* axb, double* xa,
+                        double* axa, dida_letkf_stats* stats) {
+  const auto& c = h->cfg;
+  hipStream_t st = h->stream;
+  if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
+  if (!h->have_obs) return set_error(DIDA_ERR_STATE, "set_obs was not called");
+  hipEvent_t ev[8];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  float ms_sel = 0, ms_prod = 0, ms_xf = 0;
+  HIP_TRY(hipEventRecord(ev[0], st));
+
+  Geom g;
+  g.glat = h->glat.as<float>();
+  g.glon = h->glon.as<float>();
+  g.nlat = c.nlat;
+  g.nlon = c.nlon;
+  g.cube_n = c.cube_n;
+  g.nlat_c = h->nlat_c;
+  g.nlon_c = h->nlon_c;
+
+  const long long mb = h->max_batch;
+  RC_TRY(h->cnt.ensure(sizeof(long long) * (mb + 1)));
+  RC_TRY(h->off.ensure(sizeof(long long) * (mb + 1)));
+  size_t scan_bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->cnt.as<long long>(),
+                                           h->off.as<long long>(), (int)(mb + 1), st));
+  RC_TRY(h->scan_tmp.ensure(scan_bytes));
+  RC_TRY(h->Abuf.ensure(sizeof(double) * mb * h->nzc * h->kp * h->kp));
+  RC_TRY(h->bbuf.ensure(sizeof(double) * mb * h->nzc * h->kp));
+  RC_TRY(h->status.ensure(sizeof(int) * h->n_an));
+  if (c.keep_transforms) {
+    RC_TRY(h->trans.ensure(sizeof(double) * h->n_an * c.k * c.k));
+    RC_TRY(h->wout.ensure(sizeof(double) * h->n_an * c.k));
+    h->nobsl_host.assign(h->n_an, 0);
+  }
+  long long total_rows = 0;
+  int max_rows = 0, over_cap = 0;
+  std::vector<long long> cnt_host;
+
+  for (long long h0 = 0; h0 < h->n_h; h0 += mb) {
+    const long long h1 = std::min(h->n_h, h0 + mb);
+    const int ncol = (int)(h1 - h0);
+    SelectParams sp;
+    sp.g = g;
+    sp.s_cos = h->s_cos.as<double>();
+    sp.s_sin = h->s_sin.as<double>();
+    sp.s_lonr = h->s_lonr.as<double>();
+    sp.s_row0 = h->s_row0.as<int>();
+    sp.s_nrow = h->s_nrow.as<int>();
+    sp.bin_start = h->bin_start.as<int>();
+    sp.nb_lat = h->nb_lat;
+    sp.nb_lon = h->nb_lon;
+    sp.bdeg_lat = h->bdeg_lat;
+    sp.bdeg_lon = h->bdeg_lon;
+    sp.theta_deg = h->theta_deg;
+    sp.local_dist = c.local_dist;
+    sp.gc_filter = c.gc_filter;
+    sp.h0 = (int)h0;
+    sp.h1 = (int)h1;
+    sp.cnt = h->cnt.as<long long>();
+    sp.off = h->off.as<long long>();
+    sp.row_idx = nullptr;
+    sp.row_gc = nullptr;
+    sp.cap = 0;
+    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, sizeof(long long) * (ncol + 1), st));
+    HIP_TRY(launch_select(st, sp, false));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(h->scan_tmp.p, scan_bytes, h->cnt.as<long long>(),
+                                             h->off.as<long long>(), ncol + 1, st));
+    cnt_host.resize(ncol + 1);
+    HIP_TRY(hipMemcpyAsync(cnt_host.data(), h->off.p, sizeof(long long) * (ncol + 1),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const long long rows = cnt_host[ncol];
+    for (int i = 0; i < ncol; ++i) {
+      long long r = cnt_host[i + 1] - cnt_host[i];
+      max_rows = std::max<long long>(max_rows, r);
+      if (r > 10000) ++over_cap;
+      if (c.keep_transforms)
+        for (int kc = 0; kc < h->nzc; ++kc) h->nobsl_host[(h0 + i) * h->nzc + kc] = (int)r;
+    }
+    total_rows += rows * h->nzc;
+    RC_TRY(h->row_idx.ensure(sizeof(int) * std::max(rows, 1LL)));
+    RC_TRY(h->row_gc.ensure(sizeof(double) * std::max(rows, 1LL)));
+    sp.row_idx = h->row_idx.as<int>();
+    sp.row_gc = h->row_gc.as<double>();
+    sp.cap = rows;
+    HIP_TRY(launch_select(st, sp, true));
+    HIP_TRY(hipEventRecord(ev[1], st));
+
+    ProdParams pp;
+    pp.g = g;
+    pp.k = c.k;
+    pp.kp = h->kp;
+    pp.nzc = h->nzc;
+    pp.vertical = c.vert_dist > 0.0;
+    pp.vcoord_mode = c.vcoord_mode;
+    pp.cube_nz = h->cfg.cube_nz;
+    pp.nz = c.nz;
+    pp.halo = c.halo;
+    pp.n = h->n;
+    pp.ps_index = c.ps_index;
+    pp.vert_dist = c.vert_dist;
+    pp.gc_filter_vertical = c.gc_filter_vertical;
+    pp.hyai = h->d_hyai.as<double>();
+    pp.hybi = h->d_hybi.as<double>();
+    pp.p0 = c.p0;
+    pp.ptop = c.ptop;
+    pp.axb = axb;
+    pp.row_idx = h->row_idx.as<int>();
+    pp.row_gc = h->row_gc.as<double>();
+    pp.row_off = h->off.as<long long>();
+    pp.yb64 = h->o_yb64.as<double>();
+    pp.yb32 = h->yb32.as<float>();
+    pp.dep = h->o_dep.as<double>();
+    pp.rinv32 = h->rinv32.as<float>();
+    pp.vco = h->o_vco.as<double>();
+    pp.a0 = h0 * h->nzc;
+    pp.n_an = (h1 - h0) * h->nzc;
+    pp.h0 = (int)h0;
+    pp.A = h->Abuf.as<double>();
+    pp.b = h->bbuf.as<double>();
+    HIP_TRY(launch_products(st, pp, c.precision));
+    HIP_TRY(hipEventRecord(ev[2], st));
+
+    XformParams xp;
+    xp.g = g;
+    xp.k = c.k;
+    xp.kp = h->kp;
+    xp.inflation = c.inflation_factor;
+    xp.guard = c.precision != DIDA_PREC_FP64;
+    xp.A = pp.A;
+    xp.b = pp.b;
+    xp.a0 = pp.a0;
+    xp.n_an = pp.n_an;
+    xp.nzc = h->nzc;
+    xp.vertical = pp.vertical;
+    xp.cube_nz = h->cfg.cube_nz;
+    xp.nz = c.nz;
+    xp.halo = c.halo;
+    xp.n = h->n;
+    xp.n_var = c.n2d + c.n3d;
+    {
+      int cur = 1;
+      for (int v = 0; v < c.n2d; ++v) { xp.var_start[v] = cur; xp.var_lev[v] = 1; cur += 1; }
+      for (int v = 0; v < c.n3d; ++v) {
+        xp.var_start[c.n2d + v] = cur;
+        xp.var_lev[c.n2d + v] = c.nz;
+        cur += c.nz;
+      }
+    }
+    xp.xb = xb;
+    xp.axb = axb;
+    xp.xa = xa;
+    xp.axa = axa;
+    xp.trans_out = c.keep_transforms ? h->trans.as<double>() : nullptr;
+    xp.w_out = c.keep_transforms ? h->wout.as<double>() : nullptr;
+    xp.status = h->status.as<int>();
+    HIP_TRY(launch_transform(st, xp));
+    HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(hipEventSynchronize(ev[3]));
+    float t1 = 0, t2 = 0, t3 = 0;
+    HIP_TRY(hipEventElapsedTime(&t1, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t2, ev[1], ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t3, ev[2], ev[3]));
+    ms_sel += t1;
+    ms_prod += t2;
+    ms_xf += t3;
+    HIP_TRY(hipEventRecord(ev[0], st));
+  }
+  // eigensolver flags
+  std::vector<int> stt(h->n_an);
+  HIP_TRY(hipMemcpyAsync(stt.data(), h->status.p, sizeof(int) * h->n_an, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int flagged = 0;
+  for (int s : stt) flagged += (s != 0);
+  if (stats) {
+    stats->n_analyses = h->n_an;
+    stats->total_rows = total_rows;
+    stats->max_rows = max_rows;
+    stats->n_flagged = flagged;
+    stats->rows_over_10000 = over_cap;
+    stats->ms_select = ms_sel;
+    stats->ms_products = ms_prod;
+    stats->ms_transform = ms_xf;
+    stats->ms_total = ms_sel + ms_prod + ms_xf;
+  }
+  return flagged;
+}
+
+extern "C" {
+
+int dida_letkf_analyze_device(dida_letkf_handle* h, const double* xb, const double* axb,
+                              double* xa, double* axa, dida_letkf_stats* stats) {
+  if (!h || !xb || !axb || !xa || !axa) return set_error(DIDA_ERR_ARG, "null argument");
+  return run_analysis(h, xb, axb, xa, axa, stats);
+}
+
+int dida_letkf_analyze(dida_letkf_handle* h, const double* xb, const double* axb, double* xa,
+                       double* axa, dida_letkf_stats* stats) {
+  if (!h || !xb || !axb || !xa || !axa) return set_error(DIDA_ERR_ARG, "null argument");
+  const auto& c = h->cfg;
+  const size_t ncol = (size_t)(c.nlat + 2 * c.halo) * (c.nlon + 2 * c.halo) * h->n;
+  const size_t nx = ncol * c.k;
+  hipStream_t st = h->stream;
+  RC_TRY(h->x_xb.ensure(nx * 8));
+  RC_TRY(h->x_axb.ensure(ncol * 8));
+  RC_TRY(h->x_xa.ensure(nx * 8));
+  RC_TRY(h->x_axa.ensure(ncol * 8));
+  HIP_TRY(hipMemcpyAsync(h->x_xb.p, xb, nx * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(h->x_axb.p, axb, ncol * 8, hipMemcpyHostToDevice, st));
+  // outputs start as the caller's arrays so that halo entries are left as they are
+  HIP_TRY(hipMemcpyAsync(h->x_xa.p, xa, nx * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(h->x_axa.p, axa, ncol * 8, hipMemcpyHostToDevice, st));
+  int rc = run_analysis(h, h->x_xb.as<double>(), h->x_axb.as<double>(), h->x_xa.as<double>(),
+                        h->x_axa.as<double>(), stats);
+  if (rc < 0) return rc;
+  HIP_TRY(hipMemcpyAsync(xa, h->x_xa.p, nx * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(axa, h->x_axa.p, ncol * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return rc;
+}
+
+int dida_letkf_get_transforms(dida_letkf_handle* h, long long n_an, double* trans,
+                              double* overline_w, int* nobsl) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  if (!h->cfg.keep_transforms) return set_error(DIDA_ERR_STATE, "keep_transforms is off");
+  if (n_an != h->n_an) return set_error(DIDA_ERR_ARG, "n_analyses mismatch (%lld)", h->n_an);
+  const int k = h->cfg.k;
+  if (trans)
+    HIP_TRY(hipMemcpyAsync(trans, h->trans.p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost,
+                           h->stream));
+  if (overline_w)
+    HIP_TRY(hipMemcpyAsync(overline_w, h->wout.p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (nobsl) std::memcpy(nobsl, h->nobsl_host.data(), sizeof(int) * n_an);
+  return DIDA_OK;
+}
+
+int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, const double* hxens,
+                            const float* rdiaginv, const double* dep, const double* rloc,
+                            double* trans, double* overline_w, double inflation, int precision,
+                            int* status) {
+  if (k < 2 || k > 64) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: supported 2..64", k);
+  if (n_an < 0 || !row_ptr || !trans || !overline_w)
+    return set_error(DIDA_ERR_ARG, "bad arguments");
+  if (precision < DIDA_PREC_FP64 || precision > DIDA_PREC_BF16X3)
+    return set_error(DIDA_ERR_ARG, "bad precision");
+  if (!(inflation > 0.0)) return set_error(DIDA_ERR_ARG, "inflation_factor must be > 0");
+  if (n_an == 0) return DIDA_OK;
+  if (row_ptr[0] != 0) return set_error(DIDA_ERR_ARG, "row_ptr[0] must be 0");
+  for (long long a = 0; a < n_an; ++a)
+    if (row_ptr[a + 1] < row_ptr[a]) return set_error(DIDA_ERR_ARG, "row_ptr decreasing");
+  const long long nr = row_ptr[n_an];
+  if (nr > 0 && (!hxens || !rdiaginv || !dep || !rloc))
+    return set_error(DIDA_ERR_ARG, "null input arrays");
+  const int kp = kp_of(k);
+  hipStream_t st;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct S {
+    hipStream_t s;
+    std::vector<DevBuf*> b;
+    ~S() {
+      (void)hipStreamSynchronize(s);
+      for (auto* x : b) { x->release(); delete x; }
+      (void)hipStreamDestroy(s);
+    }
+  } scope{st, {}};
+  auto mk = [&](size_t bytes, DevBuf** out) -> int {
+    DevBuf* b = new DevBuf();
+    scope.b.push_back(b);
+    *out = b;
+    return b->ensure(bytes);
+  };
+  const long long nrr = std::max(nr, 1LL);
+  DevBuf *d_y, *d_r, *d_d, *d_g, *d_idx, *d_off, *d_A, *d_b, *d_t, *d_w, *d_s, *d_y32 = nullptr;
+  RC_TRY(mk(sizeof(double) * nrr * k, &d_y));
+  RC_TRY(mk(sizeof(float) * nrr, &d_r));
+  RC_TRY(mk(sizeof(double) * nrr, &d_d));
+  RC_TRY(mk(sizeof(double) * nrr, &d_g));
+  RC_TRY(mk(sizeof(int) * nrr, &d_idx));
+  RC_TRY(mk(sizeof(long long) * (n_an + 1), &d_off));
+  RC_TRY(mk(sizeof(double) * n_an * kp * kp, &d_A));
+  RC_TRY(mk(sizeof(double) * n_an * kp, &d_b));
+  RC_TRY(mk(sizeof(double) * n_an * k * k, &d_t));
+  RC_TRY(mk(sizeof(double) * n_an * k, &d_w));
+  RC_TRY(mk(sizeof(int) * n_an, &d_s));
+  if (nr > 0) {
+    HIP_TRY(hipMemcpyAsync(d_y->p, hxens, sizeof(double) * nr * k, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_r->p, rdiaginv, sizeof(float) * nr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_d->p, dep, sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_g->p, rloc, sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, nr,
+                       d_idx->as<int>());
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(d_off->p, row_ptr, sizeof(long long) * (n_an + 1), hipMemcpyHostToDevice, st));
+  if (precision != DIDA_PREC_FP64) {
+    RC_TRY(mk(sizeof(float) * nrr * k, &d_y32));
+    if (nr > 0) {
+      hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((nr * k + 255) / 256)), dim3(256), 0, st,
+                         nr * k, d_y->as<double>(), d_y32->as<float>());
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  Geom g{};
+  g.nlat = 1; g.nlon = 1; g.cube_n = 1; g.nlat_c = 1; g.nlon_c = 1;
+  ProdParams pp{};
+  pp.g = g;
+  pp.k = k;
+  pp.kp = kp;
+  pp.nzc = 1;
+  pp.vertical = 0;
+  pp.row_idx = d_idx->as<int>();
+  pp.row_gc = d_g->as<double>();
+  pp.row_off = d_off->as<long long>();
+  pp.yb64 = d_y->as<double>();
+  pp.yb32 = d_y32 ? d_y32->as<float>() : nullptr;
+  pp.dep = d_d->as<double>();
+  pp.rinv32 = d_r->as<float>();
+  pp.a0 = 0;
+  pp.n_an = n_an;
+  pp.h0 = 0;
+  pp.A = d_A->as<double>();
+  pp.b = d_b->as<double>();
+  HIP_TRY(launch_products(st, pp, precision));
+  XformParams xp{};
+  xp.g = g;
+  xp.k = k;
+  xp.kp = kp;
+  xp.inflation = inflation;
+  xp.guard = precision != DIDA_PREC_FP64;
+  xp.A = pp.A;
+  xp.b = pp.b;
+  xp.a0 = 0;
+  xp.n_an = n_an;
+  xp.nzc = 1;
+  xp.xb = nullptr;
+  xp.trans_out = d_t->as<double>();
+  xp.w_out = d_w->as<double>();
+  xp.status = d_s->as<int>();
+  HIP_TRY(launch_transform(st, xp));
+  HIP_TRY(hipMemcpyAsync(trans, d_t->p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(overline_w, d_w->p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost, st));
+  std::vector<int> sh(n_an);
+  HIP_TRY(hipMemcpyAsync(sh.data(), d_s->p, sizeof(int) * n_an, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int flagged = 0;
+  for (long long a = 0; a < n_an; ++a) {
+    flagged += sh[a] != 0;
+    if (status) status[a] = sh[a];
+  }
+  return flagged;
+}
+
+void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,
+                 const float* rdiaginv, const double* dep, const double* rloc, double* trans,
+                 double* overline_w, const double* inflation_factor, const int* pflag) {
+  (void)pflag;
+  long long rp[2] = {0, (long long)*nobsl};
+  int st = 0;
+  int rc = dida_letkf_core_batched(*nanals, 1, rp, hxens_in, rdiaginv, dep, rloc, trans,
+                                   overline_w, *inflation_factor, DIDA_PREC_FP64, &st);
+  if (rc != 0) fprintf(stderr, "letkf_core_: rc=%d %s\n", rc, g_last_error.c_str());
+}
+
+int dida_letkf_epilogue_device(dida_letkf_handle* h, const double* xb, const double* axb,
+                               const double* xa, const double* axa, double polar_lat,
+                               double gamma1, double gamma2, double* xa_out, double* spread_b,
+                               double* spread_a) {
+  if (!h || !xb || !axb || !xa || !axa || !xa_out || !spread_b || !spread_a)
+    return set_error(DIDA_ERR_ARG, "null argument");
+  if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
+  EpiParams p;
+  p.k = h->cfg.k;
+  p.n = h->n;
+  p.nlat = h->cfg.nlat;
+  p.nlon = h->cfg.nlon;
+  p.halo = h->cfg.halo;
+  p.glat = h->glat.as<float>();
+  p.xb = xb;
+  p.axb = axb;
+  p.xa = xa;
+  p.axa = axa;
+  p.polar_lat = polar_lat;
+  p.gamma1 = gamma1;
+  p.gamma2 = gamma2;
+  p.xa_out = xa_out;
+  p.spread_b = spread_b;
+  p.spread_a = spread_a;
+  HIP_TRY(launch_epilogue(h->stream, p));
+  return DIDA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// letkf_kernels.h -- kernel parameter blocks and launchers (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dida {
+
+constexpr int kMaxVars = 16;
+
+struct Geom {
+  const float* glat;   // nlat, real(4) degrees
+  const float* glon;   // nlon
+  int nlat, nlon;
+  int cube_n;
+  int nlat_c, nlon_c;  // cubes per direction
+};
+
+struct SelectParams {
+  Geom g;
+  // sorted site index
+  const double* s_cos;
+  const double* s_sin;
+  const double* s_lonr;
+  const int* s_row0;
+  const int* s_nrow;
+  const int* bin_start;   // nb_lat*nb_lon + 1
+  int nb_lat, nb_lon;
+  double bdeg_lat, bdeg_lon;
+  double theta_deg;       // support half-angle incl. margin, degrees
+  double local_dist;
+  int gc_filter;
+  int h0, h1;             // horizontal analyses [h0, h1)
+  long long* cnt;         // count pass: rows per column (batch-local)
+  const long long* off;   // fill pass: exclusive offsets (batch-local)
+  int* row_idx;
+  double* row_gc;
+  long long cap;          // fill pass: capacity of row_idx / row_gc
+};
+
+struct ProdParams {
+  Geom g;
+  int k, kp;
+  int nzc, vertical, vcoord_mode, cube_nz, nz, halo, n, ps_index;
+  double vert_dist;
+  int gc_filter_vertical;
+  const double* hyai;
+  const double* hybi;
+  double p0, ptop;
+  const double* axb;
+  const int* row_idx;
+  const double* row_gc;
+  const long long* row_off;  // per batch column, n_cols + 1 entries
+  const double* yb64;
+  const float* yb32;
+  const double* dep;
+  const float* rinv32;
+  const double* vco;
+  long long a0, n_an;        // analyses [a0, a0 + n_an)
+  int h0;
+  double* A;                 // batch-local, kp*kp per analysis
+  double* b;                 // batch-local, kp per analysis
+};
+
+struct XformParams {
+  Geom g;
+  int k, kp;
+  double inflation;
+  int guard;
+  const double* A;
+  const double* b;
+  long long a0, n_an;
+  int nzc, vertical, cube_nz, nz, halo, n, n_var;
+  int var_start[kMaxVars];
+  int var_lev[kMaxVars];
+  const double* xb;          // NULL: transform only
+  const double* axb;
+  double* xa;
+  double* axa;
+  double* trans_out;         // optional, k*k per analysis (global index)
+  double* w_out;             // optional, k per analysis
+  int* status;               // optional, per analysis
+};
+
+struct EpiParams {
+  int k, n, nlat, nlon, halo;
+  const float* glat;
+  const double* xb;
+  const double* axb;
+  const double* xa;
+  const double* axa;
+  double polar_lat, gamma1, gamma2;
+  double* xa_out;
+  double* spread_b;
+  double* spread_a;
+};
+
+hipError_t launch_site_keys(hipStream_t st, int n_sites, const double* lat, const double* lon,
+                            double obs_polar, int nb_lat, int nb_lon, double bdeg_lat,
+                            double bdeg_lon, int* key, int* val, int* hist);
+hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, const double* lat,
+                              const double* lon, const int* row_ptr, double* s_cos, double* s_sin,
+                              double* s_lonr, int* s_row0, int* s_nrow);
+hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
+hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
+hipError_t launch_transform(hipStream_t st, const XformParams& p);
+hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
+
+}  // namespace dida

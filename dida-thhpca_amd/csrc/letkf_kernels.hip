@@ -1,0 +1,854 @@
+// letkf_kernels.hip -- gfx950 kernels of the LETKF analysis step.
+//
+//   select      local obs selection (gen_yb_dis, letkf_yb.F90:3-150) over a
+//               lat-lon bin index of the obs sites, one wavefront per column
+//   products    obs-space products A = Yb^T R^-1 rho Yb and
+//               b = Yb^T R^-1 rho d (letkf_core.F90:58-79,146-195) on MFMA
+//               (f64 16x16x4, f32 16x16x4, bf16 16x16x32 hi/lo split)
+//   transform   FP64 eigensolve of A, Pa, T = sqrt((k-1) Pa), overline_w and
+//               trans = T + overline_w 1^T (letkf_core.F90:97-225), fused with
+//               the ensemble update xb_2_xa (letkf_compute.F90:259-329) or the
+//               per-level update of the vertical loop (letkf_compute.F90:501-552)
+//
+// MFMA result layouts on gfx950 (checked on the device):
+//   f64 16x16x4 : lane l, reg r holds D[l/16 + 4r][l%16]
+//   f32 16x16x4, bf16 16x16x32 : lane l, reg r holds D[4(l/16) + r][l%16]
+// Operands: A[i][k] at lane i + 16*(k/K0), B[k][j] at lane j + 16*(k/K0),
+// with K0 = 1 (x4 shapes) or 8 consecutive k per lane (16x16x32 bf16).
+#include "letkf_kernels.h"
+#include "letkf_device.h"
+
+namespace dida {
+
+// ---------------------------------------------------------------------------
+// obs site index
+// ---------------------------------------------------------------------------
+
+__global__ void k_site_keys(int n_sites, const double* __restrict__ lat,
+                            const double* __restrict__ lon, double obs_polar, int nb_lat,
+                            int nb_lon, double bdeg_lat, double bdeg_lon, int* __restrict__ key,
+                            int* __restrict__ val, int* __restrict__ hist) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sites) return;
+  double la = lat[s];
+  int kk = nb_lat * nb_lon;  // sentinel: dropped (letkf_yb.F90:87)
+  if (fabs(la) <= obs_polar) {
+    double lo = fmod(lon[s], 360.0);
+    if (lo < 0.0) lo += 360.0;
+    int b = (int)floor((la + 90.0) / bdeg_lat);
+    b = b < 0 ? 0 : (b >= nb_lat ? nb_lat - 1 : b);
+    int j = (int)floor(lo / bdeg_lon);
+    j = j < 0 ? 0 : (j >= nb_lon ? nb_lon - 1 : j);
+    kk = b * nb_lon + j;
+    atomicAdd(&hist[kk], 1);
+  }
+  key[s] = kk;
+  val[s] = s;
+}
+
+__global__ void k_site_gather(int n_sites, const int* __restrict__ order,
+                              const double* __restrict__ lat, const double* __restrict__ lon,
+                              const int* __restrict__ row_ptr, double* __restrict__ s_cos,
+                              double* __restrict__ s_sin, double* __restrict__ s_lonr,
+                              int* __restrict__ s_row0, int* __restrict__ s_nrow) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_sites) return;
+  int s = order[i];
+  double xr = deg2rad_ref(lat[s]);
+  s_cos[i] = cos(xr);
+  s_sin[i] = sin(xr);
+  s_lonr[i] = deg2rad_ref(lon[s]);
+  int r0 = row_ptr ? row_ptr[s] : s;
+  int r1 = row_ptr ? row_ptr[s + 1] : s + 1;
+  s_row0[i] = r0;
+  s_nrow[i] = r1 - r0;
+}
+
+// ---------------------------------------------------------------------------
+// local obs selection
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void column_center(const Geom& g, int ah, int* sj, int* ej, int* si,
+                                              int* ei) {
+  const int ic = ah % g.nlat_c, jc = ah / g.nlat_c;
+  *sj = ic * g.cube_n + 1;
+  *ej = min(*sj + g.cube_n - 1, g.nlat);
+  *si = jc * g.cube_n + 1;
+  *ei = min(*si + g.cube_n - 1, g.nlon);
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_select(SelectParams p) {
+  const int lane = threadIdx.x & 63;
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int ah = p.h0 + wid;
+  if (ah >= p.h1) return;
+  int sj, ej, si, ei;
+  column_center(p.g, ah, &sj, &ej, &si, &ei);
+  const double clat = (double)p.g.glat[(sj + ej) / 2 - 1];
+  const double clon = (double)p.g.glon[(si + ei) / 2 - 1];
+  const double c2 = cos(deg2rad_ref(clat)), s2 = sin(deg2rad_ref(clat));
+  const double l2 = deg2rad_ref(clon);
+  const double th = p.theta_deg;
+  int b0 = (int)floor((clat - th + 90.0) / p.bdeg_lat);
+  int b1 = (int)floor((clat + th + 90.0) / p.bdeg_lat);
+  b0 = b0 < 0 ? 0 : b0;
+  b1 = b1 >= p.nb_lat ? p.nb_lat - 1 : b1;
+  bool full = fabs(clat) + th >= 89.99;
+  double dlam = 360.0;
+  if (!full) {
+    double sr = sin(th * (M_PI / 180.0)) / cos(fabs(clat) * (M_PI / 180.0));
+    if (sr >= 1.0) full = true;
+    else dlam = asin(sr) * (180.0 / M_PI) + 1e-3;
+  }
+  double clw = fmod(clon, 360.0);
+  if (clw < 0.0) clw += 360.0;
+  int ja[2], jb[2], nr = 1;
+  ja[0] = 0;
+  jb[0] = p.nb_lon - 1;
+  if (!full) {
+    int j0 = (int)floor((clw - dlam) / p.bdeg_lon);
+    int j1 = (int)floor((clw + dlam) / p.bdeg_lon);
+    if (j1 - j0 + 1 < p.nb_lon) {
+      if (j0 < 0) {
+        ja[0] = j0 + p.nb_lon; jb[0] = p.nb_lon - 1; ja[1] = 0; jb[1] = j1; nr = 2;
+      } else if (j1 >= p.nb_lon) {
+        ja[0] = j0; jb[0] = p.nb_lon - 1; ja[1] = 0; jb[1] = j1 - p.nb_lon; nr = 2;
+      } else {
+        ja[0] = j0; jb[0] = j1;
+      }
+    }
+  }
+  long long count = 0;
+  long long base = FILL ? p.off[wid] : 0;
+  for (int b = b0; b <= b1; ++b) {
+    for (int r = 0; r < nr; ++r) {
+      const int s0 = p.bin_start[b * p.nb_lon + ja[r]];
+      const int s1 = p.bin_start[b * p.nb_lon + jb[r] + 1];
+      for (int cs = s0; cs < s1; cs += 64) {
+        const int s = cs + lane;
+        double gc = 0.0;
+        int rc = 0;
+        if (s < s1) {
+          double d = latlon_dist_pre(p.s_cos[s], p.s_sin[s], p.s_lonr[s], c2, s2, l2);
+          gc = gc_weight(d * 1.0 / p.local_dist, p.gc_filter);
+          if (gc > 0.0) rc = p.s_nrow[s];
+        }
+        if (!FILL) {
+          count += rc;
+        } else {
+          const int pre = wave_excl_scan(rc, lane);
+          const int tot = __shfl(pre + rc, 63, 64);
+          if (rc) {
+            const long long pos = base + pre;
+            const int r0 = p.s_row0[s];
+            for (int t = 0; t < rc; ++t) {
+              if (pos + t < p.cap) {
+                p.row_idx[pos + t] = r0 + t;
+                p.row_gc[pos + t] = gc;
+              }
+            }
+          }
+          base += tot;
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    count = wave_sum_ll(count);
+    if (lane == 0) p.cnt[wid] = count;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// obs-space products
+// ---------------------------------------------------------------------------
+
+// Column profile of the vertical loop: centre coordinate of chunk kc.
+__device__ __forceinline__ double chunk_vcoord(const ProdParams& p, int ah, int kc) {
+  const int cz = p.cube_nz;
+  const int sk = kc * cz + 1;
+  const int ek = min(sk + cz - 1, p.nz);
+  const int ck = (sk + ek) / 2;
+  if (p.vcoord_mode == 0) return (double)ck;
+  // ps2alnp (obs_make.F90:418-437) with the hybrid p(ps) of
+  // vert_coord_mod.F90:556-563 at the cube centre column
+  int sj, ej, si, ei;
+  column_center(p.g, ah, &sj, &ej, &si, &ei);
+  const int la = (sj + ej) / 2 - 1 + p.halo, lo = (si + ei) / 2 - 1 + p.halo;
+  const long long off = ((long long)lo * (p.g.nlat + 2 * p.halo) + la) * p.n;
+  const double ps = p.axb[off + p.ps_index];
+  const double ph0 = p.hyai[ck - 1] * (p.p0 - p.ptop) + p.hybi[ck - 1] * (ps - p.ptop) + p.ptop;
+  const double ph1 = p.hyai[ck] * (p.p0 - p.ptop) + p.hybi[ck] * (ps - p.ptop) + p.ptop;
+  return log(0.5 * (ph0 + ph1));
+}
+
+// weight of one local obs row: rdiaginv (real(4)) * rloc, times the
+// vertical GC factor (vertical_localization, letkf_yb.F90:240-260)
+__device__ __forceinline__ double row_weight(const ProdParams& p, int row, double gc,
+                                             double vc) {
+#pragma clang fp contract(off)
+  double w = (double)p.rinv32[row] * gc;
+  if (p.vertical) {
+    double ri = fabs(vc - p.vco[row]) / p.vert_dist;
+    w = w * gc_weight(ri, p.gc_filter_vertical);
+  }
+  return w;
+}
+
+template <int T>
+struct PairTable {
+  static constexpr int N = T * (T + 1) / 2;
+  __device__ static constexpr int I(int pid) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= pid) ++i;
+    return i;
+  }
+  __device__ static constexpr int J(int pid) { return pid - I(pid) * (I(pid) + 1) / 2; }
+};
+
+template <int KP, int WS, int PPW>
+__device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long a, int ah,
+                                                  double vc, long long r0, long long r1,
+                                                  int lane) {
+  constexpr int T = KP / 16;
+  using PT = PairTable<T>;
+  constexpr int P0 = WS * PPW;
+  constexpr int P1 = (P0 + PPW < PT::N) ? P0 + PPW : PT::N;
+  constexpr int NP = P1 - P0;
+  const int i16 = lane & 15, q = lane >> 4;
+  d4 acc[NP];
+#pragma unroll
+  for (int x = 0; x < NP; ++x) acc[x] = d4{0.0, 0.0, 0.0, 0.0};
+  double bacc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  for (long long base = r0; base < r1; base += 4) {
+    const long long rr = base + q;
+    double w = 0.0, wd = 0.0;
+    double y[T];
+    if (rr < r1) {
+      const int row = p.row_idx[rr];
+      w = row_weight(p, row, p.row_gc[rr], vc);
+      wd = w * p.dep[row];
+      const double* yr = p.yb64 + (long long)row * p.k;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int m = 16 * t + i16;
+        y[t] = (m < p.k) ? yr[m] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < T; ++t) y[t] = 0.0;
+    }
+#pragma unroll
+    for (int x = 0; x < NP; ++x) {
+      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+      acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[I], w * y[J], acc[x], 0, 0, 0);
+    }
+    if (WS == 0) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) bacc[t] = fma(wd, y[t], bacc[t]);
+    }
+  }
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+#pragma unroll
+  for (int x = 0; x < NP; ++x) {
+    const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * I + q + 4 * r, col = 16 * J + i16;
+      A[(long long)col * KP + row] = acc[x][r];
+    }
+  }
+  if (WS == 0) {
+    double* b = p.b + (a - p.a0) * KP;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      double v = bacc[t];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (q == 0) b[16 * t + i16] = v;
+    }
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
+  constexpr int T = KP / 16;
+  constexpr int NPAIR = T * (T + 1) / 2;
+  constexpr int WPA = (NPAIR + 9) / 10;
+  constexpr int PPW = (NPAIR + WPA - 1) / WPA;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int apb = (blockDim.x >> 6) / WPA;
+  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const int wsub = wave % WPA;
+  if (a >= p.a0 + p.n_an) return;
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
+  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+  if (wsub == 0) products_f64_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+  if constexpr (WPA > 1) {
+    if (wsub == 1) products_f64_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 2) products_f64_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 3) products_f64_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+  }
+}
+
+template <int KP, int WS, int PPW>
+__device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long a, int ah,
+                                                  double vc, long long r0, long long r1,
+                                                  int lane) {
+  constexpr int T = KP / 16;
+  using PT = PairTable<T>;
+  constexpr int P0 = WS * PPW;
+  constexpr int P1 = (P0 + PPW < PT::N) ? P0 + PPW : PT::N;
+  constexpr int NP = P1 - P0;
+  const int i16 = lane & 15, q = lane >> 4;
+  f4 acc[NP];
+#pragma unroll
+  for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
+  double bacc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  for (long long base = r0; base < r1; base += 4) {
+    const long long rr = base + q;
+    float wf = 0.f;
+    double wd = 0.0;
+    float y[T];
+    if (rr < r1) {
+      const int row = p.row_idx[rr];
+      const double w = row_weight(p, row, p.row_gc[rr], vc);
+      wf = (float)w;
+      wd = w * p.dep[row];
+      const float* yr = p.yb32 + (long long)row * p.k;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int m = 16 * t + i16;
+        y[t] = (m < p.k) ? yr[m] : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < T; ++t) y[t] = 0.f;
+    }
+#pragma unroll
+    for (int x = 0; x < NP; ++x) {
+      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x4f32(y[I], wf * y[J], acc[x], 0, 0, 0);
+    }
+    if (WS == 0) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) bacc[t] = fma(wd, (double)y[t], bacc[t]);
+    }
+  }
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+#pragma unroll
+  for (int x = 0; x < NP; ++x) {
+    const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
+      A[(long long)col * KP + row] = (double)acc[x][r];
+    }
+  }
+  if (WS == 0) {
+    double* b = p.b + (a - p.a0) * KP;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      double v = bacc[t];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (q == 0) b[16 * t + i16] = v;
+    }
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
+  constexpr int T = KP / 16;
+  constexpr int NPAIR = T * (T + 1) / 2;
+  constexpr int WPA = (NPAIR + 9) / 10;
+  constexpr int PPW = (NPAIR + WPA - 1) / WPA;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int apb = (blockDim.x >> 6) / WPA;
+  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const int wsub = wave % WPA;
+  if (a >= p.a0 + p.n_an) return;
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
+  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+  if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+  if constexpr (WPA > 1) {
+    if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+  }
+}
+
+// BF16 hi/lo split: yhat = sqrt(w) y (FP32) = hi + lo with hi = bf16(yhat),
+// lo = bf16(yhat - hi); A ~ hi hi^T + hi lo^T + lo hi^T, FP32 accumulation.
+template <int KP, int WS, int PPW>
+__device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long long a, int ah,
+                                                   double vc, long long r0, long long r1,
+                                                   int lane) {
+  constexpr int T = KP / 16;
+  using PT = PairTable<T>;
+  constexpr int P0 = WS * PPW;
+  constexpr int P1 = (P0 + PPW < PT::N) ? P0 + PPW : PT::N;
+  constexpr int NP = P1 - P0;
+  const int i16 = lane & 15, q = lane >> 4;
+  f4 acc[NP];
+#pragma unroll
+  for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
+  double bacc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  for (long long base = r0; base < r1; base += 32) {
+    bf8 hi[T], lo[T];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long rr = base + 8 * q + u;
+      float s = 0.f;
+      double wd = 0.0;
+      const float* yr = nullptr;
+      if (rr < r1) {
+        const int row = p.row_idx[rr];
+        const double w = row_weight(p, row, p.row_gc[rr], vc);
+        s = sqrtf((float)w);
+        wd = w * p.dep[row];
+        yr = p.yb32 + (long long)row * p.k;
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int m = 16 * t + i16;
+        const float yv = (yr && m < p.k) ? yr[m] : 0.f;
+        const float yh = s * yv;
+        const __bf16 h = (__bf16)yh;
+        hi[t][u] = h;
+        lo[t][u] = (__bf16)(yh - (float)h);
+        if (WS == 0) bacc[t] = fma(wd, (double)yv, bacc[t]);
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < NP; ++x) {
+      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], hi[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], lo[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[I], hi[J], acc[x], 0, 0, 0);
+    }
+  }
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+#pragma unroll
+  for (int x = 0; x < NP; ++x) {
+    const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
+      A[(long long)col * KP + row] = (double)acc[x][r];
+    }
+  }
+  if (WS == 0) {
+    double* b = p.b + (a - p.a0) * KP;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      double v = bacc[t];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (q == 0) b[16 * t + i16] = v;
+    }
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(256) void k_products_bf16(ProdParams p) {
+  constexpr int T = KP / 16;
+  constexpr int NPAIR = T * (T + 1) / 2;
+  constexpr int WPA = (NPAIR + 9) / 10;
+  constexpr int PPW = (NPAIR + WPA - 1) / WPA;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int apb = (blockDim.x >> 6) / WPA;
+  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const int wsub = wave % WPA;
+  if (a >= p.a0 + p.n_an) return;
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
+  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+  if (wsub == 0) products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+  if constexpr (WPA > 1) {
+    if (wsub == 1) products_bf16_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 2) products_bf16_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 3) products_bf16_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// transform: FP64 eigensolve + Pa, T, overline_w, trans, fused update
+// ---------------------------------------------------------------------------
+//
+// One wavefront per analysis, lane r owns row r.  The symmetric matrix and the
+// eigenvector matrix live in LDS (column-major, leading dimension KMAX+1):
+//   1. Householder tridiagonalisation (Golub & Van Loan Alg. 8.3.1)
+//   2. explicit Q from the stored reflectors (backward accumulation)
+//   3. implicit QL with Wilkinson shift on the tridiagonal (Bowdler, Martin,
+//      Reinsch & Wilkinson 1968), rotations applied to Z = Q in place
+//   4. lambda_j += (k-1)/rho (letkf_core.F90:110-113), guard for reduced modes
+//   5. overline_w = Z diag(1/lambda) Z^T b, T = Z diag(sqrt((k-1)/lambda)) Z^T
+//   6. trans = T + overline_w 1^T; xa(:,v) = trans^T xb(:,v) + axb(v), then
+//      axa(v) = mean_e xa(e,v), xa(:,v) -= axa(v)
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_transform(XformParams p) {
+  constexpr int LD = KMAX + 1;
+  __shared__ double sA[KMAX * LD];
+  __shared__ double sZ[KMAX * LD];
+  __shared__ double sv[KMAX], sw[KMAX], sd[KMAX + 1], se[KMAX + 1], stau[KMAX], sb[KMAX],
+      st[KMAX], sx[KMAX];
+  const int lane = threadIdx.x;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const int k = p.k;
+  const bool act_r = lane < k;
+  const double* Ag = p.A + (a - p.a0) * (long long)p.kp * p.kp;
+  const double* bg = p.b + (a - p.a0) * p.kp;
+
+  // ---- load lower triangle, symmetrise, zero padding ----
+  for (int c = 0; c < KMAX; ++c) {
+    double v = 0.0;
+    if (lane < KMAX) {
+      if (act_r && c < k) v = (lane >= c) ? Ag[(long long)c * p.kp + lane] : Ag[(long long)lane * p.kp + c];
+      sA[c * LD + lane] = v;
+    }
+  }
+  if (lane < KMAX) sb[lane] = act_r ? bg[lane] : 0.0;
+  __syncthreads();
+
+  // ---- 1. tridiagonalisation ----
+  for (int j = 0; j + 2 < k; ++j) {
+    const bool act = act_r && lane > j;
+    const double x = act ? sA[j * LD + lane] : 0.0;
+    const double x0 = sA[j * LD + j + 1];
+    const double rest = wave_sum((act && lane > j + 1) ? x * x : 0.0);
+    if (rest == 0.0) {
+      if (lane == 0) { se[j] = x0; stau[j] = 0.0; }
+      __syncthreads();
+      continue;
+    }
+    const double nrm = sqrt(rest + x0 * x0);
+    const double alpha = (x0 >= 0.0) ? -nrm : nrm;
+    const double v0 = x0 - alpha;
+    const double tau = 2.0 / (rest + v0 * v0);
+    const double v = act ? ((lane == j + 1) ? v0 : x) : 0.0;
+    if (lane < KMAX) sv[lane] = v;
+    __syncthreads();
+    double pr = 0.0;
+    if (act)
+      for (int c = j + 1; c < k; ++c) pr += sA[c * LD + lane] * sv[c];
+    pr *= tau;
+    const double K = 0.5 * tau * wave_sum(v * pr);
+    const double wr = act ? pr - K * v : 0.0;
+    if (lane < KMAX) sw[lane] = wr;
+    __syncthreads();
+    if (act) {
+      for (int c = j + 1; c < k; ++c) sA[c * LD + lane] -= v * sw[c] + wr * sv[c];
+      sA[j * LD + lane] = v;   // keep the reflector below the diagonal
+    }
+    if (lane == 0) { se[j] = alpha; stau[j] = tau; }
+    __syncthreads();
+  }
+  if (act_r) sd[lane] = sA[lane * LD + lane];
+  if (lane == 0) {
+    if (k >= 2) se[k - 2] = sA[(k - 2) * LD + k - 1];
+    se[k - 1] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- 2. Z = Q = H_0 ... H_{k-3} ----
+  for (int c = 0; c < KMAX; ++c)
+    if (lane < KMAX) sZ[c * LD + lane] = (lane == c) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int j = k - 3; j >= 0; --j) {
+    const double tau = stau[j];
+    if (tau == 0.0) continue;
+    const int c = lane;
+    if (c > j && c < k) {
+      double s = 0.0;
+      for (int i = j + 1; i < k; ++i) s += sA[j * LD + i] * sZ[c * LD + i];
+      s *= tau;
+      for (int i = j + 1; i < k; ++i) sZ[c * LD + i] -= s * sA[j * LD + i];
+    }
+    __syncthreads();
+  }
+
+  // ---- 3. implicit QL on (d, e), rotations applied to Z ----
+  const double eps = 2.220446049250313e-16;
+  // negligible subdiagonal: relative to the neighbours or to the norm of T
+  const double tnorm = wave_max(act_r ? fabs(sd[lane]) + fabs(se[lane]) : 0.0);
+  int status = 0;
+  for (int l = 0; l < k && !status; ++l) {
+    int iter = 0;
+    for (;;) {
+      int m = l;
+      for (; m < k - 1; ++m) {
+        const double dd = fabs(sd[m]) + fabs(sd[m + 1]);
+        if (fabs(se[m]) <= eps * dd || fabs(se[m]) <= eps * tnorm) break;
+      }
+      if (m == l) break;
+      if (++iter > 30) { status = l + 1; break; }
+      double g = (sd[l + 1] - sd[l]) / (2.0 * se[l]);
+      double r = hypot(g, 1.0);
+      g = sd[m] - sd[l] + se[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      double carry = act_r ? sZ[m * LD + lane] : 0.0;
+      double ei = se[m - 1], di = sd[m - 1], di1 = sd[m];
+      int i = m - 1;
+      bool underflow = false;
+      for (; i >= l; --i) {
+        const double f = s * ei, bb = c * ei;
+        r = hypot(f, g);
+        se[i + 1] = r;
+        if (r == 0.0) {
+          sd[i + 1] = di1 - pp;
+          se[m] = 0.0;
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = di1 - pp;
+        r = (di - g) * s + 2.0 * c * bb;
+        pp = s * r;
+        sd[i + 1] = g + pp;
+        g = c * r - bb;
+        const double zi = act_r ? sZ[i * LD + lane] : 0.0;
+        if (act_r) sZ[(i + 1) * LD + lane] = s * zi + c * carry;
+        carry = c * zi - s * carry;
+        if (i > l) {
+          ei = se[i - 1];
+          di1 = di;
+          di = sd[i - 1];
+        }
+      }
+      if (underflow) {
+        if (act_r) sZ[(i + 1) * LD + lane] = carry;
+        __syncthreads();
+        continue;
+      }
+      if (act_r) sZ[l * LD + lane] = carry;
+      sd[l] = sd[l] - pp;
+      se[l] = g;
+      se[m] = 0.0;
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+
+  // ---- 4. eigenvalues of A + (k-1)/rho I ----
+  double lam = act_r ? sd[lane] : 0.0;
+  if (p.guard) {
+    const double mn = wave_min(act_r ? lam : 1e300);
+    if (mn < 0.0) lam = lam - mn;
+  }
+  lam = lam * 1.0 + (double)(k - 1) / p.inflation;
+
+  // ---- 5. overline_w = Z diag(1/lam) Z^T b ----
+  {
+    double t = 0.0;
+    if (act_r)
+      for (int i = 0; i < k; ++i) t += sZ[lane * LD + i] * sb[i];
+    if (lane < KMAX) st[lane] = act_r ? t / lam : 0.0;
+    if (lane < KMAX) sx[lane] = act_r ? sqrt((double)(k - 1) / lam) : 0.0;
+  }
+  __syncthreads();
+  double wbar = 0.0;
+  if (act_r)
+    for (int j = 0; j < k; ++j) wbar += sZ[j * LD + lane] * st[j];
+  // T row r: T(r,c) = sum_j z(r,j) f_j z(c,j); trans(c, r) = T(c, r) + wbar_c
+  // lane e keeps column e of trans: tcol[f] = T(f,e) + wbar_f = T(e,f) + wbar_f
+  if (lane < KMAX) sw[lane] = act_r ? wbar : 0.0;
+  __syncthreads();
+  double tcol[KMAX];
+#pragma unroll
+  for (int f = 0; f < KMAX; ++f) tcol[f] = 0.0;
+  if (act_r) {
+    double zf[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) zf[j] = (j < k) ? sZ[j * LD + lane] * sx[j] : 0.0;
+#pragma unroll
+    for (int f = 0; f < KMAX; ++f) {
+      if (f < k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+          if (j < k) s += zf[j] * sZ[j * LD + f];
+        tcol[f] = s + sw[f];
+      }
+    }
+  }
+  if (p.status) p.status[a] = status;
+  if (p.trans_out && act_r) {
+    // trans(f, e) column-major: column e is tcol of lane e
+    double* tr = p.trans_out + a * (long long)k * k + (long long)lane * k;
+#pragma unroll
+    for (int f = 0; f < KMAX; ++f)
+      if (f < k) tr[f] = tcol[f];
+  }
+  if (p.w_out && act_r) p.w_out[a * (long long)k + lane] = wbar;
+  if (!p.xb) return;
+
+  // ---- 6. update ----
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
+  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  const long long lat_ext = p.g.nlat + 2 * p.halo;
+  int sk = 1, ek = p.nz;
+  if (p.vertical) {
+    sk = kc * p.cube_nz + 1;
+    ek = min(sk + p.cube_nz - 1, p.nz);
+  }
+  for (int lo = si; lo <= ei; ++lo) {
+    for (int la = sj; la <= ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int vv = 0; vv < p.n_var; ++vv) {
+        const int start = p.var_start[vv] - 1;
+        int l0, l1;
+        if (p.var_lev[vv] == 1) {
+          if (p.vertical && ek != p.nz) continue;
+          l0 = start; l1 = start;
+        } else if (p.vertical) {
+          l0 = start + sk - 1; l1 = start + ek - 1;
+        } else {
+          l0 = start; l1 = start + p.var_lev[vv] - 1;
+        }
+        for (int nk = l0; nk <= l1; ++nk) {
+          const long long o = off + nk;
+          __syncthreads();
+          if (act_r) sx[lane] = p.xb[o * k + lane];
+          __syncthreads();
+          double s = 0.0;
+          if (act_r) {
+#pragma unroll
+            for (int f = 0; f < KMAX; ++f)
+              if (f < k) s += tcol[f] * sx[f];
+          }
+          const double ab = p.axb[o];
+          const double xe = act_r ? s + ab : 0.0;
+          const double mean = wave_sum(xe) / k;
+          if (act_r) p.xa[o * k + lane] = xe - mean;
+          if (lane == 0) p.axa[o] = mean;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// caller epilogue (module_da_parallel.F90:219-261)
+// ---------------------------------------------------------------------------
+__global__ void k_epilogue(EpiParams p) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long lat_ext = p.nlat + 2 * p.halo, lon_ext = p.nlon + 2 * p.halo;
+  const long long total = lat_ext * lon_ext * p.n;
+  if (idx >= total) return;
+  const long long v = idx % p.n;
+  const long long la = (idx / p.n) % lat_ext;
+  const long long lo = idx / (p.n * lat_ext);
+  const int k = p.k;
+  const double* b = p.xb + idx * k;
+  const double* xa = p.xa + idx * k;
+  double* o = p.xa_out + idx * k;
+  const bool interior = la >= p.halo && la < p.nlat + p.halo && lo >= p.halo && lo < p.nlon + p.halo;
+  if (!interior) {
+    for (int e = 0; e < k; ++e) o[e] = 0.0;
+    return;
+  }
+  const long long so = ((lo - p.halo) * p.nlat + (la - p.halo)) * p.n + v;
+  const double ab = p.axb[idx], aa = p.axa[idx];
+  double sb = 0.0, sa = 0.0;
+  if (fabs((double)p.glat[la - p.halo]) <= p.polar_lat) {
+    for (int e = 0; e < k; ++e) {
+#pragma clang fp contract(off)
+      const double out = p.gamma1 * (ab + b[e]) + p.gamma2 * (aa + xa[e]);
+      o[e] = out;
+      sb += b[e] * b[e];
+      sa += (out - aa) * (out - aa);
+    }
+  } else {
+    for (int e = 0; e < k; ++e) {
+      o[e] = b[e] + ab;
+      sb += b[e] * b[e];
+      sa += b[e] * b[e];
+    }
+  }
+  p.spread_b[so] = sqrt(sb / (double)(float)k);
+  p.spread_a[so] = sqrt(sa / (double)(float)k);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_site_keys(hipStream_t st, int n_sites, const double* lat, const double* lon,
+                            double obs_polar, int nb_lat, int nb_lon, double bdeg_lat,
+                            double bdeg_lon, int* key, int* val, int* hist) {
+  if (n_sites <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_site_keys, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites, lat,
+                     lon, obs_polar, nb_lat, nb_lon, bdeg_lat, bdeg_lon, key, val, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, const double* lat,
+                              const double* lon, const int* row_ptr, double* s_cos, double* s_sin,
+                              double* s_lonr, int* s_row0, int* s_nrow) {
+  if (n_sites <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_site_gather, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites,
+                     order, lat, lon, row_ptr, s_cos, s_sin, s_lonr, s_row0, s_nrow);
+  return hipGetLastError();
+}
+
+hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill) {
+  const int ncol = p.h1 - p.h0;
+  if (ncol <= 0) return hipSuccess;
+  const int blocks = (ncol + 3) / 4;
+  if (fill) hipLaunchKernelGGL(k_select<true>, dim3(blocks), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL(k_select<false>, dim3(blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+template <int KP>
+static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int precision) {
+  constexpr int T = KP / 16;
+  constexpr int NPAIR = T * (T + 1) / 2;
+  constexpr int WPA = (NPAIR + 9) / 10;
+  constexpr int APB = 4 / WPA;
+  const long long blocks = (p.n_an + APB - 1) / APB;
+  if (blocks <= 0) return hipSuccess;
+  if (precision == 0) hipLaunchKernelGGL(k_products_f64<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  else if (precision == 1) hipLaunchKernelGGL(k_products_f32<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL(k_products_bf16<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision) {
+  if (p.kp == 32) return launch_products_kp<32>(st, p, precision);
+  if (p.kp == 64) return launch_products_kp<64>(st, p, precision);
+  if (p.kp == 128) return launch_products_kp<128>(st, p, precision);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_transform(hipStream_t st, const XformParams& p) {
+  if (p.n_an <= 0) return hipSuccess;
+  if (p.k <= 32) hipLaunchKernelGGL(k_transform<32>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
+  else if (p.k <= 64) hipLaunchKernelGGL(k_transform<64>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_epilogue(hipStream_t st, const EpiParams& p) {
+  const long long total = (long long)(p.nlat + 2 * p.halo) * (p.nlon + 2 * p.halo) * p.n;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_epilogue, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+}  // namespace dida

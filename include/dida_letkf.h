@@ -1,0 +1,196 @@
+/*
+ * dida_letkf.h -- C ABI of the MI355X LETKF analysis engine.
+ *
+ * Drop-in boundary for the x86 DA LETKF analysis step of the reference
+ * (Fortran, x86/DA/LETKF/src).  Every entry point takes plain pointers and
+ * sizes; arrays use the reference's Fortran (column-major) layouts:
+ *   xb(ens, n, 1-halo:nlat+halo, 1-halo:nlon+halo)   (letkf_compute.F90:15)
+ *   axb(n, 1-halo:nlat+halo, 1-halo:nlon+halo)        (letkf_compute.F90:16)
+ *   obs rows pro_yb(ens, .) with the member index fastest (read_obs_mod.F90:25)
+ *   trans(ens, ens), overline_w(ens)                  (letkf_core.F90:26-27)
+ * The reference's nx is the latitude count and ny the longitude count
+ * (letkf.F90:21); here they are called nlat and nlon.
+ *
+ * Ownership: the caller owns every array it passes; the library copies what
+ * it needs and keeps no caller pointer after a call returns.  Device-pointer
+ * variants (*_device) take memory on the current HIP device and run on the
+ * handle's stream.
+ *
+ * Errors: functions return DIDA_OK (0), a negative DIDA_ERR_* code, or, from
+ * the analysis calls, a positive count of analyses whose eigensolver did not
+ * converge (the reference prints dsyevd's ierr and continues,
+ * letkf_core.F90:119-122).  dida_letkf_last_error() gives the message of the
+ * last failure on the calling thread.  No call aborts the process.
+ *
+ * Threading: a handle is used by one host thread at a time; different
+ * handles are independent.
+ */
+#ifndef DIDA_LETKF_H
+#define DIDA_LETKF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIDA_LETKF_ABI_VERSION 1
+
+enum {
+    DIDA_OK = 0,
+    DIDA_ERR_ARG = -1,          /* invalid argument */
+    DIDA_ERR_HIP = -2,          /* HIP runtime error */
+    DIDA_ERR_NOMEM = -3,        /* device allocation failed */
+    DIDA_ERR_STATE = -4,        /* call out of order (no grid / no obs) */
+    DIDA_ERR_ADMISSION = -5,    /* serving: job refused by the HBM budget */
+    DIDA_ERR_UNSUPPORTED = -6   /* configuration not supported */
+};
+
+/* Precision modes.  FP64 is the reference letkf_core (computing_precision
+ * >= 52).  The reduced modes follow letkf_core_mpal's pattern
+ * (letkf_core_mpal.F90): low-precision obs-space products, FP64
+ * eigendecomposition and inverse square root, eigenvalue guard. */
+enum {
+    DIDA_PREC_FP64 = 0,     /* FP64 products (MFMA f64) and FP64 update */
+    DIDA_PREC_FP32 = 1,     /* FP32 MFMA products, FP64 eigensolve, FP32 update */
+    DIDA_PREC_BF16X3 = 2    /* BF16 hi/lo split MFMA products (3 terms, FP32
+                               accumulation), FP64 eigensolve, FP32 update */
+};
+
+/* Vertical coordinate of the vertical-localisation loop. */
+enum {
+    DIDA_VCOORD_LEVEL = 0,  /* level index (obs vcoord in level units) */
+    DIDA_VCOORD_LNP = 1     /* ln p: obs vcoord = log(ph) (letkf_yb.F90:211),
+                               column profile from the hybrid p(ps) */
+};
+
+/* Mirrors namelist_da / letkf_init globals (da_namelist_mod.F90:27-49,
+ * letkf_init.F90:27-81). */
+typedef struct dida_letkf_config {
+    int k;                  /* ensemble size (svr_ensemble), 2..128 */
+    int nlat, nlon;         /* interior grid of the block (nx, ny) */
+    int nz;                 /* levels of 3-D variables (num_lev) */
+    int n2d, n3d;           /* 2-D vars first, then nz levels per 3-D var:
+                               n = n2d + n3d*nz (letkf_init.F90:63-73) */
+    int halo;               /* da_halo of xb/xa/axb/axa */
+    int cube_n;             /* da_block_interval (horizontal cube) */
+    int cube_nz;            /* da_vertical_interval (vertical chunk) */
+    double local_dist;      /* da_loc_distance, km (GC support 2*local_dist) */
+    double vert_dist;       /* da_vert_distance; <= 0 runs the horizontal loop */
+    double inflation_factor;/* da_inflation_factor */
+    double obs_polar;       /* obs with |lat| > obs_polar are skipped */
+    int gc_filter;          /* da_gc_enable: schr2 (1) or boxcar schr (0) */
+    int gc_filter_vertical; /* da_gc_enable_vertical */
+    int precision;          /* DIDA_PREC_* */
+    int vcoord_mode;        /* DIDA_VCOORD_* */
+    int ps_index;           /* 0-based index of ps in n (ps_pos - 1) */
+    const double *hyai;     /* nz+1 hybrid coefficients (DIDA_VCOORD_LNP) */
+    const double *hybi;
+    double p0, ptop;
+    int keep_transforms;    /* keep trans/overline_w of every analysis */
+    int max_batch_columns;  /* 0 = automatic */
+} dida_letkf_config;
+
+typedef struct dida_letkf_stats {
+    long long n_analyses;   /* local analyses (k x k transform solves) */
+    long long total_rows;   /* sum of local obs rows over analyses */
+    int max_rows;           /* largest local obs count */
+    int n_flagged;          /* analyses whose eigensolver did not converge */
+    int rows_over_10000;    /* analyses the reference's MAX_OBS would truncate */
+    double ms_select;       /* device time, local obs selection */
+    double ms_products;     /* device time, obs-space products */
+    double ms_transform;    /* device time, eigensolve + transform + update */
+    double ms_total;        /* device time of the whole call */
+} dida_letkf_stats;
+
+typedef struct dida_letkf_handle dida_letkf_handle;
+
+/* Defaults of namelist_da where they exist (cube_n = 1 instead of 4, overlap
+ * is not supported). */
+void dida_letkf_config_default(dida_letkf_config *cfg);
+
+/* Replaces letkf_init (letkf_init.F90:2): validates the configuration and
+ * creates device state on the current HIP device. */
+int dida_letkf_create(const dida_letkf_config *cfg, dida_letkf_handle **out);
+int dida_letkf_destroy(dida_letkf_handle *h);
+
+/* Runs all later device work of the handle on this hipStream_t (NULL: the
+ * handle's own stream). */
+int dida_letkf_set_stream(dida_letkf_handle *h, void *hip_stream);
+
+/* Block grid as real(4) degrees, like pro_info's grid_lat/grid_lon
+ * (pro_info.F90:11-12); grid_lat has nlat entries, grid_lon nlon. */
+int dida_letkf_set_grid(dida_letkf_handle *h, const float *grid_lat, const float *grid_lon);
+
+/* Observation set of one cycle, replacing the obs lists that gen_obs_struct
+ * and comm_obs_data build (letkf_obs.F90:16-187,251-466).  Sites (lat, lon
+ * in degrees) own contiguous runs of rows; site_row_ptr (n_sites+1 entries,
+ * NULL = one row per site) gives each site's rows in level-major, var-minor
+ * order (letkf_yb.F90:102-103).  Per row: yb (k values, centred
+ * perturbations), dep = y - mean H(x), rinv = 1/sigma^2 (used as real(4)
+ * like rdiaginv), vcoord = level or log(ph) (NULL if no vertical loop).
+ * Sites with |lat| > obs_polar are dropped here (letkf_yb.F90:87). */
+int dida_letkf_set_obs(dida_letkf_handle *h, int n_sites, const double *site_lat,
+                       const double *site_lon, const int *site_row_ptr, int n_rows,
+                       const double *yb, const double *dep, const double *rinv,
+                       const double *vcoord);
+int dida_letkf_set_obs_device(dida_letkf_handle *h, int n_sites, const double *site_lat,
+                              const double *site_lon, const int *site_row_ptr, int n_rows,
+                              const double *yb, const double *dep, const double *rinv,
+                              const double *vcoord);
+
+/* The analysis loop of one cycle: letkf_compute_nooverlap
+ * (letkf_compute.F90:12-154) when vert_dist <= 0, else
+ * letkf_compute_nooverlap_vertical (letkf_compute.F90:386-554).  Host
+ * arrays; interior columns of xa/axa are written, halo entries are left as
+ * they are.  stats may be NULL. */
+int dida_letkf_analyze(dida_letkf_handle *h, const double *xb, const double *axb, double *xa,
+                       double *axa, dida_letkf_stats *stats);
+/* Same with device-resident arrays (the serving and benchmark path). */
+int dida_letkf_analyze_device(dida_letkf_handle *h, const double *xb, const double *axb,
+                              double *xa, double *axa, dida_letkf_stats *stats);
+
+/* Per-analysis outputs of the last analyze call when keep_transforms was
+ * set: trans (k*k per analysis, column-major), overline_w (k), nobsl.
+ * Any pointer may be NULL.  n_analyses must equal the analysis count. */
+int dida_letkf_get_transforms(dida_letkf_handle *h, long long n_analyses, double *trans,
+                              double *overline_w, int *nobsl);
+
+/* Number of local analyses the configuration performs per cycle. */
+long long dida_letkf_n_analyses(const dida_letkf_config *cfg);
+
+/* Batched letkf_core: n_analyses independent calls of letkf_core
+ * (letkf_core.F90:14) whose inputs are packed back to back.  Analysis a uses
+ * obs columns row_ptr[a] .. row_ptr[a+1]-1 of hxens (nanals x total, column
+ * major), rdiaginv, dep and rloc.  Outputs trans (nanals*nanals per
+ * analysis) and overline_w (nanals per analysis); status (may be NULL) gets
+ * the eigensolver flag per analysis.  Host arrays. */
+int dida_letkf_core_batched(int nanals, long long n_analyses, const long long *row_ptr,
+                            const double *hxens, const float *rdiaginv, const double *dep,
+                            const double *rloc, double *trans, double *overline_w,
+                            double inflation_factor, int precision, int *status);
+
+/* Drop-in for the external letkf_core (letkf_core.F90:14-33), the same
+ * by-reference signature the Sunway C kernel uses
+ * (sw/DA/LETKF/src/letkf_half/letkf_core_half_master.c:24).  Runs one
+ * analysis on the GPU in FP64; pflag is accepted and ignored. */
+void letkf_core_(const int *nanals, const int *nobsl, const double *hxens_in,
+                 const float *rdiaginv, const double *dep, const double *rloc, double *trans,
+                 double *overline_w, const double *inflation_factor, const int *pflag);
+
+/* Caller epilogue of run_da_parallel (module_da_parallel.F90:219-261) on
+ * device arrays: polar mask, gamma1/gamma2 blend and spreads.  xa_out has the
+ * xb layout (halo entries set to 0); spread_b/spread_a are (n, nlat, nlon).
+ * grid_lat is taken from set_grid. */
+int dida_letkf_epilogue_device(dida_letkf_handle *h, const double *xb, const double *axb,
+                               const double *xa, const double *axa, double polar_lat,
+                               double gamma1, double gamma2, double *xa_out, double *spread_b,
+                               double *spread_a);
+
+const char *dida_letkf_last_error(void);
+int dida_letkf_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIDA_LETKF_H */

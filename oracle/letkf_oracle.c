@@ -245,13 +245,21 @@ static int orc_tql(int n, double *d, double *e, double *z, int max_iter)
 {
     if (n <= 1) return 0;
     e[n - 1] = 0.0;
+    /* a subdiagonal entry is negligible relative to its neighbours or to the
+       norm of T (the EISPACK tql2 test); the absolute part deflates the
+       noise-level blocks a rank-deficient A leaves */
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double t = fabs(d[i]) + fabs(e[i]);
+        if (t > tnorm) tnorm = t;
+    }
     for (int l = 0; l < n; ++l) {
         int iter = 0;
         for (;;) {
             int m;
             for (m = l; m < n - 1; ++m) {
                 double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+                if (fabs(e[m]) <= DBL_EPSILON * dd || fabs(e[m]) <= DBL_EPSILON * tnorm) break;
             }
             if (m == l) break;
             if (++iter > max_iter) return l + 1;
@@ -389,6 +397,25 @@ int orc_syev_jacobi(int n, const double *a, double *w, double *v)
 /* ------------------------------------------------------------------------ */
 
 /*
+ * Eigenvalues of A (ascending) -> eigenvalues of A + (k-1)/rho I.
+ * letkf_core.F90:107-113: eival = eival*1 + (k-1)/inflation_factor.
+ * ORC_GUARD first lifts a negative spectrum: if the smallest eigenvalue of
+ * A is below 0 every eigenvalue is shifted up by that deficit -- the guard of
+ * the reference's reduced-precision cores (Sunway letkf_core_half_master.c:
+ * 157-162; letkf_core_mpal.F90:113-118 is the same test written after the
+ * (k-1)/rho shift, identical for rho = 1).
+ */
+static void orc_shift_eigenvalues(int k, double *eival, double inflation_factor, int flags)
+{
+    if ((flags & ORC_GUARD) && eival[0] < 0.0) {
+        double mn = eival[0];
+        for (int i = 0; i < k; ++i) eival[i] = eival[i] - mn;
+    }
+    for (int i = 0; i < k; ++i)
+        eival[i] = eival[i] * 1.0 + (double)(k - 1) / inflation_factor;
+}
+
+/*
  * letkf_core.F90:14-233.  hxens_in is (nanals, nobsl) column-major; trans is
  * (nanals, nanals) column-major.  flags: ORC_EIG_JACOBI selects the Jacobi
  * eigensolver; ORC_GUARD applies the letkf_core_mpal eigenvalue guard
@@ -427,22 +454,10 @@ int orc_letkf_core(int nanals, int nobsl, const double *hxens_in, const float *r
             for (int r = 0; r < k; ++r) col[r] += y[r] * yc;
         }
     }
-    if (flags & ORC_GUARD)
-        for (int i = 0; i < k; ++i)
-            work1[IDX(i, i, k)] += (double)(k - 1) / inflation_factor;
     /* :98 dsyevd('V','L') */
     int info = (flags & ORC_EIG_JACOBI) ? orc_syev_jacobi(k, work1, eival, eivec)
                                         : orc_syev_ql(k, work1, eival, eivec);
-    if (flags & ORC_GUARD) {
-        if (eival[0] < (double)(k - 1)) {
-            double mn = eival[0] - k + 1;
-            for (int i = 0; i < k; ++i) eival[i] -= mn;
-        }
-    } else {
-        /* :107-113 rho = 1; eival = eival*rho + (k-1)/inflation */
-        for (int i = 0; i < k; ++i)
-            eival[i] = eival[i] * 1.0 + (double)(k - 1) / inflation_factor;
-    }
+    orc_shift_eigenvalues(k, eival, inflation_factor, flags);
     /* :129-143 pa = (eivec / eival) eivec^T */
     for (int j = 0; j < k; ++j)
         for (int i = 0; i < k; ++i) tmp[IDX(i, j, k)] = eivec[IDX(i, j, k)] / eival[j];
@@ -505,18 +520,9 @@ int orc_letkf_transform(int k, const double *a, const double *b, double inflatio
     double *tmp = (double *)malloc(sizeof(double) * kk);
     double *vb = (double *)malloc(sizeof(double) * k);
     memcpy(work1, a, sizeof(double) * kk);
-    if (flags & ORC_GUARD)
-        for (int i = 0; i < k; ++i) work1[IDX(i, i, k)] += (double)(k - 1) / inflation_factor;
     int info = (flags & ORC_EIG_JACOBI) ? orc_syev_jacobi(k, work1, eival, eivec)
                                         : orc_syev_ql(k, work1, eival, eivec);
-    if (flags & ORC_GUARD) {
-        if (eival[0] < (double)(k - 1)) {
-            double mn = eival[0] - k + 1;
-            for (int i = 0; i < k; ++i) eival[i] -= mn;
-        }
-    } else {
-        for (int i = 0; i < k; ++i) eival[i] = eival[i] * 1.0 + (double)(k - 1) / inflation_factor;
-    }
+    orc_shift_eigenvalues(k, eival, inflation_factor, flags);
     /* overline_w = V diag(1/eival) V^T b */
     for (int j = 0; j < k; ++j) {
         double s = 0.0;

@@ -89,6 +89,101 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 
 }  // namespace
 
+// Scratch of the split eigensolver for one batch of analyses.
+struct EigScratch {
+  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
+  std::vector<int> st_host;
+};
+
+constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
+constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
+
+// bytes of split-eigensolver scratch per analysis
+size_t eig_bytes_per_analysis(int k) {
+  const int kp = kp_of(k);
+  return sizeof(double) * ((size_t)kp * kp + 4 * kp) + 16 * (size_t)kRotCapPerK2 * k * k +
+         16 * (size_t)kHdrCapPerK * k + sizeof(int);
+}
+
+// Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
+// eigensolver (k_tridiag, k_ql_log, k_apply), then the single-kernel
+// transform for analyses the split path flagged; mode 1 runs only the single
+// kernel.
+int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode) {
+  if (mode == 1 || xp.k > 64) {
+    HIP_TRY(launch_transform(st, xp));
+    return DIDA_OK;
+  }
+  const long long n = xp.n_an;
+  const int k = xp.k, kp = xp.kp;
+  const int rot_cap = kRotCapPerK2 * k * k, hdr_cap = kHdrCapPerK * k;
+  RC_TRY(s.refl.ensure(sizeof(double) * n * kp * kp));
+  RC_TRY(s.tau.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.d.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.e.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.lam.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.rot.ensure((size_t)16 * n * rot_cap));
+  RC_TRY(s.hdr.ensure((size_t)16 * n * hdr_cap));
+  RC_TRY(s.nchase.ensure(sizeof(int) * n));
+  EigParams ep;
+  ep.g = xp.g;
+  ep.k = k;
+  ep.kp = kp;
+  ep.inflation = xp.inflation;
+  ep.guard = xp.guard;
+  ep.a0 = xp.a0;
+  ep.n_an = n;
+  ep.A = xp.A;
+  ep.b = xp.b;
+  ep.refl = s.refl.as<double>();
+  ep.tau = s.tau.as<double>();
+  ep.d = s.d.as<double>();
+  ep.e = s.e.as<double>();
+  ep.lam = s.lam.as<double>();
+  ep.rot = s.rot.as<double2>();
+  ep.hdr = s.hdr.as<int4>();
+  ep.rot_cap = rot_cap;
+  ep.hdr_cap = hdr_cap;
+  ep.nchase = s.nchase.as<int>();
+  ep.status = xp.status;
+  ep.nzc = xp.nzc;
+  ep.vertical = xp.vertical;
+  ep.cube_nz = xp.cube_nz;
+  ep.nz = xp.nz;
+  ep.halo = xp.halo;
+  ep.n = xp.n;
+  ep.n_var = xp.n_var;
+  for (int v = 0; v < kMaxVars; ++v) {
+    ep.var_start[v] = xp.var_start[v];
+    ep.var_lev[v] = xp.var_lev[v];
+  }
+  ep.xb = xp.xb;
+  ep.axb = xp.axb;
+  ep.xa = xp.xa;
+  ep.axa = xp.axa;
+  ep.trans_out = xp.trans_out;
+  ep.w_out = xp.w_out;
+  HIP_TRY(launch_eig(st, ep));
+  s.st_host.resize(n);
+  HIP_TRY(hipMemcpyAsync(s.st_host.data(), xp.status + xp.a0, sizeof(int) * n,
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> flagged;
+  for (long long i = 0; i < n; ++i)
+    if (s.st_host[i] != 0) flagged.push_back((int)(xp.a0 + i));
+  if (!flagged.empty()) {
+    RC_TRY(s.list.ensure(sizeof(int) * flagged.size()));
+    HIP_TRY(hipMemcpyAsync(s.list.p, flagged.data(), sizeof(int) * flagged.size(),
+                           hipMemcpyHostToDevice, st));
+    XformParams xf = xp;
+    xf.list = s.list.as<int>();
+    xf.n_an = (long long)flagged.size();
+    HIP_TRY(launch_transform(st, xf));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return DIDA_OK;
+}
+
 struct dida_letkf_handle {
   dida_letkf_config cfg;
   int device = 0;
@@ -113,6 +208,7 @@ struct dida_letkf_handle {
   // work
   DevBuf cnt, off, scan_tmp, row_idx, row_gc, Abuf, bbuf, status, trans, wout;
   DevBuf x_xb, x_axb, x_xa, x_axa;   // host-entry staging
+  EigScratch eig;
   std::vector<int> nobsl_host;
   long long max_batch = 0;
 };
@@ -213,9 +309,11 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->nb_lon = (int)std::ceil(360.0 / bd);
   h->bdeg_lat = 180.0 / h->nb_lat;
   h->bdeg_lon = 360.0 / h->nb_lon;
-  // columns per batch: bound the products buffer to ~4 GiB
-  long long per_col = (long long)h->nzc * h->kp * (h->kp + 1) * 8;
-  long long mb = std::max<long long>(1, (4LL << 30) / per_col);
+  // columns per batch: bound the per-batch scratch (products + eigensolver)
+  // to ~6 GiB
+  long long per_col = (long long)h->nzc *
+                      ((long long)h->kp * (h->kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k));
+  long long mb = std::max<long long>(1, (6LL << 30) / per_col);
   if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
   h->max_batch = std::min<long long>(mb, h->n_h);
   *out = h;
@@ -231,7 +329,8 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->s_lonr, &h->s_row0, &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai,
                     &h->d_hybi, &h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
-                    &h->x_xa, &h->x_axa};
+                    &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
+                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
   for (DevBuf* b : bufs) b->release();
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -563,7 +662,8 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     xp.trans_out = c.keep_transforms ? h->trans.as<double>() : nullptr;
     xp.w_out = c.keep_transforms ? h->wout.as<double>() : nullptr;
     xp.status = h->status.as<int>();
-    HIP_TRY(launch_transform(st, xp));
+    xp.list = nullptr;
+    RC_TRY(run_transform(st, h->eig, xp, c.transform_kernel));
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipEventSynchronize(ev[3]));
     float t1 = 0, t2 = 0, t3 = 0;
@@ -747,7 +847,17 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   xp.trans_out = d_t->as<double>();
   xp.w_out = d_w->as<double>();
   xp.status = d_s->as<int>();
-  HIP_TRY(launch_transform(st, xp));
+  xp.list = nullptr;
+  EigScratch es;
+  int rc_t = run_transform(st, es, xp, 0);
+  DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
+                     &es.list};
+  if (rc_t < 0) {
+    for (DevBuf* b : ebufs) b->release();
+    return rc_t;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  for (DevBuf* b : ebufs) b->release();
   HIP_TRY(hipMemcpyAsync(trans, d_t->p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(overline_w, d_w->p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost, st));
   std::vector<int> sh(n_an);

@@ -78,6 +78,36 @@ struct XformParams {
   double* trans_out;         // optional, k*k per analysis (global index)
   double* w_out;             // optional, k per analysis
   int* status;               // optional, per analysis
+  const int* list;           // optional: analyses to run (global indices)
+};
+
+struct EigParams {
+  Geom g;
+  int k, kp;
+  double inflation;
+  int guard;
+  long long a0, n_an;
+  const double* A;          // batch-local, full symmetric kp x kp
+  const double* b;          // batch-local, kp
+  double* refl;             // batch-local, kp x kp reflectors (column j = v_j)
+  double* tau;              // batch-local, kp
+  double* d;                // batch-local, kp (tridiagonal diagonal)
+  double* e;                // batch-local, kp (subdiagonal)
+  double* lam;              // batch-local, kp (eigenvalues of A)
+  double2* rot;             // batch-local, rot_cap Givens (c, s) per analysis
+  int4* hdr;                // batch-local, hdr_cap chases (lo, hi, off) per analysis
+  int rot_cap, hdr_cap;
+  int* nchase;              // batch-local
+  int* status;              // global index
+  int nzc, vertical, cube_nz, nz, halo, n, n_var;
+  int var_start[kMaxVars];
+  int var_lev[kMaxVars];
+  const double* xb;
+  const double* axb;
+  double* xa;
+  double* axa;
+  double* trans_out;        // optional, global index
+  double* w_out;            // optional, global index
 };
 
 struct EpiParams {
@@ -103,5 +133,6 @@ hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
+hipError_t launch_eig(hipStream_t st, const EigParams& p);
 
 }  // namespace dida

@@ -259,6 +259,7 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * I + q + 4 * r, col = 16 * J + i16;
       A[(long long)col * KP + row] = acc[x][r];
+      if (I != J) A[(long long)row * KP + col] = acc[x][r];
     }
   }
   if (WS == 0) {
@@ -349,6 +350,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
       A[(long long)col * KP + row] = (double)acc[x][r];
+      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
     }
   }
   if (WS == 0) {
@@ -445,6 +447,7 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
       A[(long long)col * KP + row] = (double)acc[x][r];
+      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
     }
   }
   if (WS == 0) {
@@ -504,8 +507,8 @@ __global__ __launch_bounds__(64) void k_transform(XformParams p) {
   __shared__ double sv[KMAX], sw[KMAX], sd[KMAX + 1], se[KMAX + 1], stau[KMAX], sb[KMAX],
       st[KMAX], sx[KMAX];
   const int lane = threadIdx.x;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
+  const long long a = p.list ? (long long)p.list[blockIdx.x] : p.a0 + blockIdx.x;
+  if (!p.list && a >= p.a0 + p.n_an) return;
   const int k = p.k;
   const bool act_r = lane < k;
   const double* Ag = p.A + (a - p.a0) * (long long)p.kp * p.kp;
@@ -838,6 +841,7 @@ hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision) {
 
 hipError_t launch_transform(hipStream_t st, const XformParams& p) {
   if (p.n_an <= 0) return hipSuccess;
+  // with a list, n_an is the list length
   if (p.k <= 32) hipLaunchKernelGGL(k_transform<32>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
   else if (p.k <= 64) hipLaunchKernelGGL(k_transform<64>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
   else return hipErrorInvalidValue;

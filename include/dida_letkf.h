@@ -89,6 +89,9 @@ typedef struct dida_letkf_config {
     double p0, ptop;
     int keep_transforms;    /* keep trans/overline_w of every analysis */
     int max_batch_columns;  /* 0 = automatic */
+    int transform_kernel;   /* 0: split eigensolver (tridiagonalisation,
+                               lane-per-matrix QL, rotation apply); 1: the
+                               single-kernel transform */
 } dida_letkf_config;
 
 typedef struct dida_letkf_stats {

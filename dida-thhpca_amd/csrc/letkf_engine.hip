@@ -184,6 +184,21 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   return DIDA_OK;
 }
 
+// Columns per batch: the per-batch scratch (products + eigensolver) is
+// bounded to ~6 GiB, or by max_batch_columns.
+long long max_batch_columns(const dida_letkf_config& c) {
+  const int kp = kp_of(c.k);
+  const int cz = c.cube_nz > 0 ? c.cube_nz : c.nz;
+  const long long nzc = c.vert_dist > 0.0 ? (c.nz + cz - 1) / cz : 1;
+  const long long per_col =
+      nzc * ((long long)kp * (kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k));
+  long long mb = std::max<long long>(1, (6LL << 30) / per_col);
+  if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
+  const long long nh = (long long)((c.nlat + c.cube_n - 1) / c.cube_n) *
+                       ((c.nlon + c.cube_n - 1) / c.cube_n);
+  return std::min<long long>(mb, nh);
+}
+
 struct dida_letkf_handle {
   dida_letkf_config cfg;
   int device = 0;
@@ -309,13 +324,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->nb_lon = (int)std::ceil(360.0 / bd);
   h->bdeg_lat = 180.0 / h->nb_lat;
   h->bdeg_lon = 360.0 / h->nb_lon;
-  // columns per batch: bound the per-batch scratch (products + eigensolver)
-  // to ~6 GiB
-  long long per_col = (long long)h->nzc *
-                      ((long long)h->kp * (h->kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k));
-  long long mb = std::max<long long>(1, (6LL << 30) / per_col);
-  if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
-  h->max_batch = std::min<long long>(mb, h->n_h);
+  h->max_batch = max_batch_columns(c);
   *out = h;
   return DIDA_OK;
 }
@@ -880,6 +889,36 @@ void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,
   int rc = dida_letkf_core_batched(*nanals, 1, rp, hxens_in, rdiaginv, dep, rloc, trans,
                                    overline_w, *inflation_factor, DIDA_PREC_FP64, &st);
   if (rc != 0) fprintf(stderr, "letkf_core_: rc=%d %s\n", rc, g_last_error.c_str());
+}
+
+size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows) {
+  if (!c || c->k < 2 || c->nlat < 1 || c->nlon < 1 || c->cube_n < 1) return 0;
+  const size_t k = c->k, kp = kp_of(c->k);
+  const size_t ns = std::max(n_sites, 1), nr = std::max(n_rows, 1);
+  const int cz = c->cube_nz > 0 ? c->cube_nz : c->nz;
+  const size_t nzc = c->vert_dist > 0.0 ? (c->nz + cz - 1) / cz : 1;
+  const size_t mb = (size_t)max_batch_columns(*c);
+  const size_t nh = (size_t)((c->nlat + c->cube_n - 1) / c->cube_n) *
+                    ((c->nlon + c->cube_n - 1) / c->cube_n);
+  const size_t n_an = nh * nzc;
+  const size_t n_state = (size_t)(c->nlat + 2 * c->halo) * (c->nlon + 2 * c->halo) *
+                         (c->n2d + c->n3d * c->nz);
+  size_t b = 0;
+  b += ns * (8 * 2 + 4 + 16 + 24 + 8) + 16 * ns + (1 << 20);       // sites, keys, index, sort
+  b += nr * (8 * k + 8 * 3 + 4);                                    // rows
+  if (c->precision != DIDA_PREC_FP64) b += nr * 4 * k;              // FP32 copy of yb
+  b += 2 * 4 * 200000;                                              // bins (upper bound)
+  // local obs rows of one batch: support cap fraction of the sphere, x2
+  const double th = 2.0 * c->local_dist / 6371.39306640625;
+  const double frac = std::min(1.0, (1.0 - std::cos(std::min(th, M_PI))) / 2.0 * 2.0);
+  const size_t rows_col = std::min<size_t>(nr, (size_t)(frac * nr) + 64);
+  b += 12 * mb * rows_col;
+  b += 16 * (mb + 1) * 2;
+  b += mb * nzc * (8 * kp * kp + 8 * kp + eig_bytes_per_analysis(c->k));
+  b += 4 * n_an;
+  if (c->keep_transforms) b += n_an * (8 * k * k + 8 * k);
+  b += n_state * (k + 1) * 8;                                       // serving ping-pong state
+  return b + (16 << 20);
 }
 
 int dida_letkf_epilogue_device(dida_letkf_handle* h, const double* xb, const double* axb,

@@ -1,0 +1,257 @@
+// letkf_serve.hip -- continuous batching of LETKF analysis jobs on one GPU.
+//
+// FIFO admission by HBM budget and an active-job limit; every admitted job
+// runs on its own host thread, engine handle and HIP stream, so kernels of
+// different jobs overlap on the device.  A job's ensemble stays in device
+// memory between its cycles: the analysis of cycle c (mean-zero perturbations
+// and mean) is the background of cycle c+1, ping-ponging between the caller's
+// output arrays and one server-owned state buffer.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "dida_letkf.h"
+
+namespace {
+
+double now_s() {
+  using clk = std::chrono::steady_clock;
+  static const clk::time_point t0 = clk::now();
+  return std::chrono::duration<double>(clk::now() - t0).count();
+}
+
+struct Job {
+  long long id = 0;
+  dida_serve_job spec{};
+  std::vector<float> glat, glon;
+  std::vector<dida_serve_obs> obs;
+  dida_serve_status st{};
+  std::atomic<bool> cancel{false};
+  int max_sites = 0, max_rows = 0;
+};
+
+}  // namespace
+
+struct dida_serve {
+  dida_serve_config cfg{};
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;          // job state changes
+  std::deque<std::shared_ptr<Job>> queue;
+  std::map<long long, std::shared_ptr<Job>> jobs;
+  std::vector<std::thread> workers;
+  size_t used = 0;
+  int active = 0;
+  long long next_id = 1;
+  bool stopping = false;
+  std::thread dispatcher;
+};
+
+static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
+  (void)hipSetDevice(s->device);
+  dida_serve_job& sp = j->spec;
+  dida_letkf_handle* h = nullptr;
+  int rc = dida_letkf_create(&sp.cfg, &h);
+  double* xt = nullptr;
+  double* at = nullptr;
+  long long n_an = 0;
+  const size_t n_state = (size_t)(sp.cfg.nlat + 2 * sp.cfg.halo) * (sp.cfg.nlon + 2 * sp.cfg.halo) *
+                         (sp.cfg.n2d + sp.cfg.n3d * sp.cfg.nz);
+  const size_t k = sp.cfg.k;
+  if (rc == DIDA_OK) rc = dida_letkf_set_grid(h, j->glat.data(), j->glon.data());
+  if (rc == DIDA_OK && sp.n_cycles > 1) {
+    if (hipMalloc(&xt, n_state * k * 8) != hipSuccess || hipMalloc(&at, n_state * 8) != hipSuccess)
+      rc = DIDA_ERR_NOMEM;
+  }
+  // ping-pong: cycle c reads (bx, ba) and writes (ox, oa); the last cycle
+  // writes the caller's (xa, axa)
+  const double* bx = sp.xb;
+  const double* ba = sp.axb;
+  for (int c = 0; c < sp.n_cycles && rc >= 0; ++c) {
+    if (j->cancel.load()) {
+      rc = DIDA_ERR_STATE;
+      break;
+    }
+    const int left = sp.n_cycles - 1 - c;     // cycles after this one
+    double* ox = (left % 2 == 0) ? sp.xa : xt;
+    double* oa = (left % 2 == 0) ? sp.axa : at;
+    const dida_serve_obs& o = j->obs[c];
+    int r = dida_letkf_set_obs_device(h, o.n_sites, o.site_lat, o.site_lon, o.site_row_ptr,
+                                      o.n_rows, o.yb, o.dep, o.rinv, o.vcoord);
+    if (r < 0) { rc = r; break; }
+    dida_letkf_stats stt;
+    r = dida_letkf_analyze_device(h, bx, ba, ox, oa, &stt);
+    if (r < 0) { rc = r; break; }
+    rc = std::max(rc, r);
+    n_an += stt.n_analyses;
+    bx = ox;
+    ba = oa;
+  }
+  if (h) dida_letkf_destroy(h);
+  if (xt) (void)hipFree(xt);
+  if (at) (void)hipFree(at);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    j->st.t_end = now_s();
+    j->st.rc = rc;
+    j->st.n_analyses = n_an;
+    j->st.state = j->cancel.load() && rc == DIDA_ERR_STATE ? DIDA_JOB_CANCELLED
+                  : (rc < 0 ? DIDA_JOB_FAILED : DIDA_JOB_DONE);
+    s->used -= j->st.bytes;
+    s->active -= 1;
+  }
+  s->cv.notify_all();
+}
+
+static void dispatch_loop(dida_serve* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv.wait(lk, [&] {
+      if (s->stopping) return true;
+      if (s->queue.empty() || s->active >= s->cfg.max_active) return false;
+      return s->used + s->queue.front()->st.bytes <= s->cfg.hbm_budget_bytes;
+    });
+    if (s->stopping && s->queue.empty()) break;
+    if (s->stopping) {
+      // drain: cancel what was never admitted
+      for (auto& j : s->queue) {
+        j->st.state = DIDA_JOB_CANCELLED;
+        j->st.t_end = now_s();
+      }
+      s->queue.clear();
+      s->cv.notify_all();
+      break;
+    }
+    std::shared_ptr<Job> j = s->queue.front();
+    s->queue.pop_front();
+    if (j->cancel.load()) {
+      j->st.state = DIDA_JOB_CANCELLED;
+      j->st.t_end = now_s();
+      s->cv.notify_all();
+      continue;
+    }
+    s->used += j->st.bytes;
+    s->active += 1;
+    j->st.state = DIDA_JOB_RUNNING;
+    j->st.t_start = now_s();
+    s->workers.emplace_back(run_job, s, j);
+  }
+}
+
+extern "C" {
+
+double dida_serve_now(void) { return now_s(); }
+
+int dida_serve_create(const dida_serve_config* cfg, dida_serve** out) {
+  if (!cfg || !out || cfg->max_active < 1 || cfg->hbm_budget_bytes == 0) return DIDA_ERR_ARG;
+  dida_serve* s = new (std::nothrow) dida_serve();
+  if (!s) return DIDA_ERR_NOMEM;
+  s->cfg = *cfg;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return DIDA_ERR_HIP;
+  }
+  now_s();
+  s->dispatcher = std::thread(dispatch_loop, s);
+  *out = s;
+  return DIDA_OK;
+}
+
+int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_id) {
+  if (!s || !job || !job_id || job->n_cycles < 1 || !job->obs || !job->grid_lat ||
+      !job->grid_lon || !job->xb || !job->axb || !job->xa || !job->axa)
+    return DIDA_ERR_ARG;
+  auto j = std::make_shared<Job>();
+  j->spec = *job;
+  j->spec.cfg.hyai = nullptr;   // LNP serving jobs are not supported
+  j->spec.cfg.hybi = nullptr;
+  if (job->cfg.vert_dist > 0.0 && job->cfg.vcoord_mode != DIDA_VCOORD_LEVEL) return DIDA_ERR_UNSUPPORTED;
+  j->glat.assign(job->grid_lat, job->grid_lat + job->cfg.nlat);
+  j->glon.assign(job->grid_lon, job->grid_lon + job->cfg.nlon);
+  j->obs.assign(job->obs, job->obs + job->n_cycles);
+  for (const auto& o : j->obs) {
+    j->max_sites = std::max(j->max_sites, o.n_sites);
+    j->max_rows = std::max(j->max_rows, o.n_rows);
+  }
+  j->st.bytes = dida_letkf_job_bytes(&job->cfg, j->max_sites, j->max_rows);
+  if (j->st.bytes == 0) return DIDA_ERR_ARG;
+  if (j->st.bytes > s->cfg.hbm_budget_bytes) return DIDA_ERR_ADMISSION;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) return DIDA_ERR_STATE;
+    j->id = s->next_id++;
+    j->st.state = DIDA_JOB_QUEUED;
+    j->st.t_submit = now_s();
+    s->jobs[j->id] = j;
+    s->queue.push_back(j);
+    *job_id = j->id;
+  }
+  s->cv.notify_all();
+  return DIDA_OK;
+}
+
+int dida_serve_poll(dida_serve* s, long long id, dida_serve_status* st) {
+  if (!s || !st) return DIDA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DIDA_ERR_ARG;
+  *st = it->second->st;
+  return DIDA_OK;
+}
+
+int dida_serve_wait(dida_serve* s, long long id, dida_serve_status* st) {
+  if (!s) return DIDA_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DIDA_ERR_ARG;
+  auto j = it->second;
+  s->cv.wait(lk, [&] { return j->st.state >= DIDA_JOB_DONE; });
+  if (st) *st = j->st;
+  return DIDA_OK;
+}
+
+int dida_serve_cancel(dida_serve* s, long long id) {
+  if (!s) return DIDA_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return DIDA_ERR_ARG;
+  it->second->cancel.store(true);
+  if (it->second->st.state == DIDA_JOB_QUEUED) {
+    for (auto q = s->queue.begin(); q != s->queue.end(); ++q)
+      if ((*q)->id == id) {
+        s->queue.erase(q);
+        break;
+      }
+    it->second->st.state = DIDA_JOB_CANCELLED;
+    it->second->st.t_end = now_s();
+    s->cv.notify_all();
+  }
+  return DIDA_OK;
+}
+
+int dida_serve_destroy(dida_serve* s) {
+  if (!s) return DIDA_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv.notify_all();
+  if (s->dispatcher.joinable()) s->dispatcher.join();
+  // running jobs finish on their own threads
+  for (auto& t : s->workers)
+    if (t.joinable()) t.join();
+  delete s;
+  return DIDA_OK;
+}
+
+}  // extern "C"

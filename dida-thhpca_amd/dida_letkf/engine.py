@@ -41,6 +41,34 @@ class DidaConfig(C.Structure):
                 ("max_batch_columns", C.c_int), ("transform_kernel", C.c_int)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int)]
+
+
+class ServeObs(C.Structure):
+    _fields_ = [("n_sites", C.c_int), ("site_lat", _vp), ("site_lon", _vp),
+                ("site_row_ptr", _vp), ("n_rows", C.c_int), ("yb", _vp), ("dep", _vp),
+                ("rinv", _vp), ("vcoord", _vp)]
+
+
+class ServeJob(C.Structure):
+    _fields_ = [("cfg", DidaConfig), ("grid_lat", _fp), ("grid_lon", _fp), ("xb", _vp),
+                ("axb", _vp), ("n_cycles", C.c_int), ("obs", C.POINTER(ServeObs)),
+                ("xa", _vp), ("axa", _vp)]
+
+
+class ServeStatus(C.Structure):
+    _fields_ = [("state", C.c_int), ("rc", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_end", C.c_double), ("n_analyses", C.c_longlong),
+                ("bytes", C.c_size_t)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
+
+
 class DidaStats(C.Structure):
     _fields_ = [("n_analyses", C.c_longlong), ("total_rows", C.c_longlong),
                 ("max_rows", C.c_int), ("n_flagged", C.c_int), ("rows_over_10000", C.c_int),
@@ -57,7 +85,9 @@ EXPORTS = [
     "dida_letkf_set_obs_device", "dida_letkf_analyze", "dida_letkf_analyze_device",
     "dida_letkf_get_transforms", "dida_letkf_n_analyses", "dida_letkf_core_batched",
     "letkf_core_", "dida_letkf_epilogue_device", "dida_letkf_last_error",
-    "dida_letkf_abi_version",
+    "dida_letkf_abi_version", "dida_serve_create", "dida_serve_submit", "dida_serve_poll",
+    "dida_serve_wait", "dida_serve_cancel", "dida_serve_destroy", "dida_serve_now",
+    "dida_letkf_job_bytes",
 ]
 
 _lib = None
@@ -97,6 +127,15 @@ def lib(path: Optional[str] = None):
     L.dida_letkf_epilogue_device.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
                                              C.c_double, _vp, _vp, _vp]
     L.dida_letkf_last_error.restype = C.c_char_p
+    L.dida_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(_vp)]
+    L.dida_serve_submit.argtypes = [_vp, C.POINTER(ServeJob), C.POINTER(C.c_longlong)]
+    L.dida_serve_poll.argtypes = [_vp, C.c_longlong, C.POINTER(ServeStatus)]
+    L.dida_serve_wait.argtypes = [_vp, C.c_longlong, C.POINTER(ServeStatus)]
+    L.dida_serve_cancel.argtypes = [_vp, C.c_longlong]
+    L.dida_serve_destroy.argtypes = [_vp]
+    L.dida_serve_now.restype = C.c_double
+    L.dida_letkf_job_bytes.argtypes = [C.POINTER(DidaConfig), C.c_int, C.c_int]
+    L.dida_letkf_job_bytes.restype = C.c_size_t
     _lib = L
     return L
 
@@ -306,3 +345,80 @@ def letkf_compute_nooverlap(inst, precision: Optional[int] = None, keep_transfor
             t, w, nob = eng.get_transforms()
             out.update(trans=t, overline_w=w, nobsl=nob)
         return out
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+
+class Server:
+    """Continuous-batching server (dida_serve_*): FIFO admission by HBM budget,
+    one stream per running job, device-resident ensembles across cycles."""
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 4):
+        c = ServeConfig(hbm_budget_bytes=int(hbm_budget_bytes), max_active=int(max_active))
+        h = _vp()
+        _check(lib().dida_serve_create(C.byref(c), C.byref(h)), "dida_serve_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, cfg: LetkfConfig, grid_lat, grid_lon, xb_ptr, axb_ptr, obs_cycles,
+               xa_ptr, axa_ptr) -> int:
+        """obs_cycles: list of dicts with device pointers n_sites, site_lat, site_lon,
+        site_row_ptr, n_rows, yb, dep, rinv, vcoord."""
+        job = ServeJob()
+        job.cfg = to_c_config(cfg)
+        la = np.ascontiguousarray(grid_lat, np.float32)
+        lo = np.ascontiguousarray(grid_lon, np.float32)
+        arr = (ServeObs * len(obs_cycles))()
+        for i, o in enumerate(obs_cycles):
+            arr[i] = ServeObs(n_sites=int(o["n_sites"]), site_lat=o["site_lat"],
+                              site_lon=o["site_lon"], site_row_ptr=o.get("site_row_ptr"),
+                              n_rows=int(o["n_rows"]), yb=o["yb"], dep=o["dep"],
+                              rinv=o["rinv"], vcoord=o.get("vcoord"))
+        job.grid_lat, job.grid_lon = _p(la, _fp), _p(lo, _fp)
+        job.xb, job.axb, job.xa, job.axa = xb_ptr, axb_ptr, xa_ptr, axa_ptr
+        job.n_cycles = len(obs_cycles)
+        job.obs = C.cast(arr, C.POINTER(ServeObs))
+        jid = C.c_longlong()
+        _check(lib().dida_serve_submit(self.h, C.byref(job), C.byref(jid)), "dida_serve_submit")
+        return int(jid.value)
+
+    def poll(self, jid: int) -> dict:
+        st = ServeStatus()
+        _check(lib().dida_serve_poll(self.h, jid, C.byref(st)), "dida_serve_poll")
+        return st.as_dict()
+
+    def wait(self, jid: int) -> dict:
+        st = ServeStatus()
+        _check(lib().dida_serve_wait(self.h, jid, C.byref(st)), "dida_serve_wait")
+        return st.as_dict()
+
+    def cancel(self, jid: int):
+        _check(lib().dida_serve_cancel(self.h, jid), "dida_serve_cancel")
+
+    @staticmethod
+    def now() -> float:
+        return lib().dida_serve_now()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dida_serve_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_bytes(cfg: LetkfConfig, n_sites: int, n_rows: int) -> int:
+    c = to_c_config(cfg)
+    return int(lib().dida_letkf_job_bytes(C.byref(c), int(n_sites), int(n_rows)))

@@ -190,6 +190,74 @@ int dida_letkf_epilogue_device(dida_letkf_handle *h, const double *xb, const dou
                                double gamma1, double gamma2, double *xa_out, double *spread_b,
                                double *spread_a);
 
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of analysis jobs on one GPU.
+ *
+ * No counterpart in the reference (it runs one DA cycle per MPI job).  A job
+ * is a configuration, a grid, a device-resident background and the obs of
+ * n_cycles cycles; the server admits queued jobs in FIFO order while its HBM
+ * budget and active-job limit allow, runs each admitted job on its own HIP
+ * stream (jobs overlap on the device), keeps the ensemble on the device
+ * between a job's cycles (the analysis of cycle c is the background of cycle
+ * c+1), and records submit/start/end times.  A job's arithmetic does not
+ * depend on what else runs: results are the same alone and in a mix.
+ * ------------------------------------------------------------------------ */
+typedef struct dida_serve dida_serve;
+
+typedef struct dida_serve_config {
+    size_t hbm_budget_bytes;  /* device memory admitted jobs may hold in total */
+    int max_active;           /* jobs running at once (one stream each), >= 1 */
+} dida_serve_config;
+
+/* one cycle's obs set; pointers are device memory (as dida_letkf_set_obs_device) */
+typedef struct dida_serve_obs {
+    int n_sites;
+    const double *site_lat, *site_lon;
+    const int *site_row_ptr;
+    int n_rows;
+    const double *yb, *dep, *rinv, *vcoord;
+} dida_serve_obs;
+
+typedef struct dida_serve_job {
+    dida_letkf_config cfg;
+    const float *grid_lat, *grid_lon;   /* host, copied at submit */
+    const double *xb, *axb;             /* device, background of cycle 0 (read only) */
+    int n_cycles;
+    const dida_serve_obs *obs;          /* host array of n_cycles entries (copied at
+                                           submit); the device arrays it points to must
+                                           stay valid until the job has ended */
+    double *xa, *axa;                   /* device, analysis of the last cycle */
+} dida_serve_job;
+
+enum {
+    DIDA_JOB_QUEUED = 0,
+    DIDA_JOB_RUNNING = 1,
+    DIDA_JOB_DONE = 2,
+    DIDA_JOB_FAILED = 3,
+    DIDA_JOB_CANCELLED = 4
+};
+
+typedef struct dida_serve_status {
+    int state;                /* DIDA_JOB_* */
+    int rc;                   /* analysis return code (or error) of the job */
+    double t_submit, t_start, t_end;   /* seconds on dida_serve_now()'s clock */
+    long long n_analyses;     /* local analyses over all cycles */
+    size_t bytes;             /* HBM the job was admitted with */
+} dida_serve_status;
+
+int dida_serve_create(const dida_serve_config *cfg, dida_serve **out);
+/* Queues a job; DIDA_ERR_ADMISSION if it can never fit the budget. */
+int dida_serve_submit(dida_serve *s, const dida_serve_job *job, long long *job_id);
+int dida_serve_poll(dida_serve *s, long long job_id, dida_serve_status *st);
+int dida_serve_wait(dida_serve *s, long long job_id, dida_serve_status *st);
+/* Cancels a queued job; a running job stops after its current cycle. */
+int dida_serve_cancel(dida_serve *s, long long job_id);
+/* Cancels queued jobs, waits for running ones, frees the server. */
+int dida_serve_destroy(dida_serve *s);
+double dida_serve_now(void);
+/* Device bytes one job of this shape needs (what admission charges). */
+size_t dida_letkf_job_bytes(const dida_letkf_config *cfg, int n_sites, int n_rows);
+
 const char *dida_letkf_last_error(void);
 int dida_letkf_abi_version(void);
 

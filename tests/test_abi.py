@@ -56,9 +56,11 @@ def _c_layout():
 #include <stddef.h>
 #include "dida_letkf.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(dida_letkf_config), offsetof(dida_letkf_config, local_dist),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(dida_letkf_config), offsetof(dida_letkf_config, local_dist),
          offsetof(dida_letkf_config, hyai), offsetof(dida_letkf_config, keep_transforms),
-         sizeof(dida_letkf_stats), offsetof(dida_letkf_stats, ms_select));
+         sizeof(dida_letkf_stats), offsetof(dida_letkf_stats, ms_select),
+         sizeof(dida_serve_job), offsetof(dida_serve_job, obs), sizeof(dida_serve_obs),
+         sizeof(dida_serve_status));
   return 0;
 }
 '''
@@ -79,6 +81,9 @@ def test_struct_layout_matches_header():
     assert s[3] == DidaConfig.keep_transforms.offset
     assert s[4] == C.sizeof(DidaStats)
     assert s[5] == DidaStats.ms_select.offset
+    from dida_letkf.engine import ServeJob, ServeObs, ServeStatus
+    assert s[6] == C.sizeof(ServeJob) and s[7] == ServeJob.obs.offset
+    assert s[8] == C.sizeof(ServeObs) and s[9] == C.sizeof(ServeStatus)
 
 
 def test_n_analyses_matches_config(L):

@@ -37,6 +37,79 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+KERNEL_NAMES = {"products": "k_products", "tridiag": "k_tridiag", "ql": "k_ql_log",
+                "apply": "k_apply"}
+
+
+def kernel_work(name, stats, cfg, n_an, mixed):
+    """Algorithmic flops of one cycle's launches of a kernel, and its peak.
+
+    products: the symmetric SYRK k(k+1) p + Yb^T R^-1 d 2k p per analysis
+              (BF16 MFMA peak in the split-bf16 mode, else FP64);
+    tridiag:  Householder tridiagonalisation 4/3 k^3 (Golub & Van Loan 8.3.1);
+    ql:       the QL recurrence, ~30 flops per Givens step, ~1.1 k^2 steps;
+    apply:    Q = H_0..H_{k-3} 4/3 k^3, rotation accumulation 6 k^3 (GvL 8.3.5),
+              T = V f V^T 2k^3, overline_w 4k^2, update 2 k^2 m.
+    The FP64 vector and FP64 matrix peaks of MI355X are equal (78.6 TFLOP/s)."""
+    k, m = cfg.k, cfg.n
+    if name == "products":
+        p = stats["total_rows"]
+        fl = k * (k + 1) * p + 2 * k * p
+        return fl, (PEAK_BF16_TFLOPS if mixed == 2 else PEAK_FP64_TFLOPS)
+    if name == "tridiag":
+        return n_an * (4.0 / 3.0) * k ** 3, PEAK_FP64_TFLOPS
+    if name == "ql":
+        return n_an * 30 * 1.1 * k ** 2, PEAK_FP64_TFLOPS
+    return n_an * ((4.0 / 3.0) * k ** 3 + 6 * k ** 3 + 2 * k ** 3 + 4 * k ** 2 + 2 * k ** 2 * m), \
+        PEAK_FP64_TFLOPS
+
+
+def roofline(km, stats, cfg, n_an, mixed):
+    cands = [x for x in ("products", "tridiag", "ql", "apply") if km.get(x, 0.0) > 0]
+    dom = max(cands, key=lambda x: km[x])
+    fl, peak = kernel_work(dom, stats, cfg, n_an, mixed)
+    achieved = fl / (km[dom] * 1e-3) / 1e12
+    return dict(bound="mfma", kernel=KERNEL_NAMES[dom], achieved=round(achieved, 4), peak=peak,
+                unit="TFLOP/s", frac=round(achieved / peak, 5), traffic=None,
+                ms_per_launch=round(km[dom], 4), flops_per_launch=fl,
+                kernel_ms={KERNEL_NAMES[x]: round(km[x], 4) for x in cands})
+
+
+def serving_leg(torch, dev, args, precision):
+    """p50/p99 job latency of the serving loop at fractions of its saturation rate."""
+    from dida_letkf import serving
+    pools = serving.Pools(torch, dev, args.seed, serving.GRIDS, serving.OBS_RANGE[1])
+    n_sat = args.serve_sat_jobs or args.serve_jobs
+    sat_jobs = serving.draw_jobs(n_sat, None, args.seed + 1)
+    # build every pool before timing (backgrounds and obs pools are inputs)
+    for j in sat_jobs:
+        pools.background(j.grid, j.k)
+        pools.obs_pool(j.k, serving.GRIDS[j.grid][2])
+    torch.cuda.synchronize()
+    log(f"serving: pools ready, {torch.cuda.memory_allocated() / 2**30:.1f} GiB allocated")
+    sat = serving.run(torch, sat_jobs, pools, precision=precision, seed=args.seed)
+    sat_rate = sat.n_jobs / sat.makespan
+    out = dict(saturation=sat.summary(), runs=[])
+    log(f"serving saturation: {out['saturation']}")
+    for frac in [float(x) for x in args.serve_rates.split(",") if x]:
+        jobs = serving.draw_jobs(args.serve_jobs, frac * sat_rate, args.seed + 2)
+        for j in jobs:
+            pools.background(j.grid, j.k)
+            pools.obs_pool(j.k, serving.GRIDS[j.grid][2])
+        torch.cuda.synchronize()
+        r = serving.run(torch, jobs, pools, precision=precision, seed=args.seed + 3)
+        r.rate = frac * sat_rate
+        summ = r.summary()
+        summ["load"] = frac
+        out["runs"].append(summ)
+        log(f"serving at {frac:.0%} of saturation: {summ}")
+    out["workload"] = (f"Poisson stream, grid from {serving.GRIDS} (nlat, nlon, nz), k from "
+                       f"{serving.KS}, obs rows log-uniform in {serving.OBS_RANGE}, "
+                       f"{serving.CYCLES} cycles per job, c={serving.LOCAL_DIST_KM:g} km, "
+                       f"precision {precision}")
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +122,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16, help="every Nth column for the CPU baseline")
     ap.add_argument("--parity-cols", type=int, default=48)
+    ap.add_argument("--serve-jobs", type=int, default=0,
+                    help="jobs per arrival rate in the serving leg (0: no serving leg)")
+    ap.add_argument("--serve-rates", default="0.5,0.8",
+                    help="arrival rates as fractions of the measured saturation rate")
+    ap.add_argument("--serve-sat-jobs", type=int, default=0,
+                    help="jobs of the closed burst that measures saturation (0: = serve-jobs)")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     return ap.parse_args()
 
 
@@ -64,6 +144,10 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
+    if args.serve_only:
+        out = serving_leg(torch, dev, args, args.mixed)
+        print(json.dumps(dict(metric="LETKF serving job latency (p50/p99)", serving=out)), flush=True)
+        return
     t0 = time.time()
     inst = synth.generate(args.config, seed=args.seed)
     cfg = inst.cfg
@@ -100,7 +184,7 @@ def main():
         for _ in range(args.warmup):
             step(eng, xa, axa)
         torch.cuda.synchronize()
-        k_ms = dict(select=0.0, products=0.0, transform=0.0)
+        k_ms = dict(select=0.0, products=0.0, transform=0.0, tridiag=0.0, ql=0.0, apply=0.0)
         ts = time.perf_counter()
         flagged = 0
         for _ in range(args.steps):
@@ -109,6 +193,9 @@ def main():
             k_ms["select"] += st.ms_select
             k_ms["products"] += st.ms_products
             k_ms["transform"] += st.ms_transform
+            k_ms["tridiag"] += st.ms_tridiag
+            k_ms["ql"] += st.ms_ql
+            k_ms["apply"] += st.ms_apply
         torch.cuda.synchronize()
         el = time.perf_counter() - ts
         stats = eng.stats.as_dict()
@@ -167,24 +254,9 @@ def main():
                           f"own Householder+QL eigensolver, FP64")
         log(f"cpu baseline: {cpu}")
 
-    # ---- roofline of the dominant kernel (largest average duration)
+    # ---- roofline of the dominant kernel (largest average device time)
     head = res[args.mixed]
-    km = head["kernel_ms"]
-    dom = max(("products", "transform"), key=lambda x: km[x])
-    k = cfg.k
-    st = head["stats"]
-    m_col = cfg.n
-    if dom == "transform":
-        # textbook symmetric-QR eigensolve with vectors 9k^3 (Golub & Van Loan
-        # Sec. 8.3) + T = V f V^T 2k^3 + overline_w 4k^2 + update 2k^2 m
-        flops = N_A * (9 * k**3 + 2 * k**3 + 4 * k**2 + 2 * k**2 * m_col)
-        peak = PEAK_FP64_TFLOPS
-    else:
-        flops = k * (k + 1) * st["total_rows"] + 2 * k * st["total_rows"]
-        peak = PEAK_BF16_TFLOPS if args.mixed == 2 else PEAK_FP64_TFLOPS * 2
-    achieved = flops / (km[dom] * 1e-3) / 1e12
-    roof = dict(bound="mfma", kernel=dom, achieved=round(achieved, 4), peak=peak, unit="TFLOP/s",
-                frac=round(achieved / peak, 5), traffic=None)
+    roof = roofline(head["kernel_ms"], head["stats"], cfg, N_A, args.mixed)
 
     line = {
         "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
@@ -214,6 +286,10 @@ def main():
         line["fp64_ms_per_step"] = round(res[PREC_FP64]["ms_step"], 3)
         line["fp64_kernel_ms"] = res[PREC_FP64]["kernel_ms"]
         line["mixed_over_fp64"] = round(head["value"] / res[PREC_FP64]["value"], 3)
+    if args.serve_jobs > 0:
+        del d
+        torch.cuda.empty_cache()
+        line["serving"] = serving_leg(torch, dev, args, args.mixed)
     print(json.dumps(line), flush=True)
 
 

@@ -73,14 +73,23 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     const double v = act ? ((lane == j + 1) ? v0 : x) : 0.0;
     if (lane < KMAX) sv[lane] = v;
     __syncthreads();
-    // p = tau A v over the trailing columns (sv is zero for c <= j)
-    double pr = 0.0;
+    // p = tau A v over the trailing columns (sv is zero for c <= j); four
+    // partial sums, one block of 8 columns in flight at a time
+    double pr0 = 0.0, pr1 = 0.0, pr2 = 0.0, pr3 = 0.0;
 #pragma unroll
     for (int B = 0; B < NB; ++B) {
       if (8 * B + 7 <= j) continue;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) pr = fma(arow[8 * B + t], sv[8 * B + t], pr);
+      pr0 = fma(arow[8 * B + 0], sv[8 * B + 0], pr0);
+      pr1 = fma(arow[8 * B + 1], sv[8 * B + 1], pr1);
+      pr2 = fma(arow[8 * B + 2], sv[8 * B + 2], pr2);
+      pr3 = fma(arow[8 * B + 3], sv[8 * B + 3], pr3);
+      pr0 = fma(arow[8 * B + 4], sv[8 * B + 4], pr0);
+      pr1 = fma(arow[8 * B + 5], sv[8 * B + 5], pr1);
+      pr2 = fma(arow[8 * B + 6], sv[8 * B + 6], pr2);
+      pr3 = fma(arow[8 * B + 7], sv[8 * B + 7], pr3);
+      __builtin_amdgcn_sched_barrier(0);
     }
+    double pr = (pr0 + pr1) + (pr2 + pr3);
     pr = act ? pr * tau : 0.0;
     const double K = 0.5 * tau * wave_sum(v * pr);
     const double wr = act ? pr - K * v : 0.0;
@@ -95,6 +104,7 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
         arow[c] -= v * sw[c] + wr * sv[c];
         xn = (c == j + 1) ? arow[c] : xn;
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (lane < KMAX) refl[(long long)j * kp + lane] = v;
     if (lane == 0) { se_s[j] = alpha; st_s[j] = tau; }
@@ -170,25 +180,35 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       const int off = nrot;
       int i = m - 1;
       bool underflow = false;
+      // d[i], e[i] of the next step are read before this step's stores
+      double ei = se[i][lane], di = sd[i][lane], di1 = sd[i + 1][lane];
       for (; i >= l; --i) {
-        const double ei = se[i][lane];
+        double ein = 0.0, din = 0.0;
+        if (i > l) {
+          ein = se[i - 1][lane];
+          din = sd[i - 1][lane];
+        }
         const double f = s * ei, bb = c * ei;
-        r = hypot(f, g);
+        r = sqrt(f * f + g * g);
         se[i + 1][lane] = r;
         if (r == 0.0) {
-          sd[i + 1][lane] -= pp;
+          sd[i + 1][lane] = di1 - pp;
           se[m][lane] = 0.0;
           underflow = true;
           break;
         }
-        s = f / r;
-        c = g / r;
-        g = sd[i + 1][lane] - pp;
-        r = (sd[i][lane] - g) * s + 2.0 * c * bb;
+        const double rinv = 1.0 / r;
+        s = f * rinv;
+        c = g * rinv;
+        g = di1 - pp;
+        r = (di - g) * s + 2.0 * c * bb;
         pp = s * r;
         sd[i + 1][lane] = g + pp;
         g = c * r - bb;
         rot[nrot++] = make_double2(c, s);
+        ei = ein;
+        di1 = di;
+        di = din;
       }
       const int lo = underflow ? i + 1 : l;
       if (m - lo > 0) hdr[nhdr++] = make_int4(lo, m, off, 0);
@@ -211,8 +231,16 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   constexpr int NB = KMAX / 8;
   constexpr int TT = KMAX / 16;          // 16x16 tiles per dimension
   constexpr int LDV = KMAX + 1;
-  __shared__ double sV[KMAX * LDV];
+  constexpr int HMAX = kHdrCapPerK * KMAX;            // chase headers per analysis
+  constexpr int RING = 16 * KMAX;                      // rotation ring (entries)
+  constexpr int REFILL = RING / 2;                     // entries per ring refill
+  constexpr int SBUF = (KMAX * LDV > 2 * HMAX + 2 * RING) ? KMAX * LDV : 2 * HMAX + 2 * RING;
+  static_assert(REFILL >= KMAX && REFILL % 64 == 0, "a refill covers any chase");
+  // phase A: reflectors (row j = v_j); phase B: chase headers + rotation ring;
+  // phases C, D: V rows, then staged background vectors
+  __shared__ __align__(16) double sbuf[SBUF];
   __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX];
+  double* sV = sbuf;
   const int lane = threadIdx.x;
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
@@ -221,40 +249,88 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   const int k = p.k, kp = p.kp;
   const bool in_r = lane < k;
 
-  // ---- Q rows by forward accumulation: Z <- Z H_j ----
+  // ---- A. Q rows by forward accumulation: Z <- Z H_j ----
+  // All reflectors and taus are staged in LDS first (loads issued in groups
+  // of 16, one memory latency per group instead of one per reflector).
+  const double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + al * kp;
+  const int nref = k > 2 ? k - 2 : 0;
+#pragma unroll
+  for (int j0 = 0; j0 < KMAX; j0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      v[t] = (j0 + t < nref && lane < KMAX) ? refl[(long long)(j0 + t) * kp + lane] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (j0 + t < nref && lane < KMAX) sV[(j0 + t) * LDV + lane] = v[t];
+  }
+  if (lane < nref) st[lane] = tg[lane];
+  __syncthreads();
   double z[KMAX];
 #pragma unroll
   for (int c = 0; c < KMAX; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
-  const double* __restrict__ refl = p.refl + al * (long long)kp * kp;
-  const double* __restrict__ tg = p.tau + al * kp;
-  for (int j = 0; j + 2 < k; ++j) {
-    const double tau = tg[j];
+  for (int j = 0; j < nref; ++j) {
+    const double tau = st[j];
     if (tau == 0.0) continue;
-    const double* __restrict__ vj = refl + (long long)j * kp;
-    double dot = 0.0;
+    const double* q = sV + j * LDV;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
     for (int B = 0; B < NB; ++B) {
       if (8 * B + 7 <= j) continue;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) dot = fma(z[8 * B + t], vj[8 * B + t], dot);
+      d0 = fma(z[8 * B + 0], q[8 * B + 0], d0);
+      d1 = fma(z[8 * B + 1], q[8 * B + 1], d1);
+      d2 = fma(z[8 * B + 2], q[8 * B + 2], d2);
+      d3 = fma(z[8 * B + 3], q[8 * B + 3], d3);
+      d0 = fma(z[8 * B + 4], q[8 * B + 4], d0);
+      d1 = fma(z[8 * B + 5], q[8 * B + 5], d1);
+      d2 = fma(z[8 * B + 6], q[8 * B + 6], d2);
+      d3 = fma(z[8 * B + 7], q[8 * B + 7], d3);
     }
-    dot *= tau;
+    const double dot = ((d0 + d1) + (d2 + d3)) * tau;
 #pragma unroll
     for (int B = 0; B < NB; ++B) {
       if (8 * B + 7 <= j) continue;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) z[8 * B + t] = fma(-dot, vj[8 * B + t], z[8 * B + t]);
+      for (int t = 0; t < 8; ++t) z[8 * B + t] = fma(-dot, q[8 * B + t], z[8 * B + t]);
     }
   }
+  __syncthreads();
 
-  // ---- the logged QL rotations, chase by chase ----
+  // ---- B. the logged QL rotations, chase by chase ----
+  // Headers are staged in LDS once; rotations stream through an LDS ring,
+  // REFILL entries (REFILL / 64 coalesced loads per lane) at a time.
+  int4* shdr = reinterpret_cast<int4*>(sbuf);
+  double2* sring = reinterpret_cast<double2*>(sbuf + 2 * HMAX);
   const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
   const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
-  const int nch = p.nchase[al];
+  const int nch = min(p.nchase[al], HMAX);
+  for (int i = lane; i < nch; i += 64) shdr[i] = hdr[i];
+  __syncthreads();
+  int nrot = 0;
+  if (nch > 0) {
+    const int4 hl = shdr[nch - 1];
+    nrot = hl.z + (hl.y - hl.x);
+  }
+  int committed = 0;                     // rotation entries present in the ring
   for (int ch = 0; ch < nch; ++ch) {
-    const int4 h = hdr[ch];
-    const int lo = h.x, hi = h.y, off = h.z;
+    const int4 h = shdr[ch];
+    const int lo = h.x, hi = h.y;
+    if (h.z + (hi - lo) > committed) {
+      double2 v[REFILL / 64];
+#pragma unroll
+      for (int t = 0; t < REFILL / 64; ++t) {
+        const int e = committed + 64 * t + lane;
+        v[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
+      }
+#pragma unroll
+      for (int t = 0; t < REFILL / 64; ++t)
+        sring[(committed + 64 * t + lane) & (RING - 1)] = v[t];
+      committed += REFILL;
+      __syncthreads();
+    }
     const int bhi = (hi - 1) >> 3, blo = lo >> 3;
+    const int base = h.z + hi - 1;       // entry of the rotation at position i is base - i
     double carry = 0.0;
 #pragma unroll
     for (int B = NB - 1; B >= 0; --B) {
@@ -267,7 +343,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
         if (i > KMAX - 2) continue;
         double c = 1.0, s = 0.0;
         if (i >= lo && i < hi) {
-          const double2 cs = rot[off + (hi - 1 - i)];
+          const double2 cs = sring[(base - i) & (RING - 1)];
           c = cs.x;
           s = cs.y;
         }
@@ -278,6 +354,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
       if (B == blo) z[8 * B] = carry;
     }
   }
+  __syncthreads();
 
   // ---- eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
   double lam = in_r ? p.lam[al * kp + lane] : 0.0;
@@ -385,27 +462,47 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
     }
   }
   for (int t = 0; t < nrun; ++t) nvec += rl[t];
+  // state index of vector v of the column (-1 past the end)
+  auto sidx = [&](int v) {
+    int rem = v;
+    for (int t = 0; t < nrun; ++t) {
+      if (rem < rl[t]) return rs[t] + rem;
+      rem -= rl[t];
+    }
+    return -1;
+  };
+  // one group = 16 state vectors: lane e loads member e of each, plus the
+  // means of the 4 vectors whose outputs it writes; the next group is loaded
+  // while the current one is multiplied
+  double xn[16], abn[4];
+  auto load_group = [&](long long off, int v0) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+      xn[u] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + lane] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int v = v0 + (lane >> 4) + 4 * r;
+      const int s_i = (v < nvec) ? sidx(v) : -1;
+      abn[r] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
+    }
+  };
   for (int lo = si; lo <= ei; ++lo) {
     for (int la = sj; la <= ej; ++la) {
       const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      load_group(off, 0);
       for (int v0 = 0; v0 < nvec; v0 += 16) {
-        // state index of vector v0 + i for i = 0..15 (lane i16 maps its own)
-        auto sidx = [&](int v) {
-          int rem = v;
-          for (int t = 0; t < nrun; ++t) {
-            if (rem < rl[t]) return rs[t] + rem;
-            rem -= rl[t];
-          }
-          return -1;
-        };
-        // stage 16 xb vectors (k members each) in LDS, coalesced per vector
         __syncthreads();
-        for (int u = 0; u < 16; ++u) {
-          const int st_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          if (lane < KMAX)
-            sV[u * LDV + lane] = (st_i >= 0 && in_r) ? p.xb[(off + st_i) * k + lane] : 0.0;
+        if (lane < KMAX) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) sV[u * LDV + lane] = xn[u];
         }
+        double ab4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ab4[r] = abn[r];
         __syncthreads();
+        if (v0 + 16 < nvec) load_group(off, v0 + 16);
         d4 out[TT];
 #pragma unroll
         for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
@@ -422,7 +519,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
         for (int r = 0; r < 4; ++r) {
           const int v = v0 + q + 4 * r;
           const int st_i = (v < nvec) ? sidx(v) : -1;
-          const double ab = (st_i >= 0) ? p.axb[off + st_i] : 0.0;
+          const double ab = ab4[r];
           double sum = 0.0;
           double xe[TT];
 #pragma unroll
@@ -450,24 +547,341 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// k = 65..128: the matrix does not fit a wavefront's registers, so the two
+// per-matrix phases keep it in LDS (one 128-thread workgroup per matrix,
+// thread r owns row r); the QL recurrence is k_ql_log<128>.
+// ---------------------------------------------------------------------------
+
+// Sum over the 128 threads of a workgroup; every thread gets the same value
+// (wave sums, then the two partials in a fixed order).  `red` holds two slots
+// of two partials so that back-to-back calls need one barrier each.
+__device__ __forceinline__ double block2_sum(double v, double* red, int& slot) {
+  const double w = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[2 * slot + wv] = w;
+  __syncthreads();
+  const double r = red[2 * slot] + red[2 * slot + 1];
+  slot ^= 1;
+  return r;
+}
+
+__device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
+  const double w = wave_min(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[2 * slot + wv] = w;
+  __syncthreads();
+  const double r = fmin(red[2 * slot], red[2 * slot + 1]);
+  slot ^= 1;
+  return r;
+}
+
+// Householder tridiagonalisation with A in LDS; same outputs as k_tridiag.
+template <int KMAX>
+__global__ __launch_bounds__(KMAX) void k_tridiag_lds(EigParams p) {
+  static_assert(KMAX == 128, "two wavefronts per matrix");
+  constexpr int LD = KMAX + 1;
+  __shared__ double sA[KMAX * LD];
+  __shared__ double sv[KMAX], sw[KMAX];
+  __shared__ double se_s[KMAX + 1], st_s[KMAX + 1];
+  __shared__ double red[4];
+  const int r = threadIdx.x;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  const int k = p.k, kp = p.kp;
+  const double* __restrict__ Ag = p.A + al * (long long)kp * kp;
+  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const bool in_r = r < k;
+  for (int c = 0; c < KMAX; ++c)
+    sA[c * LD + r] = (in_r && c < k) ? Ag[(long long)c * kp + r] : 0.0;
+  int slot = 0;
+  __syncthreads();
+  for (int j = 0; j + 2 < k; ++j) {
+    const bool act = in_r && r > j;
+    const double x = act ? sA[j * LD + r] : 0.0;
+    const double x0 = sA[j * LD + j + 1];
+    const double rest = block2_sum((act && r > j + 1) ? x * x : 0.0, red, slot);
+    if (rest == 0.0) {
+      refl[(long long)j * kp + r] = 0.0;
+      if (r == 0) { se_s[j] = x0; st_s[j] = 0.0; }
+      continue;
+    }
+    const double nrm = sqrt(rest + x0 * x0);
+    const double alpha = (x0 >= 0.0) ? -nrm : nrm;
+    const double v0 = x0 - alpha;
+    const double tau = 2.0 / (rest + v0 * v0);
+    const double v = act ? ((r == j + 1) ? v0 : x) : 0.0;
+    sv[r] = v;
+    __syncthreads();
+    double pr0 = 0.0, pr1 = 0.0;
+    if (act) {
+      int c = j + 1;
+      for (; c + 1 < k; c += 2) {
+        pr0 = fma(sA[c * LD + r], sv[c], pr0);
+        pr1 = fma(sA[(c + 1) * LD + r], sv[c + 1], pr1);
+      }
+      if (c < k) pr0 = fma(sA[c * LD + r], sv[c], pr0);
+    }
+    const double pr = act ? (pr0 + pr1) * tau : 0.0;
+    const double K = 0.5 * tau * block2_sum(v * pr, red, slot);
+    const double wr = act ? pr - K * v : 0.0;
+    sw[r] = wr;
+    __syncthreads();
+    if (act)
+      for (int c = j + 1; c < k; ++c) sA[c * LD + r] -= v * sw[c] + wr * sv[c];
+    refl[(long long)j * kp + r] = v;
+    if (r == 0) { se_s[j] = alpha; st_s[j] = tau; }
+    __syncthreads();
+  }
+  __syncthreads();
+  double* __restrict__ dg = p.d + al * kp;
+  double* __restrict__ eg = p.e + al * kp;
+  double* __restrict__ tg = p.tau + al * kp;
+  dg[r] = in_r ? sA[r * LD + r] : 0.0;
+  double ev = 0.0, tv = 0.0;
+  if (r + 2 < k) { ev = se_s[r]; tv = st_s[r]; }
+  else if (r == k - 2) ev = sA[(k - 2) * LD + k - 1];
+  eg[r] = ev;
+  tg[r] = tv;
+}
+
+// Q = H_0 ... H_{k-3} and the logged rotations applied to it in LDS, then
+// overline_w, trans and the ensemble update (as k_apply, with the products
+// regrouped: T x = V diag(f) (V^T x) and overline_w . x = st . (V^T x)).
+// Every analysis is applied, flagged ones included: a rotation log that
+// overflowed or a recurrence that did not converge leaves an inexact basis,
+// and the flag is returned to the caller.
+template <int KMAX>
+__global__ __launch_bounds__(KMAX) void k_apply_lds(EigParams p) {
+  static_assert(KMAX == 128, "two wavefronts per matrix");
+  constexpr int LD = KMAX + 1;
+  constexpr int NV = 8;                              // state vectors per update step
+  __shared__ double sZ[KMAX * LD];                   // Z[r][c] at sZ[c * LD + r]
+  __shared__ __align__(16) double sbuf[2 * NV * KMAX];   // phases A/B: sq, scs; D: sx, sy
+  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX];
+  __shared__ double red[4 * NV];
+  const int r = threadIdx.x;
+  const int lane = r & 63, wv = r >> 6;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  const int k = p.k, kp = p.kp;
+  const bool in_r = r < k;
+  double* sq = sbuf;                                  // [2][KMAX]
+  double2* scs = reinterpret_cast<double2*>(sbuf + 2 * KMAX);   // [2][KMAX]
+
+  // ---- A. Z <- Z H_j, row r per thread ----
+  for (int c = 0; c < KMAX; ++c) sZ[c * LD + r] = (c == r) ? 1.0 : 0.0;
+  const double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + al * kp;
+  for (int j = 0; j + 2 < k; ++j) {
+    const int buf = j & 1;
+    sq[buf * KMAX + r] = refl[(long long)j * kp + r];
+    const double tau = tg[j];
+    __syncthreads();
+    if (tau == 0.0) continue;
+    const double* q = sq + buf * KMAX;
+    double d0 = 0.0, d1 = 0.0;
+    int c = j + 1;
+    for (; c + 1 < k; c += 2) {
+      d0 = fma(sZ[c * LD + r], q[c], d0);
+      d1 = fma(sZ[(c + 1) * LD + r], q[c + 1], d1);
+    }
+    if (c < k) d0 = fma(sZ[c * LD + r], q[c], d0);
+    const double dot = (d0 + d1) * tau;
+    for (c = j + 1; c < k; ++c) sZ[c * LD + r] = fma(-dot, q[c], sZ[c * LD + r]);
+  }
+
+  // ---- B. the logged QL rotations, chase by chase ----
+  const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
+  const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
+  const int nch = p.nchase[al];
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    const int4 h = hdr[ch];
+    const int lo = h.x, hi = h.y;
+    if (r < hi - lo) scs[buf * KMAX + r] = rot[h.z + r];
+    __syncthreads();
+    const double2* cs = scs + buf * KMAX;
+    double carry = sZ[hi * LD + r];
+    double zi = sZ[(hi - 1) * LD + r];
+    for (int i = hi - 1; i >= lo; --i) {
+      const double zn = (i > lo) ? sZ[(i - 1) * LD + r] : 0.0;
+      const double2 g = cs[hi - 1 - i];
+      sZ[(i + 1) * LD + r] = g.y * zi + g.x * carry;
+      carry = g.x * zi - g.y * carry;
+      zi = zn;
+    }
+    sZ[lo * LD + r] = carry;
+  }
+  __syncthreads();
+
+  // ---- C. eigenvalue functions, overline_w, trans ----
+  int slot = 0;
+  double lam = in_r ? p.lam[al * kp + r] : 0.0;
+  if (p.guard) {
+    const double mn = block2_min(in_r ? lam : 1e300, red, slot);
+    if (mn < 0.0) lam = lam - mn;
+  }
+  lam = lam * 1.0 + (double)(k - 1) / p.inflation;
+  sf[r] = in_r ? sqrt((double)(k - 1) / lam) : 0.0;
+  sg[r] = in_r ? 1.0 / lam : 0.0;
+  sb[r] = in_r ? p.b[al * kp + r] : 0.0;
+  __syncthreads();
+  {
+    // st_c = (V^T b)_c / lambda_c, thread r computes column c = r
+    double tb = 0.0;
+    for (int i = 0; i < k; ++i) tb = fma(sZ[r * LD + i], sb[i], tb);
+    st[r] = tb * sg[r];
+  }
+  __syncthreads();
+  double wbar = 0.0;
+  for (int c = 0; c < k; ++c) wbar = fma(sZ[c * LD + r], st[c], wbar);
+  sw[r] = in_r ? wbar : 0.0;
+  if (p.w_out && in_r) p.w_out[a * (long long)k + r] = wbar;
+  if (p.trans_out) {
+    // trans(f, e) = sum_c V[f][c] f_c V[e][c] + overline_w(f); thread f = r
+    double* tr = p.trans_out + a * (long long)k * k;
+    for (int e = 0; e < k; ++e) {
+      double t = 0.0;
+      for (int c = 0; c < k; ++c) t = fma(sZ[c * LD + r] * sf[c], sZ[c * LD + e], t);
+      if (in_r) tr[(long long)e * k + r] = t + wbar;
+    }
+  }
+  if (!p.xb) return;
+
+  // ---- D. ensemble update ----
+  double* sx = sbuf;                 // [NV][KMAX] background members
+  double* sy = sbuf + NV * KMAX;     // [NV][KMAX] (V^T x)_c
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
+  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  const long long lat_ext = p.g.nlat + 2 * p.halo;
+  int sk = 1, ek = p.nz;
+  if (p.vertical) {
+    sk = kc * p.cube_nz + 1;
+    ek = min(sk + p.cube_nz - 1, p.nz);
+  }
+  int rs[kMaxVars], rl[kMaxVars];
+  int nrun = 0, nvec = 0;
+  if (!p.vertical) {
+    rs[0] = 0; rl[0] = p.n; nrun = 1;
+  } else {
+    for (int vv = 0; vv < p.n_var; ++vv) {
+      const int start = p.var_start[vv] - 1;
+      if (p.var_lev[vv] == 1) {
+        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
+      } else {
+        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
+      }
+    }
+  }
+  for (int t = 0; t < nrun; ++t) nvec += rl[t];
+  auto sidx = [&](int v) {
+    int rem = v;
+    for (int t = 0; t < nrun; ++t) {
+      if (rem < rl[t]) return rs[t] + rem;
+      rem -= rl[t];
+    }
+    return -1;
+  };
+  for (int lo = si; lo <= ei; ++lo) {
+    for (int la = sj; la <= ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int v0 = 0; v0 < nvec; v0 += NV) {
+        __syncthreads();
+        for (int u = 0; u < NV; ++u) {
+          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+          sx[u * KMAX + r] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + r] : 0.0;
+        }
+        __syncthreads();
+        // y_u,c = (V^T x_u)_c, thread r computes column c = r
+        double y[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) y[u] = 0.0;
+        for (int i = 0; i < k; ++i) {
+          const double z = sZ[r * LD + i];
+#pragma unroll
+          for (int u = 0; u < NV; ++u) y[u] = fma(z, sx[u * KMAX + i], y[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < NV; ++u) sy[u * KMAX + r] = y[u];
+        __syncthreads();
+        // xa_u,e = sum_c V[e][c] f_c y_u,c + sum_c st_c y_u,c + axb_u
+        double o[NV], wd[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) { o[u] = 0.0; wd[u] = 0.0; }
+        for (int c = 0; c < k; ++c) {
+          const double zf = sZ[c * LD + r] * sf[c];
+          const double tc = st[c];
+#pragma unroll
+          for (int u = 0; u < NV; ++u) {
+            const double yc = sy[u * KMAX + c];
+            o[u] = fma(zf, yc, o[u]);
+            wd[u] = fma(tc, yc, wd[u]);
+          }
+        }
+        double xe[NV], part[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+          const double ab = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
+          xe[u] = in_r ? (o[u] + wd[u]) + ab : 0.0;
+          part[u] = wave_sum(xe[u]);
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < NV; ++u) red[u * 2 + wv] = part[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+          if (s_i < 0) continue;
+          const double mean = (red[2 * u] + red[2 * u + 1]) / k;
+          const long long o2 = off + s_i;
+          if (in_r) p.xa[o2 * k + r] = xe[u] - mean;
+          if (r == 0) p.axa[o2] = mean;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_eig(hipStream_t st, const EigParams& p) {
+hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
   if (p.n_an <= 0) return hipSuccess;
   const unsigned nb = (unsigned)p.n_an;
   const unsigned nq = (unsigned)((p.n_an + 63) / 64);
+  auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+  hipError_t e = mark(0);
+  if (e != hipSuccess) return e;
   if (p.k <= 32) {
     hipLaunchKernelGGL(k_tridiag<32>, dim3(nb), dim3(64), 0, st, p);
+    if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
+    if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
   } else if (p.k <= 64) {
     hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
+    if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
+    if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
+  } else if (p.k <= 128) {
+    hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
+    if ((e = mark(1)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
+    if ((e = mark(2)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);
   } else {
     return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return mark(3);
 }
 
 }  // namespace dida

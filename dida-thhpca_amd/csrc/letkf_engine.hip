@@ -95,8 +95,6 @@ struct EigScratch {
   std::vector<int> st_host;
 };
 
-constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
-constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 
 // bytes of split-eigensolver scratch per analysis
 size_t eig_bytes_per_analysis(int k) {
@@ -108,9 +106,13 @@ size_t eig_bytes_per_analysis(int k) {
 // Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
 // eigensolver (k_tridiag, k_ql_log, k_apply), then the single-kernel
 // transform for analyses the split path flagged; mode 1 runs only the single
-// kernel.
-int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode) {
-  if (mode == 1 || xp.k > 64) {
+// kernel.  k > 64 runs the LDS variant of the split path (k_tridiag_lds,
+// k_ql_log, k_apply_lds), which applies flagged analyses itself.
+// kev: null, or 4 events for the split kernels (recorded only if *split is set)
+int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
+                  hipEvent_t* kev = nullptr, bool* split = nullptr) {
+  if (split) *split = false;
+  if (mode == 1 && xp.k <= 64) {
     HIP_TRY(launch_transform(st, xp));
     return DIDA_OK;
   }
@@ -125,7 +127,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   RC_TRY(s.rot.ensure((size_t)16 * n * rot_cap));
   RC_TRY(s.hdr.ensure((size_t)16 * n * hdr_cap));
   RC_TRY(s.nchase.ensure(sizeof(int) * n));
-  EigParams ep;
+  EigParams ep{};
   ep.g = xp.g;
   ep.k = k;
   ep.kp = kp;
@@ -163,7 +165,9 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.axa = xp.axa;
   ep.trans_out = xp.trans_out;
   ep.w_out = xp.w_out;
-  HIP_TRY(launch_eig(st, ep));
+  HIP_TRY(launch_eig(st, ep, kev));
+  if (split) *split = true;
+  if (k > 64) return DIDA_OK;
   s.st_host.resize(n);
   HIP_TRY(hipMemcpyAsync(s.st_host.data(), xp.status + xp.a0, sizeof(int) * n,
                          hipMemcpyDeviceToHost, st));
@@ -274,7 +278,6 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   *out = nullptr;
   const dida_letkf_config& c = *cfg;
   if (c.k < 2 || c.k > 128) return set_error(DIDA_ERR_ARG, "k=%d outside 2..128", c.k);
-  if (c.k > 64) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: transform kernel supports k<=64", c.k);
   if (c.nlat < 1 || c.nlon < 1 || c.nz < 1 || c.n2d < 0 || c.n3d < 0 || c.n2d + c.n3d < 1)
     return set_error(DIDA_ERR_ARG, "bad grid/variable sizes");
   if (c.n2d + c.n3d > kMaxVars) return set_error(DIDA_ERR_ARG, "more than %d variables", kMaxVars);
@@ -522,7 +525,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
       for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
     }
   } guard{ev};
-  float ms_sel = 0, ms_prod = 0, ms_xf = 0;
+  float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[3] = {0, 0, 0};
   HIP_TRY(hipEventRecord(ev[0], st));
 
   Geom g;
@@ -603,7 +606,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     HIP_TRY(launch_select(st, sp, true));
     HIP_TRY(hipEventRecord(ev[1], st));
 
-    ProdParams pp;
+    ProdParams pp{};
     pp.g = g;
     pp.k = c.k;
     pp.kp = h->kp;
@@ -638,7 +641,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     HIP_TRY(launch_products(st, pp, c.precision));
     HIP_TRY(hipEventRecord(ev[2], st));
 
-    XformParams xp;
+    XformParams xp{};
     xp.g = g;
     xp.k = c.k;
     xp.kp = h->kp;
@@ -672,9 +675,17 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     xp.w_out = c.keep_transforms ? h->wout.as<double>() : nullptr;
     xp.status = h->status.as<int>();
     xp.list = nullptr;
-    RC_TRY(run_transform(st, h->eig, xp, c.transform_kernel));
+    bool split = false;
+    RC_TRY(run_transform(st, h->eig, xp, c.transform_kernel, ev + 4, &split));
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipEventSynchronize(ev[3]));
+    if (split) {
+      for (int i = 0; i < 3; ++i) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, ev[4 + i], ev[5 + i]));
+        ms_k[i] += t;
+      }
+    }
     float t1 = 0, t2 = 0, t3 = 0;
     HIP_TRY(hipEventElapsedTime(&t1, ev[0], ev[1]));
     HIP_TRY(hipEventElapsedTime(&t2, ev[1], ev[2]));
@@ -700,6 +711,9 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     stats->ms_products = ms_prod;
     stats->ms_transform = ms_xf;
     stats->ms_total = ms_sel + ms_prod + ms_xf;
+    stats->ms_tridiag = ms_k[0];
+    stats->ms_ql = ms_k[1];
+    stats->ms_apply = ms_k[2];
   }
   return flagged;
 }
@@ -758,7 +772,7 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
                             const float* rdiaginv, const double* dep, const double* rloc,
                             double* trans, double* overline_w, double inflation, int precision,
                             int* status) {
-  if (k < 2 || k > 64) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: supported 2..64", k);
+  if (k < 2 || k > 128) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: supported 2..128", k);
   if (n_an < 0 || !row_ptr || !trans || !overline_w)
     return set_error(DIDA_ERR_ARG, "bad arguments");
   if (precision < DIDA_PREC_FP64 || precision > DIDA_PREC_BF16X3)

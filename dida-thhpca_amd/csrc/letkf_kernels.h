@@ -5,6 +5,8 @@
 namespace dida {
 
 constexpr int kMaxVars = 16;
+constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
+constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 
 struct Geom {
   const float* glat;   // nlat, real(4) degrees
@@ -58,6 +60,8 @@ struct ProdParams {
   int h0;
   double* A;                 // batch-local, kp*kp per analysis
   double* b;                 // batch-local, kp per analysis
+  const int* order;          // optional processing order (batch-local analysis offsets)
+  int n_blocks;              // set by the launcher
 };
 
 struct XformParams {
@@ -133,6 +137,7 @@ hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
-hipError_t launch_eig(hipStream_t st, const EigParams& p);
+// ev: null, or 4 events recorded before k_tridiag, k_ql_log, k_apply and after
+hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev = nullptr);
 
 }  // namespace dida

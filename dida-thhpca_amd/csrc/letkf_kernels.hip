@@ -196,6 +196,14 @@ __device__ __forceinline__ double row_weight(const ProdParams& p, int row, doubl
   return w;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs; remap them so that each
+// XCD walks a contiguous range of logical blocks (neighbouring analyses share
+// obs rows in that XCD's L2).  The grid is padded to a multiple of 8.
+__device__ __forceinline__ int xcd_block(int n_blocks) {
+  const int per = (n_blocks + 7) / 8;
+  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
 template <int T>
 struct PairTable {
   static constexpr int N = T * (T + 1) / 2;
@@ -223,10 +231,10 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
   double bacc[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) bacc[t] = 0.0;
-  for (long long base = r0; base < r1; base += 4) {
-    const long long rr = base + q;
-    double w = 0.0, wd = 0.0;
-    double y[T];
+  // rows are loaded one group (4 rows) ahead of the MFMAs that use them
+  auto load = [&](long long rr, double* y, double& w, double& wd) {
+    w = 0.0;
+    wd = 0.0;
     if (rr < r1) {
       const int row = p.row_idx[rr];
       w = row_weight(p, row, p.row_gc[rr], vc);
@@ -241,6 +249,15 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
 #pragma unroll
       for (int t = 0; t < T; ++t) y[t] = 0.0;
     }
+  };
+  double yn[T], wn, wdn;
+  load(r0 + q, yn, wn, wdn);
+  for (long long base = r0; base < r1; base += 4) {
+    double y[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) y[t] = yn[t];
+    const double w = wn, wd = wdn;
+    load(base + 4 + q, yn, wn, wdn);
 #pragma unroll
     for (int x = 0; x < NP; ++x) {
       const int I = PT::I(P0 + x), J = PT::J(P0 + x);
@@ -282,9 +299,10 @@ __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int apb = (blockDim.x >> 6) / WPA;
-  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (a >= p.a0 + p.n_an) return;
+  if (li >= p.n_an) return;
+  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
@@ -312,11 +330,9 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
   double bacc[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) bacc[t] = 0.0;
-  for (long long base = r0; base < r1; base += 4) {
-    const long long rr = base + q;
-    float wf = 0.f;
-    double wd = 0.0;
-    float y[T];
+  auto load = [&](long long rr, float* y, float& wf, double& wd) {
+    wf = 0.f;
+    wd = 0.0;
     if (rr < r1) {
       const int row = p.row_idx[rr];
       const double w = row_weight(p, row, p.row_gc[rr], vc);
@@ -332,6 +348,17 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
 #pragma unroll
       for (int t = 0; t < T; ++t) y[t] = 0.f;
     }
+  };
+  float yn[T], wfn;
+  double wdn;
+  load(r0 + q, yn, wfn, wdn);
+  for (long long base = r0; base < r1; base += 4) {
+    float y[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) y[t] = yn[t];
+    const float wf = wfn;
+    const double wd = wdn;
+    load(base + 4 + q, yn, wfn, wdn);
 #pragma unroll
     for (int x = 0; x < NP; ++x) {
       const int I = PT::I(P0 + x), J = PT::J(P0 + x);
@@ -373,9 +400,10 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int apb = (blockDim.x >> 6) / WPA;
-  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (a >= p.a0 + p.n_an) return;
+  if (li >= p.n_an) return;
+  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
@@ -470,9 +498,10 @@ __global__ __launch_bounds__(256) void k_products_bf16(ProdParams p) {
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int apb = (blockDim.x >> 6) / WPA;
-  const long long a = p.a0 + (long long)blockIdx.x * apb + wave / WPA;
+  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (a >= p.a0 + p.n_an) return;
+  if (li >= p.n_an) return;
+  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
@@ -826,9 +855,12 @@ static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int pr
   constexpr int APB = 4 / WPA;
   const long long blocks = (p.n_an + APB - 1) / APB;
   if (blocks <= 0) return hipSuccess;
-  if (precision == 0) hipLaunchKernelGGL(k_products_f64<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
-  else if (precision == 1) hipLaunchKernelGGL(k_products_f32<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
-  else hipLaunchKernelGGL(k_products_bf16<KP>, dim3((unsigned)blocks), dim3(256), 0, st, p);
+  ProdParams q = p;
+  q.n_blocks = (int)blocks;
+  const unsigned grid = (unsigned)(((blocks + 7) / 8) * 8);
+  if (precision == 0) hipLaunchKernelGGL(k_products_f64<KP>, dim3(grid), dim3(256), 0, st, q);
+  else if (precision == 1) hipLaunchKernelGGL(k_products_f32<KP>, dim3(grid), dim3(256), 0, st, q);
+  else hipLaunchKernelGGL(k_products_bf16<KP>, dim3(grid), dim3(256), 0, st, q);
   return hipGetLastError();
 }
 

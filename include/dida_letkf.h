@@ -104,6 +104,9 @@ typedef struct dida_letkf_stats {
     double ms_products;     /* device time, obs-space products */
     double ms_transform;    /* device time, eigensolve + transform + update */
     double ms_total;        /* device time of the whole call */
+    double ms_tridiag;      /* of ms_transform: Householder tridiagonalisation kernel */
+    double ms_ql;           /* of ms_transform: QL recurrence kernel */
+    double ms_apply;        /* of ms_transform: eigenvector, transform and update kernel */
 } dida_letkf_stats;
 
 typedef struct dida_letkf_handle dida_letkf_handle;

@@ -60,7 +60,8 @@ def _case(k, p, seed):
 
 
 @pytest.mark.parametrize("k,p", [(2, 0), (2, 1), (3, 17), (20, 0), (20, 1), (20, 49), (32, 5),
-                                 (32, 300), (64, 0), (64, 63), (64, 1000)])
+                                 (32, 300), (64, 0), (64, 63), (64, 1000), (65, 30), (100, 2000),
+                                 (128, 0), (128, 1), (128, 127), (128, 3000)])
 def test_letkf_core_dropin(k, p):
     import orc
     from dida_letkf import engine
@@ -266,3 +267,38 @@ def test_transform_paths_match_oracle(tk):
         gpu = engine.letkf_compute_nooverlap(inst, keep_transforms=True, transform_kernel=tk)
         ref = orc.letkf_compute(inst, want_trans=True)
         _assert_parity(gpu, ref, what=f"tk={tk} k={inst.cfg.k}")
+
+
+# ---------------------------------------------------------------------------
+# k = 65..128 (LDS-resident tridiagonalisation and update)
+# ---------------------------------------------------------------------------
+
+def test_config3_shape_k128_reduced():
+    """configs[3] shape at k=128 (clustered obs, log-uniform sigma) on a reduced grid."""
+    from dida_letkf import synth
+    inst = synth.generate("c3", seed=23, k=128, nlat=12, nlon=24, nz=3, n_obs=15000,
+                          local_dist=900.0)
+    gpu, ref = _run_both(inst)
+    assert gpu["stats"]["max_rows"] > 128
+    _assert_parity(gpu, ref, what="c3-k128-reduced")
+
+
+def test_vertical_k100_padded():
+    """k=100 (padded to 128 lanes) on the per-level vertical loop."""
+    from dida_letkf import synth
+    inst = synth.generate("c1", seed=24, k=100, nlat=8, nlon=16, nz=4, n_obs=2500)
+    gpu, ref = _run_both(inst)
+    _assert_parity(gpu, ref, what="c1-k100")
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-4)])
+def test_reduced_precision_k128(prec, tol):
+    import orc
+    from dida_letkf import synth, engine
+    inst = synth.generate("c3", seed=25, k=128, nlat=12, nlon=24, nz=3, n_obs=15000,
+                          local_dist=900.0)
+    gpu = engine.letkf_compute_nooverlap(inst, precision=prec)
+    ref = orc.letkf_compute(inst)
+    err = increment_error(inst, gpu["xa"], gpu["axa"], ref["xa"], ref["axa"])
+    print("k=128 precision", prec, "increment rel RMS err per var", err)
+    assert max(err.values()) < tol

@@ -299,7 +299,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->cfg = c;
   h->cfg.hyai = nullptr;
   h->cfg.hybi = nullptr;
-  if (c.vert_dist > 0.0 && c.vcoord_mode == DIDA_VCOORD_LNP) {
+  if (c.hyai && c.hybi) {
     h->hyai.assign(c.hyai, c.hyai + c.nz + 1);
     h->hybi.assign(c.hybi, c.hybi + c.nz + 1);
   }
@@ -932,7 +932,70 @@ size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows)
   b += 4 * n_an;
   if (c->keep_transforms) b += n_an * (8 * k * k + 8 * k);
   b += n_state * (k + 1) * 8;                                       // serving ping-pong state
+  b += nr * (k + 1) * 8;                                            // serving operator Yb, d
   return b + (16 << 20);
+}
+
+int dida_letkf_fill_halo_device(dida_letkf_handle* h, double* x, double* ax) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  const auto& c = h->cfg;
+  if (x) HIP_TRY(launch_fill_halo(h->stream, x, c.nlat, c.nlon, c.halo, (long long)h->n * c.k));
+  if (ax) HIP_TRY(launch_fill_halo(h->stream, ax, c.nlat, c.nlon, c.halo, (long long)h->n));
+  return DIDA_OK;
+}
+
+int dida_letkf_obs_operator_device(dida_letkf_handle* h, const double* xb, const double* axb,
+                                   int n_sites, const double* site_lat, const double* site_lon,
+                                   const int* site_row_ptr, int n_rows, const int* row_var,
+                                   const double* row_ph, const double* yo, double* yb,
+                                   double* dep) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  if (n_sites < 0 || n_rows < 0) return set_error(DIDA_ERR_ARG, "negative sizes");
+  if (n_sites > 0 && (!xb || !axb || !site_lat || !site_lon || !site_row_ptr))
+    return set_error(DIDA_ERR_ARG, "null argument");
+  if (n_rows > 0 && (!row_var || !row_ph || !yo || !yb || !dep))
+    return set_error(DIDA_ERR_ARG, "null argument");
+  if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
+  const auto& c = h->cfg;
+  if (c.halo < 1) return set_error(DIDA_ERR_ARG, "the observation operator needs halo >= 1");
+  if (h->hyai.empty()) return set_error(DIDA_ERR_ARG, "the observation operator needs hyai/hybi");
+  if (c.ps_index < 0 || c.ps_index >= h->n) return set_error(DIDA_ERR_ARG, "bad ps_index");
+  ObsOpParams p{};
+  p.k = c.k;
+  p.n = h->n;
+  p.nz = c.nz;
+  p.nlat = c.nlat;
+  p.nlon = c.nlon;
+  p.halo = c.halo;
+  p.ps_index = c.ps_index;
+  p.n_var = c.n2d + c.n3d;
+  p.n_rows = n_rows;
+  int cur = 1;
+  for (int v = 0; v < c.n2d; ++v) { p.var_start[v] = cur; p.var_lev[v] = 1; cur += 1; }
+  for (int v = 0; v < c.n3d; ++v) {
+    p.var_start[c.n2d + v] = cur;
+    p.var_lev[c.n2d + v] = c.nz;
+    cur += c.nz;
+  }
+  p.glat = h->glat.as<float>();
+  p.glon = h->glon.as<float>();
+  p.hyai = h->d_hyai.as<double>();
+  p.hybi = h->d_hybi.as<double>();
+  p.p0 = c.p0;
+  p.ptop = c.ptop;
+  p.xb = xb;
+  p.axb = axb;
+  p.n_sites = n_sites;
+  p.site_lat = site_lat;
+  p.site_lon = site_lon;
+  p.site_row_ptr = site_row_ptr;
+  p.row_var = row_var;
+  p.row_ph = row_ph;
+  p.yo = yo;
+  p.yb = yb;
+  p.dep = dep;
+  HIP_TRY(launch_obs_operator(h->stream, p));
+  return DIDA_OK;
 }
 
 int dida_letkf_epilogue_device(dida_letkf_handle* h, const double* xb, const double* axb,

@@ -114,6 +114,29 @@ struct EigParams {
   double* w_out;            // optional, global index
 };
 
+// observation operator + centring (gen_obs_struct)
+struct ObsOpParams {
+  int k, n, nz, nlat, nlon, halo, ps_index, n_var, n_rows;
+  int var_start[kMaxVars];   // 1-based
+  int var_lev[kMaxVars];
+  const float* glat;         // nlat (closed by 90)
+  const float* glon;         // nlon (closed by 360)
+  const double* hyai;        // nz + 1
+  const double* hybi;
+  double p0, ptop;
+  const double* xb;
+  const double* axb;
+  int n_sites;
+  const double* site_lat;
+  const double* site_lon;
+  const int* site_row_ptr;
+  const int* row_var;        // 0-based DA variable
+  const double* row_ph;      // level pressure (Pa)
+  const double* yo;
+  double* yb;                // n_rows x k
+  double* dep;
+};
+
 struct EpiParams {
   int k, n, nlat, nlon, halo;
   const float* glat;
@@ -138,6 +161,9 @@ hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
 // ev: null, or 4 events recorded before k_tridiag, k_ql_log, k_apply and after
+hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p);
+// halo of a [lon + 2h][lat + 2h][n] field with `inner` values per element
+hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);
 hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev = nullptr);
 
 }  // namespace dida

@@ -33,6 +33,8 @@ struct Job {
   long long id = 0;
   dida_serve_job spec{};
   std::vector<float> glat, glon;
+  std::vector<double> hyai, hybi;      // the config's pointers are redirected here
+  bool op_mode = false;                // some cycle computes Yb, d on the device
   std::vector<dida_serve_obs> obs;
   dida_serve_status st{};
   std::atomic<bool> cancel{false};
@@ -63,6 +65,8 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
   int rc = dida_letkf_create(&sp.cfg, &h);
   double* xt = nullptr;
   double* at = nullptr;
+  double* yb_op = nullptr;
+  double* dep_op = nullptr;
   long long n_an = 0;
   const size_t n_state = (size_t)(sp.cfg.nlat + 2 * sp.cfg.halo) * (sp.cfg.nlon + 2 * sp.cfg.halo) *
                          (sp.cfg.n2d + sp.cfg.n3d * sp.cfg.nz);
@@ -70,6 +74,11 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
   if (rc == DIDA_OK) rc = dida_letkf_set_grid(h, j->glat.data(), j->glon.data());
   if (rc == DIDA_OK && sp.n_cycles > 1) {
     if (hipMalloc(&xt, n_state * k * 8) != hipSuccess || hipMalloc(&at, n_state * 8) != hipSuccess)
+      rc = DIDA_ERR_NOMEM;
+  }
+  if (rc == DIDA_OK && j->op_mode) {
+    const size_t nr = (size_t)std::max(j->max_rows, 1);
+    if (hipMalloc(&yb_op, nr * k * 8) != hipSuccess || hipMalloc(&dep_op, nr * 8) != hipSuccess)
       rc = DIDA_ERR_NOMEM;
   }
   // ping-pong: cycle c reads (bx, ba) and writes (ox, oa); the last cycle
@@ -85,12 +94,28 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
     double* ox = (left % 2 == 0) ? sp.xa : xt;
     double* oa = (left % 2 == 0) ? sp.axa : at;
     const dida_serve_obs& o = j->obs[c];
+    const double* yb = o.yb;
+    const double* dep = o.dep;
+    if (!yb) {
+      // H(x) and centring from this cycle's background, on the job's stream
+      int r = dida_letkf_obs_operator_device(h, bx, ba, o.n_sites, o.site_lat, o.site_lon,
+                                             o.site_row_ptr, o.n_rows, o.row_var, o.row_ph, o.yo,
+                                             yb_op, dep_op);
+      if (r < 0) { rc = r; break; }
+      yb = yb_op;
+      dep = dep_op;
+    }
     int r = dida_letkf_set_obs_device(h, o.n_sites, o.site_lat, o.site_lon, o.site_row_ptr,
-                                      o.n_rows, o.yb, o.dep, o.rinv, o.vcoord);
+                                      o.n_rows, yb, dep, o.rinv, o.vcoord);
     if (r < 0) { rc = r; break; }
     dida_letkf_stats stt;
     r = dida_letkf_analyze_device(h, bx, ba, ox, oa, &stt);
     if (r < 0) { rc = r; break; }
+    if (j->op_mode && sp.cfg.halo > 0) {
+      // the next cycle's observation operator reads the halo
+      const int rh = dida_letkf_fill_halo_device(h, ox, oa);
+      if (rh < 0) { rc = rh; break; }
+    }
     rc = std::max(rc, r);
     n_an += stt.n_analyses;
     bx = ox;
@@ -99,6 +124,8 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
   if (h) dida_letkf_destroy(h);
   if (xt) (void)hipFree(xt);
   if (at) (void)hipFree(at);
+  if (yb_op) (void)hipFree(yb_op);
+  if (dep_op) (void)hipFree(dep_op);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     j->st.t_end = now_s();
@@ -173,13 +200,27 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
     return DIDA_ERR_ARG;
   auto j = std::make_shared<Job>();
   j->spec = *job;
-  j->spec.cfg.hyai = nullptr;   // LNP serving jobs are not supported
-  j->spec.cfg.hybi = nullptr;
-  if (job->cfg.vert_dist > 0.0 && job->cfg.vcoord_mode != DIDA_VCOORD_LEVEL) return DIDA_ERR_UNSUPPORTED;
+  if (job->cfg.hyai && job->cfg.hybi) {
+    j->hyai.assign(job->cfg.hyai, job->cfg.hyai + job->cfg.nz + 1);
+    j->hybi.assign(job->cfg.hybi, job->cfg.hybi + job->cfg.nz + 1);
+    j->spec.cfg.hyai = j->hyai.data();
+    j->spec.cfg.hybi = j->hybi.data();
+  } else {
+    j->spec.cfg.hyai = nullptr;
+    j->spec.cfg.hybi = nullptr;
+  }
   j->glat.assign(job->grid_lat, job->grid_lat + job->cfg.nlat);
   j->glon.assign(job->grid_lon, job->grid_lon + job->cfg.nlon);
   j->obs.assign(job->obs, job->obs + job->n_cycles);
   for (const auto& o : j->obs) {
+    if (o.n_sites < 0 || o.n_rows < 0) return DIDA_ERR_ARG;
+    if (!o.yb) {
+      if (!o.row_var || !o.row_ph || !o.yo) return DIDA_ERR_ARG;
+      if (job->cfg.halo < 1 || j->hyai.empty()) return DIDA_ERR_ARG;
+      j->op_mode = true;
+    } else if (!o.dep) {
+      return DIDA_ERR_ARG;
+    }
     j->max_sites = std::max(j->max_sites, o.n_sites);
     j->max_rows = std::max(j->max_rows, o.n_rows);
   }

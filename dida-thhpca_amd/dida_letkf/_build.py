@@ -11,7 +11,7 @@ REPO_ROOT = os.path.dirname(PKG_ROOT)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libdida_letkf.so")
-SOURCES = ["letkf_kernels.hip", "letkf_eig.hip", "letkf_engine.hip", "letkf_serve.hip"]
+SOURCES = ["letkf_kernels.hip", "letkf_eig.hip", "letkf_obsop.hip", "letkf_engine.hip", "letkf_serve.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

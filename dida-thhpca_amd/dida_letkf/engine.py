@@ -48,7 +48,8 @@ class ServeConfig(C.Structure):
 class ServeObs(C.Structure):
     _fields_ = [("n_sites", C.c_int), ("site_lat", _vp), ("site_lon", _vp),
                 ("site_row_ptr", _vp), ("n_rows", C.c_int), ("yb", _vp), ("dep", _vp),
-                ("rinv", _vp), ("vcoord", _vp)]
+                ("rinv", _vp), ("vcoord", _vp), ("row_var", _vp), ("row_ph", _vp),
+                ("yo", _vp)]
 
 
 class ServeJob(C.Structure):
@@ -88,7 +89,7 @@ EXPORTS = [
     "letkf_core_", "dida_letkf_epilogue_device", "dida_letkf_last_error",
     "dida_letkf_abi_version", "dida_serve_create", "dida_serve_submit", "dida_serve_poll",
     "dida_serve_wait", "dida_serve_cancel", "dida_serve_destroy", "dida_serve_now",
-    "dida_letkf_job_bytes",
+    "dida_letkf_job_bytes", "dida_letkf_obs_operator_device", "dida_letkf_fill_halo_device",
 ]
 
 _lib = None
@@ -320,12 +321,47 @@ class LetkfEngine:
                                                _p(nob, _ip)), "get_transforms")
         return np.transpose(trans, (0, 2, 1)).copy(), w, nob
 
+    def obs_operator_device(self, xb_ptr, axb_ptr, n_sites, lat_ptr, lon_ptr, row_ptr_ptr,
+                            n_rows, row_var_ptr, row_ph_ptr, yo_ptr, yb_ptr, dep_ptr):
+        """H(x) and Yb/d centring (gen_obs_struct) on device arrays."""
+        _check(lib().dida_letkf_obs_operator_device(
+            self.h, _vp(xb_ptr), _vp(axb_ptr), int(n_sites), _vp(lat_ptr), _vp(lon_ptr),
+            _vp(row_ptr_ptr), int(n_rows), _vp(row_var_ptr), _vp(row_ph_ptr), _vp(yo_ptr),
+            _vp(yb_ptr), _vp(dep_ptr)), "obs_operator")
+
+    def fill_halo_device(self, x_ptr, ax_ptr):
+        _check(lib().dida_letkf_fill_halo_device(self.h, _vp(x_ptr), _vp(ax_ptr)), "fill_halo")
+
     def epilogue_device(self, xb_ptr, axb_ptr, xa_ptr, axa_ptr, polar_lat, gamma1, gamma2,
                         xa_out_ptr, sb_ptr, sa_ptr):
         _check(lib().dida_letkf_epilogue_device(self.h, _vp(xb_ptr), _vp(axb_ptr), _vp(xa_ptr),
                                                 _vp(axa_ptr), float(polar_lat), float(gamma1),
                                                 float(gamma2), _vp(xa_out_ptr), _vp(sb_ptr),
                                                 _vp(sa_ptr)), "epilogue")
+
+
+def obs_operator(case):
+    """Convenience: the device observation operator on a synth.ObsOpCase
+    (inputs copied to the GPU).  Returns (yb [n_rows, k], dep [n_rows])."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt))).to(dev)
+    inst = case.inst
+    cfg = inst.cfg
+    nr = int(case.site_row_ptr[-1])
+    xb, axb = t(inst.xb), t(inst.axb)
+    la, lo, rp = t(case.site_lat, np.float64), t(case.site_lon, np.float64), t(case.site_row_ptr, np.int32)
+    rv, rph, yo = t(case.row_var, np.int32), t(case.row_ph, np.float64), t(case.yo, np.float64)
+    yb = torch.empty((max(nr, 1), cfg.k), dtype=torch.float64, device=dev)
+    dep = torch.empty(max(nr, 1), dtype=torch.float64, device=dev)
+    with LetkfEngine(cfg) as eng:
+        eng.set_grid(inst.grid_lat, inst.grid_lon)
+        eng.set_stream(torch.cuda.current_stream().cuda_stream)
+        eng.obs_operator_device(xb.data_ptr(), axb.data_ptr(), case.site_lat.size, la.data_ptr(),
+                                lo.data_ptr(), rp.data_ptr(), nr, rv.data_ptr(), rph.data_ptr(),
+                                yo.data_ptr(), yb.data_ptr(), dep.data_ptr())
+        torch.cuda.synchronize()
+    return yb[:nr].cpu().numpy(), dep[:nr].cpu().numpy()
 
 
 def letkf_compute_nooverlap(inst, precision: Optional[int] = None, keep_transforms=False,
@@ -366,7 +402,8 @@ class Server:
     def submit(self, cfg: LetkfConfig, grid_lat, grid_lon, xb_ptr, axb_ptr, obs_cycles,
                xa_ptr, axa_ptr) -> int:
         """obs_cycles: list of dicts with device pointers n_sites, site_lat, site_lon,
-        site_row_ptr, n_rows, yb, dep, rinv, vcoord."""
+        site_row_ptr, n_rows, rinv, vcoord and either yb, dep or (operator mode)
+        row_var, row_ph, yo."""
         job = ServeJob()
         job.cfg = to_c_config(cfg)
         la = np.ascontiguousarray(grid_lat, np.float32)
@@ -375,8 +412,9 @@ class Server:
         for i, o in enumerate(obs_cycles):
             arr[i] = ServeObs(n_sites=int(o["n_sites"]), site_lat=o["site_lat"],
                               site_lon=o["site_lon"], site_row_ptr=o.get("site_row_ptr"),
-                              n_rows=int(o["n_rows"]), yb=o["yb"], dep=o["dep"],
-                              rinv=o["rinv"], vcoord=o.get("vcoord"))
+                              n_rows=int(o["n_rows"]), yb=o.get("yb"), dep=o.get("dep"),
+                              rinv=o["rinv"], vcoord=o.get("vcoord"), row_var=o.get("row_var"),
+                              row_ph=o.get("row_ph"), yo=o.get("yo"))
         job.grid_lat, job.grid_lon = _p(la, _fp), _p(lo, _fp)
         job.xb, job.axb, job.xa, job.axa = xb_ptr, axb_ptr, xa_ptr, axa_ptr
         job.n_cycles = len(obs_cycles)

@@ -383,3 +383,101 @@ def generate(name: str, seed: int = 1, k: Optional[int] = None, precision: int =
 
 def full_members(inst: Instance) -> np.ndarray:
     return inst.xb + inst.axb[..., None]
+
+
+# --------------------------------------------------------------------------
+# observation-operator cases (gen_obs_struct inputs)
+# --------------------------------------------------------------------------
+
+@dataclass
+class ObsOpCase:
+    """Inputs of the observation operator: an Instance whose xb/axb carry a
+    filled halo of 1 and a hybrid vertical coordinate (cfg.hyai/hybi/p0/ptop,
+    ps at state index cfg.ps_index), obs sites with level-major rows, the
+    0-based DA variable and level pressure (Pa) of each row, and the observed
+    values yo."""
+    inst: Instance
+    site_lat: np.ndarray
+    site_lon: np.ndarray
+    site_row_ptr: np.ndarray
+    row_var: np.ndarray
+    row_ph: np.ndarray
+    yo: np.ndarray
+
+
+def hybrid_coefficients(nz: int, p0: float = 1.0e5, ptop: float = 200.0):
+    """Interface coefficients (nz + 1, top to bottom) of a hybrid sigma-pressure
+    coordinate: pure pressure near the top, terrain following near the ground."""
+    eta = np.linspace(0.0, 1.0, nz + 1)
+    hybi = eta ** 2
+    hyai = eta - hybi
+    return hyai, hybi, p0, ptop
+
+
+def fill_halo(a: np.ndarray, h: int) -> np.ndarray:
+    """Halo of a [lon + 2h, lat + 2h, ...] array: periodic in longitude; past
+    the poles, the nearest interior row."""
+    nlon = a.shape[0] - 2 * h
+    a[:h] = a[nlon:nlon + h]
+    a[nlon + h:] = a[h:2 * h]
+    a[:, :h] = a[:, h:h + 1]
+    a[:, a.shape[1] - h:] = a[:, a.shape[1] - h - 1:a.shape[1] - h]
+    return a
+
+
+def obs_operator_case(name: str = "c2", seed: int = 1, k: Optional[int] = None,
+                      nlat: int = 16, nlon: int = 32, nz: int = 6, n_obs: int = 2000,
+                      n_stations: int = 20, n_ps: int = 100) -> ObsOpCase:
+    """A seeded operator case on config `name`'s variables (ps + 3-D u, v, T):
+    single-level obs at random pressures, station columns at every level of a
+    sorted pressure profile (all 3-D variables per level), and surface-pressure
+    obs.  Positions follow the uniform mix, including the closing cells near
+    lat = 90 and lon = 360."""
+    inst = generate(name, seed=seed, k=k, nlat=nlat, nlon=nlon, nz=nz, n_obs=8, halo=1)
+    cfg = inst.cfg
+    hyai, hybi, p0, ptop = hybrid_coefficients(cfg.nz)
+    cfg.hyai, cfg.hybi, cfg.p0, cfg.ptop = list(hyai), list(hybi), p0, ptop
+    cfg.ps_index = 0
+    rng = _rng(seed, 7)
+    # ps: ~1e5 Pa mean with a smooth pattern, member spread ~300 Pa
+    h = cfg.halo
+    lat2 = np.deg2rad(inst.grid_lat)[None, :]
+    lon2 = np.deg2rad(inst.grid_lon)[:, None]
+    inst.axb[h:-h, h:-h, 0] = 1.0e5 - 3000.0 * np.sin(lat2) ** 2 + 800.0 * np.cos(2 * lon2) * np.cos(lat2)
+    inst.xb[h:-h, h:-h, 0, :] *= 300.0
+    fill_halo(inst.xb, h)
+    fill_halo(inst.axb, h)
+    three_d = [v for v in range(cfg.n_var) if cfg.var_lev()[v] > 1]
+    two_d = [v for v in range(cfg.n_var) if cfg.var_lev()[v] == 1]
+    # sites: single-level obs, station columns, ps obs
+    la1, lo1 = _uniform_sphere(rng, n_obs)
+    las, los = _uniform_sphere(rng, n_stations)
+    lap, lop = _uniform_sphere(rng, n_ps)
+    # a few sites in the closing cells (north of the last grid latitude, east of the last longitude)
+    m = min(4, n_obs)
+    la1[:m] = [89.9, 89.7, -89.9, 10.0][:m]
+    lo1[:m] = [359.9, 0.1, 359.95, 359.99][:m]
+    row_var, row_ph, rows = [], [], []
+    for i in range(n_obs):
+        row_var.append(three_d[rng.integers(0, len(three_d))])
+        row_ph.append(float(np.exp(rng.uniform(np.log(ptop * 2), np.log(1.05e5)))))
+        rows.append(1)
+    for i in range(n_stations):
+        prof = np.sort(np.exp(rng.uniform(np.log(1000.0), np.log(1.0e5), cfg.nz)))
+        for pl in prof:
+            for v in three_d:
+                row_var.append(v)
+                row_ph.append(float(pl))
+        rows.append(cfg.nz * len(three_d))
+    for i in range(n_ps):
+        row_var.append(two_d[0] if two_d else three_d[0])
+        row_ph.append(1.0e5)
+        rows.append(1)
+    site_lat = np.concatenate([la1, las, lap])
+    site_lon = np.concatenate([lo1, los, lop])
+    rp = np.zeros(len(rows) + 1, np.int32)
+    np.cumsum(rows, out=rp[1:])
+    yo = rng.standard_normal(int(rp[-1]))
+    return ObsOpCase(inst=inst, site_lat=site_lat, site_lon=site_lon, site_row_ptr=rp,
+                     row_var=np.asarray(row_var, np.int32), row_ph=np.asarray(row_ph),
+                     yo=yo)

@@ -184,6 +184,31 @@ void letkf_core_(const int *nanals, const int *nobsl, const double *hxens_in,
                  const float *rdiaginv, const double *dep, const double *rloc, double *trans,
                  double *overline_w, const double *inflation_factor, const int *pflag);
 
+/* Observation operator and Yb/d centring of gen_obs_struct
+ * (letkf_obs.F90:16-187: find_obs_grid :192-248, ps2ph_3d_interp and
+ * obs_interpolation_3d_2 obs_make.F90:233-381, centring :158-170) on device
+ * arrays, on the handle's stream.  Needs set_grid, halo >= 1, hyai/hybi
+ * (nz + 1 hybrid interface coefficients) in the config, and ps at state index
+ * ps_index.  Sites: lat/lon in degrees and site_row_ptr (n_sites + 1); the
+ * rows of a site are level-major (all variables of one level together, levels
+ * in the order of the obs profile) with row_var (0-based DA variable) and
+ * row_ph (level pressure, Pa); yo holds the observed values.  Writes
+ * yb (n_rows x k, member fastest) = H(x_e) - mean_e H(x_e) and
+ * dep (n_rows) = yo - mean_e H(x_e), the inputs of set_obs_device.  A row
+ * whose variable is out of range gets NaN. */
+int dida_letkf_obs_operator_device(dida_letkf_handle *h, const double *xb, const double *axb,
+                                   int n_sites, const double *site_lat, const double *site_lon,
+                                   const int *site_row_ptr, int n_rows, const int *row_var,
+                                   const double *row_ph, const double *yo, double *yb,
+                                   double *dep);
+
+/* Fills the halo of a state on device arrays (xb layout, and the mean with
+ * the axb layout; either may be NULL): periodic in longitude, the nearest
+ * interior latitude row past either pole.  The reference gets its halo from
+ * the model's halo exchange between cycles; device-resident cycling uses this
+ * in its place.  Interior entries are not changed. */
+int dida_letkf_fill_halo_device(dida_letkf_handle *h, double *x, double *ax);
+
 /* Caller epilogue of run_da_parallel (module_da_parallel.F90:219-261) on
  * device arrays: polar mask, gamma1/gamma2 blend and spreads.  xa_out has the
  * xb layout (halo entries set to 0); spread_b/spread_a are (n, nlat, nlon).
@@ -212,13 +237,19 @@ typedef struct dida_serve_config {
     int max_active;           /* jobs running at once (one stream each), >= 1 */
 } dida_serve_config;
 
-/* one cycle's obs set; pointers are device memory (as dida_letkf_set_obs_device) */
+/* one cycle's obs set; pointers are device memory (as dida_letkf_set_obs_device).
+ * Either yb and dep are given, or yb is NULL and row_var, row_ph and yo are:
+ * then the server computes yb and dep from the cycle's background with
+ * dida_letkf_obs_operator_device (the background of cycle c > 0 is the
+ * analysis of cycle c - 1, still on the device). */
 typedef struct dida_serve_obs {
     int n_sites;
     const double *site_lat, *site_lon;
     const int *site_row_ptr;
     int n_rows;
     const double *yb, *dep, *rinv, *vcoord;
+    const int *row_var;
+    const double *row_ph, *yo;
 } dida_serve_obs;
 
 typedef struct dida_serve_job {

@@ -833,3 +833,111 @@ void orc_epilogue(const orc_problem *pb, const double *xb, const double *axb, co
             }
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* observation operator (gen_obs_struct, letkf_obs.F90:16-187)               */
+/* ------------------------------------------------------------------------ */
+
+/* find_obs_grid (letkf_obs.F90:192-248): first 1-based index with
+ * grid >= value over the closed grid (n + 1 entries: the last is 90 / 360);
+ * the weight of the point before it, with the spacing taken in real(4)
+ * (grid_lat(tx) - grid_lat(tx-1) is a real(4) expression, :234). */
+static void orc_find_obs_grid(const float *g, int n, float g_close, double v, int *t, double *w)
+{
+    int tx = 0;
+    for (int i = 1; i <= n + 1; ++i) {
+        double gi = (i <= n) ? (double)g[i - 1] : (double)g_close;
+        if (gi >= v) { tx = i; break; }
+    }
+    *t = tx;
+    if (tx > 1) {
+        float ga = g[tx - 2];
+        float gb = (tx <= n) ? g[tx - 1] : g_close;
+        double d = (double)(float)(gb - ga);
+        *w = (d - v + (double)ga) / d;
+    } else {
+        *w = 0.0;
+    }
+}
+
+/* da_hybrid_coord_calc_ph_lev (vert_coord_mod.F90:556-561), 1-based level */
+static double orc_ph_lev(const orc_problem *pb, int lev, double phs)
+{
+    return pb->hyai[lev - 1] * (pb->p0 - pb->ptop) + pb->hybi[lev - 1] * (phs - pb->ptop) + pb->ptop;
+}
+
+int orc_obs_operator(const orc_problem *pb, const double *xb, const double *axb, int n_sites,
+                     const double *site_lat, const double *site_lon, const int *site_row_ptr,
+                     const int *row_var, const double *row_ph, const double *yo, double *yb,
+                     double *dep)
+{
+    const int k = pb->k, n = pb->n, nz = pb->nz, h = pb->halo;
+    if (h < 1 || !pb->hyai || !pb->hybi || pb->ps_index < 0 || pb->ps_index >= n) return -1;
+    const size_t lat_ext = (size_t)pb->nlat + 2 * h;
+    const size_t np = (size_t)nz + 1;
+    double *ph = (double *)malloc(sizeof(double) * np * (size_t)k);
+    double *lev = (double *)malloc(sizeof(double) * np);
+    int *jpos = (int *)malloc(sizeof(int) * (size_t)k);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)k);
+    if (!ph || !lev || !jpos || !tmp) { free(ph); free(lev); free(jpos); free(tmp); return -1; }
+    for (int s = 0; s < n_sites; ++s) {
+        int tx, ty;
+        double ax, ay;
+        orc_find_obs_grid(pb->grid_lat, pb->nlat, 90.0f, site_lat[s], &tx, &ax);
+        orc_find_obs_grid(pb->grid_lon, pb->nlon, 360.0f, site_lon[s], &ty, &ay);
+        /* the 2 x 2 points (tx-1 | tx) x (ty-1 | ty), 1-based with the halo */
+        size_t pt[2][2];
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                pt[a][b] = ((size_t)(ty - 2 + b + h) * lat_ext + (size_t)(tx - 2 + a + h)) * n;
+        /* bilinear value of state element el for member e (obs_make.F90:284-293) */
+#define ORC_BIL(el, e) ( ((xb[(pt[0][0] + (el)) * k + (e)] + axb[pt[0][0] + (el)]) * ax +          \
+                          (xb[(pt[1][0] + (el)) * k + (e)] + axb[pt[1][0] + (el)]) * (1 - ax)) * ay + \
+                         ((xb[(pt[0][1] + (el)) * k + (e)] + axb[pt[0][1] + (el)]) * ax +          \
+                          (xb[(pt[1][1] + (el)) * k + (e)] + axb[pt[1][1] + (el)]) * (1 - ax)) * (1 - ay) )
+        /* member pressure profiles (ps2ph_3d_interp, obs_make.F90:345-381) */
+        for (int e = 0; e < k; ++e) {
+            const double ps = ORC_BIL(pb->ps_index, e);
+            for (int l = 1; l <= nz + 1; ++l) lev[l - 1] = orc_ph_lev(pb, l, ps);
+            double *p = ph + (size_t)e * np;
+            for (int l = 0; l < nz; ++l) p[l] = 0.5 * (lev[l] + lev[l + 1]);
+            p[nz] = p[nz - 1];
+            jpos[e] = 1;
+        }
+        const int r0 = site_row_ptr[s], r1 = site_row_ptr[s + 1];
+        for (int r = r0; r < r1; ++r) {
+            const int v = row_var[r];
+            const int vs = pb->var_start[v] - 1;
+            for (int e = 0; e < k; ++e) {
+                const double *p = ph + (size_t)e * np;
+                /* ph_pos search, continuing from the previous level (:251-258) */
+                int j = jpos[e];
+                while (j <= nz && row_ph[r] > p[j - 1]) ++j;
+                jpos[e] = j;
+                const int pos = j - 1;
+                if (pb->var_lev[v] > 1) {
+                    if (pos >= 1 && pos < nz) {
+                        const int tl = vs + pos - 1;
+                        const double t3 = (log(p[pos]) - log(row_ph[r])) / (log(p[pos]) - log(p[pos - 1]));
+                        const double rA = ORC_BIL(tl, e), rB = ORC_BIL(tl + 1, e);
+                        tmp[e] = rA * t3 + rB * (1 - t3);
+                    } else {
+                        const int tl = (pos == 0) ? vs : vs + nz - 1;
+                        tmp[e] = ORC_BIL(tl, e);
+                    }
+                } else {
+                    tmp[e] = ORC_BIL(vs, e);
+                }
+            }
+            /* centring (letkf_obs.F90:158-170) */
+            double sum = 0.0;
+            for (int e = 0; e < k; ++e) sum += tmp[e];
+            sum = sum / k;
+            dep[r] = yo[r] - sum;
+            for (int e = 0; e < k; ++e) yb[(size_t)r * k + e] = tmp[e] - sum;
+        }
+#undef ORC_BIL
+    }
+    free(ph); free(lev); free(jpos); free(tmp);
+    return 0;
+}

@@ -84,6 +84,18 @@ void orc_epilogue(const orc_problem *pb, const double *xb, const double *axb, co
                   const double *axa, double polar_lat, double gamma1, double gamma2,
                   double *xa_out, double *spread_b, double *spread_a);
 
+/* Observation operator and Yb/d centring of gen_obs_struct.
+ * Rows of one site are level-major (all variables of a level together) and
+ * carry the level pressure row_ph (Pa) and the 0-based DA variable row_var.
+ * grid_lat/grid_lon of pb are closed by 90 and 360 (pro_info.F90:77-78);
+ * xb/axb need a halo >= 1.  yo is the observed value; outputs are
+ * yb (n_rows x k, member fastest) and dep = yo - mean_e H(x_e).
+ * Returns 0, or -1 for bad arguments. */
+int orc_obs_operator(const orc_problem *pb, const double *xb, const double *axb, int n_sites,
+                     const double *site_lat, const double *site_lon, const int *site_row_ptr,
+                     const int *row_var, const double *row_ph, const double *yo, double *yb,
+                     double *dep);
+
 #ifdef __cplusplus
 }
 #endif

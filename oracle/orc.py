@@ -254,3 +254,28 @@ def epilogue(inst, xa, axa, polar_lat=None, gamma1=None, gamma2=None):
     lib().orc_epilogue(C.byref(pb), _p(xb, _dp), _p(axb, _dp), _p(xa, _dp), _p(axa, _dp),
                        float(pl), float(g1), float(g2), _p(xa_out, _dp), _p(sb, _dp), _p(sa, _dp))
     return xa_out, sb, sa
+
+
+def obs_operator(case):
+    """gen_obs_struct's H(x) and centring on an ObsOpCase (synth.obs_operator_case).
+    Returns (yb [n_rows, k], dep [n_rows])."""
+    inst = case.inst
+    pb, keep = _problem(inst)
+    xb = np.ascontiguousarray(inst.xb, np.float64)
+    axb = np.ascontiguousarray(inst.axb, np.float64)
+    o = case
+    nr = int(o.site_row_ptr[-1])
+    yb = np.zeros((nr, inst.cfg.k))
+    dep = np.zeros(nr)
+    la = keep(np.ascontiguousarray(o.site_lat, np.float64))
+    lo = keep(np.ascontiguousarray(o.site_lon, np.float64))
+    rp = keep(np.ascontiguousarray(o.site_row_ptr, np.int32))
+    rv = keep(np.ascontiguousarray(o.row_var, np.int32))
+    rph = keep(np.ascontiguousarray(o.row_ph, np.float64))
+    yo = keep(np.ascontiguousarray(o.yo, np.float64))
+    rc = lib().orc_obs_operator(C.byref(pb), _p(xb, _dp), _p(axb, _dp), int(la.size), _p(la, _dp),
+                                _p(lo, _dp), _p(rp, _ip), _p(rv, _ip), _p(rph, _dp), _p(yo, _dp),
+                                _p(yb, _dp), _p(dep, _dp))
+    if rc != 0:
+        raise ValueError("orc_obs_operator: bad arguments")
+    return yb, dep

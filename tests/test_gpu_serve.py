@@ -133,3 +133,55 @@ def test_admission_refusal_and_cancel(torch_dev):
     # FIFO with room for one: job i starts after job i-1 ended
     for a, b in zip(sts[:2], sts[1:3]):
         assert b["t_start"] >= a["t_end"]
+
+
+def test_operator_mode_cycles_equal_engine_calls(torch_dev):
+    """Serving jobs whose cycles carry raw obs (row_var, row_ph, yo): the
+    server runs the observation operator on each cycle's device-resident
+    background; the result equals operator + analysis called cycle by cycle."""
+    torch, dev = torch_dev
+    from dida_letkf import synth, engine
+    c = synth.obs_operator_case("c2", seed=61, k=24, nlat=12, nlon=24, nz=4, n_obs=1500,
+                                n_stations=10, n_ps=30)
+    inst = c.inst
+    cfg = inst.cfg
+    cfg.local_dist = 900.0
+    t = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt))).to(dev)
+    nr = int(c.site_row_ptr[-1])
+    keep = dict(la=t(c.site_lat), lo=t(c.site_lon), rp=t(c.site_row_ptr, np.int32),
+                rv=t(c.row_var, np.int32), rph=t(c.row_ph), rinv=t(np.full(nr, 0.5)),
+                vco=t(np.log(c.row_ph)))
+    yos = [t(c.yo + 0.3 * i) for i in range(3)]
+    cyc = [dict(n_sites=c.site_lat.size, site_lat=keep["la"].data_ptr(),
+                site_lon=keep["lo"].data_ptr(), site_row_ptr=keep["rp"].data_ptr(), n_rows=nr,
+                rinv=keep["rinv"].data_ptr(), vcoord=keep["vco"].data_ptr(),
+                row_var=keep["rv"].data_ptr(), row_ph=keep["rph"].data_ptr(), yo=y.data_ptr())
+           for y in yos]
+    xb, axb = t(inst.xb), t(inst.axb)
+    # engine, cycle by cycle
+    yb = torch.empty((nr, cfg.k), dtype=torch.float64, device=dev)
+    dep = torch.empty(nr, dtype=torch.float64, device=dev)
+    bx, ba = xb, axb
+    with engine.LetkfEngine(cfg) as eng:
+        eng.set_grid(inst.grid_lat, inst.grid_lon)
+        eng.set_stream(torch.cuda.current_stream().cuda_stream)
+        for o in cyc:
+            ox, oa = torch.zeros_like(xb), torch.zeros_like(axb)
+            eng.obs_operator_device(bx.data_ptr(), ba.data_ptr(), o["n_sites"], o["site_lat"],
+                                    o["site_lon"], o["site_row_ptr"], nr, o["row_var"],
+                                    o["row_ph"], o["yo"], yb.data_ptr(), dep.data_ptr())
+            eng.set_obs_device(o["n_sites"], o["site_lat"], o["site_lon"], o["site_row_ptr"], nr,
+                               yb.data_ptr(), dep.data_ptr(), o["rinv"], o["vcoord"])
+            eng.analyze_device(bx.data_ptr(), ba.data_ptr(), ox.data_ptr(), oa.data_ptr())
+            eng.fill_halo_device(ox.data_ptr(), oa.data_ptr())
+            bx, ba = ox, oa
+        torch.cuda.synchronize()
+    ref_x, ref_a = bx.cpu().numpy(), ba.cpu().numpy()
+    xa, axa = torch.zeros_like(xb), torch.zeros_like(axb)
+    with engine.Server(16 << 30, 2) as srv:
+        jid = srv.submit(cfg, inst.grid_lat, inst.grid_lon, xb.data_ptr(), axb.data_ptr(), cyc,
+                         xa.data_ptr(), axa.data_ptr())
+        st = srv.wait(jid)
+    assert st["state"] == 2 and st["rc"] == 0, st
+    # the last cycle's halo is filled too
+    assert np.array_equal(xa.cpu().numpy(), ref_x) and np.array_equal(axa.cpu().numpy(), ref_a)

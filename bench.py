@@ -273,7 +273,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: 360x180x32 grid, n=97, k=64, 500k clustered obs, "
                                f"c={cfg.local_dist:g} km, N_A={N_A} analyses/cycle",
-                   "n_analyses": N_A, "k": k, "obs_rows": o.n_rows,
+                   "n_analyses": N_A, "k": cfg.k, "obs_rows": o.n_rows,
                    "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
                    "parallelism": "1 GPU"},
         "kernel_ms": head["kernel_ms"],

@@ -23,9 +23,10 @@
 using namespace dida;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
+namespace dida {
 int set_error(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -35,6 +36,9 @@ int set_error(int code, const char* fmt, ...) {
   g_last_error = buf;
   return code;
 }
+}  // namespace dida
+
+namespace {
 
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \

@@ -5,6 +5,8 @@
 namespace dida {
 
 constexpr int kMaxVars = 16;
+// records the message returned by dida_letkf_last_error (per host thread)
+int set_error(int code, const char* fmt, ...);
 constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
 constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 

@@ -20,6 +20,9 @@
 #include <vector>
 
 #include "dida_letkf.h"
+#include "letkf_kernels.h"
+
+using dida::set_error;
 
 namespace {
 
@@ -180,7 +183,8 @@ extern "C" {
 double dida_serve_now(void) { return now_s(); }
 
 int dida_serve_create(const dida_serve_config* cfg, dida_serve** out) {
-  if (!cfg || !out || cfg->max_active < 1 || cfg->hbm_budget_bytes == 0) return DIDA_ERR_ARG;
+  if (!cfg || !out || cfg->max_active < 1 || cfg->hbm_budget_bytes == 0)
+    return set_error(DIDA_ERR_ARG, "serve config: max_active >= 1 and a budget are needed");
   dida_serve* s = new (std::nothrow) dida_serve();
   if (!s) return DIDA_ERR_NOMEM;
   s->cfg = *cfg;
@@ -197,7 +201,7 @@ int dida_serve_create(const dida_serve_config* cfg, dida_serve** out) {
 int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_id) {
   if (!s || !job || !job_id || job->n_cycles < 1 || !job->obs || !job->grid_lat ||
       !job->grid_lon || !job->xb || !job->axb || !job->xa || !job->axa)
-    return DIDA_ERR_ARG;
+    return set_error(DIDA_ERR_ARG, "submit: null argument or n_cycles < 1");
   auto j = std::make_shared<Job>();
   j->spec = *job;
   if (job->cfg.hyai && job->cfg.hybi) {
@@ -213,23 +217,27 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
   j->glon.assign(job->grid_lon, job->grid_lon + job->cfg.nlon);
   j->obs.assign(job->obs, job->obs + job->n_cycles);
   for (const auto& o : j->obs) {
-    if (o.n_sites < 0 || o.n_rows < 0) return DIDA_ERR_ARG;
+    if (o.n_sites < 0 || o.n_rows < 0) return set_error(DIDA_ERR_ARG, "submit: negative obs sizes");
     if (!o.yb) {
-      if (!o.row_var || !o.row_ph || !o.yo) return DIDA_ERR_ARG;
-      if (job->cfg.halo < 1 || j->hyai.empty()) return DIDA_ERR_ARG;
+      if (!o.row_var || !o.row_ph || !o.yo)
+        return set_error(DIDA_ERR_ARG, "submit: a cycle without yb needs row_var, row_ph and yo");
+      if (job->cfg.halo < 1 || j->hyai.empty())
+        return set_error(DIDA_ERR_ARG, "submit: the observation operator needs halo >= 1 and hyai/hybi");
       j->op_mode = true;
     } else if (!o.dep) {
-      return DIDA_ERR_ARG;
+      return set_error(DIDA_ERR_ARG, "submit: yb without dep");
     }
     j->max_sites = std::max(j->max_sites, o.n_sites);
     j->max_rows = std::max(j->max_rows, o.n_rows);
   }
   j->st.bytes = dida_letkf_job_bytes(&job->cfg, j->max_sites, j->max_rows);
-  if (j->st.bytes == 0) return DIDA_ERR_ARG;
-  if (j->st.bytes > s->cfg.hbm_budget_bytes) return DIDA_ERR_ADMISSION;
+  if (j->st.bytes == 0) return set_error(DIDA_ERR_ARG, "submit: bad job configuration");
+  if (j->st.bytes > s->cfg.hbm_budget_bytes)
+    return set_error(DIDA_ERR_ADMISSION, "submit: job needs %zu bytes, budget %zu", j->st.bytes,
+                     (size_t)s->cfg.hbm_budget_bytes);
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stopping) return DIDA_ERR_STATE;
+    if (s->stopping) return set_error(DIDA_ERR_STATE, "submit: server is stopping");
     j->id = s->next_id++;
     j->st.state = DIDA_JOB_QUEUED;
     j->st.t_submit = now_s();

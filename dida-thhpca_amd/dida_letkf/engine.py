@@ -406,6 +406,10 @@ class Server:
         row_var, row_ph, yo."""
         job = ServeJob()
         job.cfg = to_c_config(cfg)
+        if cfg.hyai is not None:          # copied by the server at submit
+            hy = np.ascontiguousarray(cfg.hyai, np.float64)
+            hb = np.ascontiguousarray(cfg.hybi, np.float64)
+            job.cfg.hyai, job.cfg.hybi = _p(hy, _dp), _p(hb, _dp)
         la = np.ascontiguousarray(grid_lat, np.float32)
         lo = np.ascontiguousarray(grid_lon, np.float32)
         arr = (ServeObs * len(obs_cycles))()

@@ -110,6 +110,45 @@ def serving_leg(torch, dev, args, precision):
     return out
 
 
+def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
+    """The one JSON line: the headline (mixed) mode, the FP64 form beside it."""
+    from dida_letkf.config import PREC_FP64
+    head = res[args.mixed]
+    st = head["stats"]
+    roof = roofline(head["kernel_ms"], st, cfg, n_a, args.mixed)
+    line = {
+        "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
+        "value": round(head["value"], 2),
+        "unit": "local analyses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+f64" if args.mixed == 2 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {cfg.nlon}x{cfg.nlat}x{cfg.nz} grid, n={cfg.n}, "
+                               f"k={cfg.k}, {n_obs_rows} clustered obs rows, c={cfg.local_dist:g} km, "
+                               f"N_A={n_a} analyses/cycle",
+                   "n_analyses": n_a, "k": cfg.k, "obs_rows": n_obs_rows,
+                   "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
+                   "parallelism": "1 GPU"},
+        "kernel_ms": head["kernel_ms"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+    }
+    if PREC_FP64 in res:
+        r64 = res[PREC_FP64]
+        line["fp64_value"] = round(r64["value"], 2)
+        line["fp64_ms_per_step"] = round(r64["ms_step"], 3)
+        line["fp64_kernel_ms"] = r64["kernel_ms"]
+        line["mixed_over_fp64"] = round(head["value"] / r64["value"], 3)
+    return line
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,38 +293,7 @@ def main():
                           f"own Householder+QL eigensolver, FP64")
         log(f"cpu baseline: {cpu}")
 
-    # ---- roofline of the dominant kernel (largest average device time)
-    head = res[args.mixed]
-    roof = roofline(head["kernel_ms"], head["stats"], cfg, N_A, args.mixed)
-
-    line = {
-        "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
-        "value": round(head["value"], 2),
-        "unit": "local analyses/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(head["ms_step"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "bf16x3+f64" if args.mixed == 2 else "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.config}: 360x180x32 grid, n=97, k=64, 500k clustered obs, "
-                               f"c={cfg.local_dist:g} km, N_A={N_A} analyses/cycle",
-                   "n_analyses": N_A, "k": cfg.k, "obs_rows": o.n_rows,
-                   "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
-                   "parallelism": "1 GPU"},
-        "kernel_ms": head["kernel_ms"],
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "accuracy": acc,
-    }
-    if PREC_FP64 in res:
-        line["fp64_value"] = round(res[PREC_FP64]["value"], 2)
-        line["fp64_ms_per_step"] = round(res[PREC_FP64]["ms_step"], 3)
-        line["fp64_kernel_ms"] = res[PREC_FP64]["kernel_ms"]
-        line["mixed_over_fp64"] = round(head["value"] / res[PREC_FP64]["value"], 3)
+    line = make_line(args, cfg, N_A, o.n_rows, res, acc, cpu)
     if args.serve_jobs > 0:
         del d
         torch.cuda.empty_cache()

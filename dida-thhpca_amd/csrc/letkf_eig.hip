@@ -189,15 +189,18 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
           din = sd[i - 1][lane];
         }
         const double f = s * ei, bb = c * ei;
-        r = sqrt(f * f + g * g);
-        se[i + 1][lane] = r;
-        if (r == 0.0) {
+        // one reciprocal square root instead of sqrt and a divide on the
+        // serial chain: r = sqrt(f^2 + g^2), s = f / r, c = g / r
+        const double r2 = fma(f, f, g * g);
+        if (r2 == 0.0) {
+          se[i + 1][lane] = 0.0;
           sd[i + 1][lane] = di1 - pp;
           se[m][lane] = 0.0;
           underflow = true;
           break;
         }
-        const double rinv = 1.0 / r;
+        const double rinv = rsqrt(r2);
+        se[i + 1][lane] = r2 * rinv;
         s = f * rinv;
         c = g * rinv;
         g = di1 - pp;

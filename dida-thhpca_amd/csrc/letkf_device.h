@@ -62,6 +62,17 @@ __device__ __forceinline__ double deg2rad_ref(double x) {
   return x / 180.0 * kPiF32;
 }
 
+// A value every active lane holds identically, moved to a scalar register:
+// branches and loop bounds on it then compile to uniform (scalar) control
+// flow instead of exec-mask divergence.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffu));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Butterfly sums: every lane ends with the bitwise-same value (IEEE addition
 // is commutative, so partners add the same two operands).
 __device__ __forceinline__ double wave_sum(double v) {

@@ -54,8 +54,8 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
   for (int j = 0; j + 2 < k; ++j) {
     const bool act = in_r && lane > j;
     const double x = act ? xcol : 0.0;
-    const double x0 = __shfl(xcol, j + 1, 64);
-    const double rest = wave_sum((act && lane > j + 1) ? x * x : 0.0);
+    const double x0 = uniform(__shfl(xcol, j + 1, 64));
+    const double rest = uniform(wave_sum((act && lane > j + 1) ? x * x : 0.0));
     double xn = 0.0;
     if (rest == 0.0) {
       // column already in tridiagonal form
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     }
     double pr = (pr0 + pr1) + (pr2 + pr3);
     pr = act ? pr * tau : 0.0;
-    const double K = 0.5 * tau * wave_sum(v * pr);
+    const double K = 0.5 * tau * uniform(wave_sum(v * pr));
     const double wr = act ? pr - K * v : 0.0;
     if (lane < KMAX) sw[lane] = wr;
     __syncthreads();
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
 #pragma unroll
   for (int c = 0; c < KMAX; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
   for (int j = 0; j < nref; ++j) {
-    const double tau = st[j];
+    const double tau = uniform(st[j]);
     if (tau == 0.0) continue;
     const double* q = sV + j * LDV;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
@@ -313,11 +313,14 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   int nrot = 0;
   if (nch > 0) {
     const int4 hl = shdr[nch - 1];
-    nrot = hl.z + (hl.y - hl.x);
+    nrot = uniform(hl.z + (hl.y - hl.x));
   }
   int committed = 0;                     // rotation entries present in the ring
   for (int ch = 0; ch < nch; ++ch) {
-    const int4 h = shdr[ch];
+    int4 h = shdr[ch];
+    h.x = uniform(h.x);
+    h.y = uniform(h.y);
+    h.z = uniform(h.z);
     const int lo = h.x, hi = h.y;
     if (h.z + (hi - lo) > committed) {
       double2 v[REFILL / 64];
@@ -340,16 +343,19 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
       if (B > bhi || B < blo) continue;
       const int top = (8 * B + 7 < KMAX - 1) ? 8 * B + 7 : KMAX - 2;
       if (B == bhi) carry = z[top + 1];
+      // the block's 8 ring entries are read unconditionally (the masked index
+      // is always inside the ring) so the reads issue back to back; positions
+      // outside [lo, hi) take the identity rotation
+      double2 cs[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) cs[t] = sring[(base - (8 * B + t)) & (RING - 1)];
 #pragma unroll
       for (int t = 7; t >= 0; --t) {
         const int i = 8 * B + t;
         if (i > KMAX - 2) continue;
-        double c = 1.0, s = 0.0;
-        if (i >= lo && i < hi) {
-          const double2 cs = sring[(base - i) & (RING - 1)];
-          c = cs.x;
-          s = cs.y;
-        }
+        const bool in = i >= lo && i < hi;
+        const double c = in ? cs[t].x : 1.0;
+        const double s = in ? cs[t].y : 0.0;
         const double zi = z[i];
         z[i + 1] = s * zi + c * carry;
         carry = c * zi - s * carry;
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   // ---- eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
   double lam = in_r ? p.lam[al * kp + lane] : 0.0;
   if (p.guard) {
-    const double mn = wave_min(in_r ? lam : 1e300);
+    const double mn = uniform(wave_min(in_r ? lam : 1e300));
     if (mn < 0.0) lam = lam - mn;
   }
   lam = lam * 1.0 + (double)(k - 1) / p.inflation;
@@ -555,17 +561,24 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
 // thread r owns row r); the QL recurrence is k_ql_log<128>.
 // ---------------------------------------------------------------------------
 
-// Sum over the 128 threads of a workgroup; every thread gets the same value
-// (wave sums, then the two partials in a fixed order).  `red` holds two slots
-// of two partials so that back-to-back calls need one barrier each.
-__device__ __forceinline__ double block2_sum(double v, double* red, int& slot) {
+// Sum over the NW wavefronts of a workgroup; every thread gets the same value
+// (wave sums, then the partials in a fixed order).  `red` holds two slots of
+// NW partials so that back-to-back calls need one barrier each.
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double* red, int& slot) {
   const double w = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[2 * slot + wv] = w;
-  __syncthreads();
-  const double r = red[2 * slot] + red[2 * slot + 1];
-  slot ^= 1;
-  return r;
+  if constexpr (NW == 1) {
+    return uniform(w);
+  } else {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[NW * slot + wv] = w;
+    __syncthreads();
+    double r = red[NW * slot];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r += red[NW * slot + i];
+    slot ^= 1;
+    return uniform(r);
+  }
 }
 
 __device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
@@ -575,18 +588,20 @@ __device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
   __syncthreads();
   const double r = fmin(red[2 * slot], red[2 * slot + 1]);
   slot ^= 1;
-  return r;
+  return uniform(r);
 }
 
-// Householder tridiagonalisation with A in LDS; same outputs as k_tridiag.
+// Householder tridiagonalisation with A in LDS, one thread per row (one
+// workgroup of max(KMAX, 64) threads per matrix); same outputs as k_tridiag.
 template <int KMAX>
-__global__ __launch_bounds__(KMAX) void k_tridiag_lds(EigParams p) {
-  static_assert(KMAX == 128, "two wavefronts per matrix");
+__global__ __launch_bounds__(KMAX < 64 ? 64 : KMAX) void k_tridiag_lds(EigParams p) {
+  constexpr int NT = KMAX < 64 ? 64 : KMAX;
+  constexpr int NW = NT / 64;
   constexpr int LD = KMAX + 1;
   __shared__ double sA[KMAX * LD];
   __shared__ double sv[KMAX], sw[KMAX];
   __shared__ double se_s[KMAX + 1], st_s[KMAX + 1];
-  __shared__ double red[4];
+  __shared__ double red[2 * NW];
   const int r = threadIdx.x;
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
@@ -594,18 +609,20 @@ __global__ __launch_bounds__(KMAX) void k_tridiag_lds(EigParams p) {
   const int k = p.k, kp = p.kp;
   const double* __restrict__ Ag = p.A + al * (long long)kp * kp;
   double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const bool row = r < KMAX;             // thread owns a row of the LDS matrix
   const bool in_r = r < k;
-  for (int c = 0; c < KMAX; ++c)
-    sA[c * LD + r] = (in_r && c < k) ? Ag[(long long)c * kp + r] : 0.0;
+  if (row)
+    for (int c = 0; c < KMAX; ++c)
+      sA[c * LD + r] = (in_r && c < k) ? Ag[(long long)c * kp + r] : 0.0;
   int slot = 0;
   __syncthreads();
   for (int j = 0; j + 2 < k; ++j) {
     const bool act = in_r && r > j;
     const double x = act ? sA[j * LD + r] : 0.0;
-    const double x0 = sA[j * LD + j + 1];
-    const double rest = block2_sum((act && r > j + 1) ? x * x : 0.0, red, slot);
+    const double x0 = uniform(sA[j * LD + j + 1]);
+    const double rest = block_sum<NW>((act && r > j + 1) ? x * x : 0.0, red, slot);
     if (rest == 0.0) {
-      refl[(long long)j * kp + r] = 0.0;
+      if (row) refl[(long long)j * kp + r] = 0.0;
       if (r == 0) { se_s[j] = x0; st_s[j] = 0.0; }
       continue;
     }
@@ -614,7 +631,7 @@ __global__ __launch_bounds__(KMAX) void k_tridiag_lds(EigParams p) {
     const double v0 = x0 - alpha;
     const double tau = 2.0 / (rest + v0 * v0);
     const double v = act ? ((r == j + 1) ? v0 : x) : 0.0;
-    sv[r] = v;
+    if (row) sv[r] = v;
     __syncthreads();
     double pr0 = 0.0, pr1 = 0.0;
     if (act) {
@@ -626,17 +643,18 @@ __global__ __launch_bounds__(KMAX) void k_tridiag_lds(EigParams p) {
       if (c < k) pr0 = fma(sA[c * LD + r], sv[c], pr0);
     }
     const double pr = act ? (pr0 + pr1) * tau : 0.0;
-    const double K = 0.5 * tau * block2_sum(v * pr, red, slot);
+    const double K = 0.5 * tau * block_sum<NW>(v * pr, red, slot);
     const double wr = act ? pr - K * v : 0.0;
-    sw[r] = wr;
+    if (row) sw[r] = wr;
     __syncthreads();
     if (act)
       for (int c = j + 1; c < k; ++c) sA[c * LD + r] -= v * sw[c] + wr * sv[c];
-    refl[(long long)j * kp + r] = v;
+    if (row) refl[(long long)j * kp + r] = v;
     if (r == 0) { se_s[j] = alpha; st_s[j] = tau; }
     __syncthreads();
   }
   __syncthreads();
+  if (!row) return;
   double* __restrict__ dg = p.d + al * kp;
   double* __restrict__ eg = p.e + al * kp;
   double* __restrict__ tg = p.tau + al * kp;
@@ -680,7 +698,7 @@ __global__ __launch_bounds__(KMAX) void k_apply_lds(EigParams p) {
   for (int j = 0; j + 2 < k; ++j) {
     const int buf = j & 1;
     sq[buf * KMAX + r] = refl[(long long)j * kp + r];
-    const double tau = tg[j];
+    const double tau = uniform(tg[j]);
     __syncthreads();
     if (tau == 0.0) continue;
     const double* q = sq + buf * KMAX;
@@ -701,7 +719,10 @@ __global__ __launch_bounds__(KMAX) void k_apply_lds(EigParams p) {
   const int nch = p.nchase[al];
   for (int ch = 0; ch < nch; ++ch) {
     const int buf = ch & 1;
-    const int4 h = hdr[ch];
+    int4 h = hdr[ch];
+    h.x = uniform(h.x);
+    h.y = uniform(h.y);
+    h.z = uniform(h.z);
     const int lo = h.x, hi = h.y;
     if (r < hi - lo) scs[buf * KMAX + r] = rot[h.z + r];
     __syncthreads();
@@ -863,13 +884,15 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
   if (p.k <= 32) {
-    hipLaunchKernelGGL(k_tridiag<32>, dim3(nb), dim3(64), 0, st, p);
+    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<32>, dim3(nb), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL(k_tridiag<32>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
   } else if (p.k <= 64) {
-    hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
+    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -100,6 +101,16 @@ struct EigScratch {
 };
 
 
+// Tridiagonalisation variant for k <= 64: DIDA_TRIDIAG=lds selects the
+// LDS-resident kernel, anything else the register-resident one.
+int tridiag_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("DIDA_TRIDIAG");
+    return (e && std::strcmp(e, "lds") == 0) ? 1 : 0;
+  }();
+  return v;
+}
+
 // bytes of split-eigensolver scratch per analysis
 size_t eig_bytes_per_analysis(int k) {
   const int kp = kp_of(k);
@@ -150,6 +161,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.hdr = s.hdr.as<int4>();
   ep.rot_cap = rot_cap;
   ep.hdr_cap = hdr_cap;
+  ep.tridiag_lds = tridiag_variant();
   ep.nchase = s.nchase.as<int>();
   ep.status = xp.status;
   ep.nzc = xp.nzc;

@@ -103,6 +103,7 @@ struct EigParams {
   double2* rot;             // batch-local, rot_cap Givens (c, s) per analysis
   int4* hdr;                // batch-local, hdr_cap chases (lo, hi, off) per analysis
   int rot_cap, hdr_cap;
+  int tridiag_lds;          // k <= 64: tridiagonalise in LDS (1) or in VGPRs (0)
   int* nchase;              // batch-local
   int* status;              // global index
   int nzc, vertical, cube_nz, nz, halo, n, n_var;

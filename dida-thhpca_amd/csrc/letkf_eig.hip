@@ -54,8 +54,8 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
   for (int j = 0; j + 2 < k; ++j) {
     const bool act = in_r && lane > j;
     const double x = act ? xcol : 0.0;
-    const double x0 = uniform(__shfl(xcol, j + 1, 64));
-    const double rest = uniform(wave_sum((act && lane > j + 1) ? x * x : 0.0));
+    const double x0 = __shfl(xcol, j + 1, 64);
+    const double rest = wave_sum((act && lane > j + 1) ? x * x : 0.0);
     double xn = 0.0;
     if (rest == 0.0) {
       // column already in tridiagonal form
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     }
     double pr = (pr0 + pr1) + (pr2 + pr3);
     pr = act ? pr * tau : 0.0;
-    const double K = 0.5 * tau * uniform(wave_sum(v * pr));
+    const double K = 0.5 * tau * wave_sum(v * pr);
     const double wr = act ? pr - K * v : 0.0;
     if (lane < KMAX) sw[lane] = wr;
     __syncthreads();
@@ -237,7 +237,9 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   constexpr int HMAX = kHdrCapPerK * KMAX;            // chase headers per analysis
   constexpr int RING = 16 * KMAX;                      // rotation ring (entries)
   constexpr int REFILL = RING / 2;                     // entries per ring refill
-  constexpr int SBUF = (KMAX * LDV > 2 * HMAX + 2 * RING) ? KMAX * LDV : 2 * HMAX + 2 * RING;
+  // phase B layout (doubles): headers 2*HMAX, ring 2*RING, dense chase table 2*KMAX
+  constexpr int SB_B = 2 * HMAX + 2 * RING + 2 * KMAX;
+  constexpr int SBUF = (KMAX * LDV > SB_B) ? KMAX * LDV : SB_B;
   static_assert(REFILL >= KMAX && REFILL % 64 == 0, "a refill covers any chase");
   // phase A: reflectors (row j = v_j); phase B: chase headers + rotation ring;
   // phases C, D: V rows, then staged background vectors
@@ -305,6 +307,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   // REFILL entries (REFILL / 64 coalesced loads per lane) at a time.
   int4* shdr = reinterpret_cast<int4*>(sbuf);
   double2* sring = reinterpret_cast<double2*>(sbuf + 2 * HMAX);
+  double2* stab = reinterpret_cast<double2*>(sbuf + 2 * HMAX + 2 * RING);
   const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
   const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
   const int nch = min(p.nchase[al], HMAX);
@@ -337,31 +340,34 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
     }
     const int bhi = (hi - 1) >> 3, blo = lo >> 3;
     const int base = h.z + hi - 1;       // entry of the rotation at position i is base - i
+    // dense table of this chase: position i holds its rotation, positions
+    // outside [lo, hi) the identity, so the loop below is multiplies only
+    if (lane < KMAX) {
+      const bool in = lane >= lo && lane < hi;
+      const double2 g = sring[(base - lane) & (RING - 1)];
+      stab[lane] = in ? g : make_double2(1.0, 0.0);
+    }
+    __syncthreads();
     double carry = 0.0;
 #pragma unroll
     for (int B = NB - 1; B >= 0; --B) {
       if (B > bhi || B < blo) continue;
       const int top = (8 * B + 7 < KMAX - 1) ? 8 * B + 7 : KMAX - 2;
       if (B == bhi) carry = z[top + 1];
-      // the block's 8 ring entries are read unconditionally (the masked index
-      // is always inside the ring) so the reads issue back to back; positions
-      // outside [lo, hi) take the identity rotation
       double2 cs[8];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) cs[t] = sring[(base - (8 * B + t)) & (RING - 1)];
+      for (int t = 0; t < 8; ++t) cs[t] = stab[8 * B + t];
 #pragma unroll
       for (int t = 7; t >= 0; --t) {
         const int i = 8 * B + t;
         if (i > KMAX - 2) continue;
-        const bool in = i >= lo && i < hi;
-        const double c = in ? cs[t].x : 1.0;
-        const double s = in ? cs[t].y : 0.0;
         const double zi = z[i];
-        z[i + 1] = s * zi + c * carry;
-        carry = c * zi - s * carry;
+        z[i + 1] = cs[t].y * zi + cs[t].x * carry;
+        carry = cs[t].x * zi - cs[t].y * carry;
       }
       if (B == blo) z[8 * B] = carry;
     }
+    __syncthreads();
   }
   __syncthreads();
 

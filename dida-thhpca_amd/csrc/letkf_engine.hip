@@ -204,15 +204,22 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   return DIDA_OK;
 }
 
-// Columns per batch: the per-batch scratch (products + eigensolver) is
-// bounded to ~6 GiB, or by max_batch_columns.
-long long max_batch_columns(const dida_letkf_config& c) {
+namespace dida {
+// columns whose per-batch scratch fits in `bytes`
+long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes) {
   const int kp = kp_of(c.k);
   const int cz = c.cube_nz > 0 ? c.cube_nz : c.nz;
   const long long nzc = c.vert_dist > 0.0 ? (c.nz + cz - 1) / cz : 1;
   const long long per_col =
       nzc * ((long long)kp * (kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k));
-  long long mb = std::max<long long>(1, (6LL << 30) / per_col);
+  return std::max<long long>(1, (long long)(bytes / (size_t)per_col));
+}
+}  // namespace dida
+
+// Columns per batch: the per-batch scratch (products + eigensolver) is
+// bounded to 16 GiB (a 360x180 k=64 cycle is one batch), or by max_batch_columns.
+long long max_batch_columns(const dida_letkf_config& c) {
+  long long mb = batch_columns_for_bytes(c, 16ull << 30);
   if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
   const long long nh = (long long)((c.nlat + c.cube_n - 1) / c.cube_n) *
                        ((c.nlon + c.cube_n - 1) / c.cube_n);

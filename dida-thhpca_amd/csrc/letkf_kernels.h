@@ -7,6 +7,11 @@ namespace dida {
 constexpr int kMaxVars = 16;
 // records the message returned by dida_letkf_last_error (per host thread)
 int set_error(int code, const char* fmt, ...);
+}  // namespace dida
+struct dida_letkf_config;
+namespace dida {
+// analysis columns per batch whose scratch fits in `bytes`
+long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes);
 constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
 constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 

@@ -213,6 +213,10 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
     j->spec.cfg.hyai = nullptr;
     j->spec.cfg.hybi = nullptr;
   }
+  // a serving job's batches are bounded to 4 GiB of scratch unless set
+  if (j->spec.cfg.max_batch_columns <= 0)
+    j->spec.cfg.max_batch_columns = (int)std::min<long long>(
+        dida::batch_columns_for_bytes(job->cfg, 4ull << 30), 1 << 30);
   j->glat.assign(job->grid_lat, job->grid_lat + job->cfg.nlat);
   j->glon.assign(job->grid_lon, job->grid_lon + job->cfg.nlon);
   j->obs.assign(job->obs, job->obs + job->n_cycles);
@@ -230,7 +234,7 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
     j->max_sites = std::max(j->max_sites, o.n_sites);
     j->max_rows = std::max(j->max_rows, o.n_rows);
   }
-  j->st.bytes = dida_letkf_job_bytes(&job->cfg, j->max_sites, j->max_rows);
+  j->st.bytes = dida_letkf_job_bytes(&j->spec.cfg, j->max_sites, j->max_rows);
   if (j->st.bytes == 0) return set_error(DIDA_ERR_ARG, "submit: bad job configuration");
   if (j->st.bytes > s->cfg.hbm_budget_bytes)
     return set_error(DIDA_ERR_ADMISSION, "submit: job needs %zu bytes, budget %zu", j->st.bytes,

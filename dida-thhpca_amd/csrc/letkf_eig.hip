@@ -36,6 +36,7 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
   constexpr int NB = KMAX / 8;
   __shared__ double sv[KMAX], sw[KMAX];
   __shared__ double se_s[KMAX + 1], st_s[KMAX + 1];
+  __shared__ double sxn[64];             // column j+1 after step j's update
   const int lane = threadIdx.x;
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
@@ -56,14 +57,16 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     const double x = act ? xcol : 0.0;
     const double x0 = __shfl(xcol, j + 1, 64);
     const double rest = wave_sum((act && lane > j + 1) ? x * x : 0.0);
-    double xn = 0.0;
     if (rest == 0.0) {
-      // column already in tridiagonal form
+      // column already in tridiagonal form; column j+1 goes through LDS,
+      // written under a uniform branch (no per-slot selects)
 #pragma unroll
-      for (int c = 0; c < KMAX; ++c) xn = (c == j + 1) ? arow[c] : xn;
+      for (int c = 0; c < KMAX; ++c)
+        if (c == j + 1) sxn[lane] = arow[c];
       if (lane < KMAX) refl[(long long)j * kp + lane] = 0.0;
       if (lane == 0) { se_s[j] = x0; st_s[j] = 0.0; }
-      xcol = xn;
+      __syncthreads();
+      xcol = sxn[lane];
       continue;
     }
     const double nrm = sqrt(rest + x0 * x0);
@@ -102,14 +105,14 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
       for (int t = 0; t < 8; ++t) {
         const int c = 8 * B + t;
         arow[c] -= v * sw[c] + wr * sv[c];
-        xn = (c == j + 1) ? arow[c] : xn;
+        if (c == j + 1) sxn[lane] = arow[c];
       }
       __builtin_amdgcn_sched_barrier(0);
     }
     if (lane < KMAX) refl[(long long)j * kp + lane] = v;
     if (lane == 0) { se_s[j] = alpha; st_s[j] = tau; }
-    xcol = xn;
     __syncthreads();
+    xcol = sxn[lane];
   }
   // diagonal and the last subdiagonal entry
   double dr = 0.0, ek2 = 0.0;

@@ -38,41 +38,51 @@ def log(*a):
 
 
 KERNEL_NAMES = {"products": "k_products", "tridiag": "k_tridiag", "ql": "k_ql_log",
-                "apply": "k_apply"}
+                "apply": "k_vec", "update": "k_update"}
 
 
 def kernel_work(name, stats, cfg, n_an, mixed):
-    """Algorithmic flops of one cycle's launches of a kernel, and its peak.
+    """Algorithmic work of one cycle's launch of a kernel: (amount, peak, bound, unit).
 
-    products: the symmetric SYRK k(k+1) p + Yb^T R^-1 d 2k p per analysis
+    products: the symmetric SYRK k(k+1) p + Yb^T R^-1 d 2k p flops per analysis
               (BF16 MFMA peak in the split-bf16 mode, else FP64);
-    tridiag:  Householder tridiagonalisation 4/3 k^3 (Golub & Van Loan 8.3.1);
+    tridiag:  Householder tridiagonalisation 4/3 k^3 flops (Golub & Van Loan 8.3.1);
     ql:       the QL recurrence, ~30 flops per Givens step, ~1.1 k^2 steps;
-    apply:    Q = H_0..H_{k-3} 4/3 k^3, rotation accumulation 6 k^3 (GvL 8.3.5),
-              T = V f V^T 2k^3, overline_w 4k^2, update 2 k^2 m.
+    apply:    eigenvectors and transform: Q = H_0..H_{k-3} 4/3 k^3, rotation
+              accumulation 6 k^3 (GvL 8.3.5), T = V f V^T 2k^3, overline_w 4k^2;
+              (with the fused kernel, k <= 64 only under DIDA_APPLY=fused, this
+              also carries the update's 2 k^2 m);
+    update:   the ensemble update, HBM-bound: Xb read and Xa written (2*8*k*m
+              bytes) plus the means (16 m bytes) per analysis.
     The FP64 vector and FP64 matrix peaks of MI355X are equal (78.6 TFLOP/s)."""
     k, m = cfg.k, cfg.n
     if name == "products":
         p = stats["total_rows"]
         fl = k * (k + 1) * p + 2 * k * p
-        return fl, (PEAK_BF16_TFLOPS if mixed == 2 else PEAK_FP64_TFLOPS)
+        return fl, (PEAK_BF16_TFLOPS if mixed == 2 else PEAK_FP64_TFLOPS), "mfma", "TFLOP/s"
     if name == "tridiag":
-        return n_an * (4.0 / 3.0) * k ** 3, PEAK_FP64_TFLOPS
+        return n_an * (4.0 / 3.0) * k ** 3, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
     if name == "ql":
-        return n_an * 30 * 1.1 * k ** 2, PEAK_FP64_TFLOPS
-    return n_an * ((4.0 / 3.0) * k ** 3 + 6 * k ** 3 + 2 * k ** 3 + 4 * k ** 2 + 2 * k ** 2 * m), \
-        PEAK_FP64_TFLOPS
+        return n_an * 30 * 1.1 * k ** 2, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
+    if name == "update":
+        return n_an * (2 * 8 * k * m + 16 * m), PEAK_HBM_GBS / 1000.0, "hbm", "TB/s"
+    fl = n_an * ((4.0 / 3.0) * k ** 3 + 6 * k ** 3 + 2 * k ** 3 + 4 * k ** 2)
+    if stats.get("ms_update", 0.0) == 0.0:
+        fl += n_an * 2 * k ** 2 * m
+    return fl, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
 
 
 def roofline(km, stats, cfg, n_an, mixed):
-    cands = [x for x in ("products", "tridiag", "ql", "apply") if km.get(x, 0.0) > 0]
+    cands = [x for x in ("products", "tridiag", "ql", "apply", "update") if km.get(x, 0.0) > 0]
     dom = max(cands, key=lambda x: km[x])
-    fl, peak = kernel_work(dom, stats, cfg, n_an, mixed)
-    achieved = fl / (km[dom] * 1e-3) / 1e12
-    return dict(bound="mfma", kernel=KERNEL_NAMES[dom], achieved=round(achieved, 4), peak=peak,
-                unit="TFLOP/s", frac=round(achieved / peak, 5), traffic=None,
-                ms_per_launch=round(km[dom], 4), flops_per_launch=fl,
-                kernel_ms={KERNEL_NAMES[x]: round(km[x], 4) for x in cands})
+    amount, peak, bound, unit = kernel_work(dom, stats, cfg, n_an, mixed)
+    achieved = amount / (km[dom] * 1e-3) / 1e12
+    out = dict(bound=bound, kernel=KERNEL_NAMES[dom], achieved=round(achieved, 4), peak=peak,
+               unit=unit, frac=round(achieved / peak, 5), traffic=None,
+               ms_per_launch=round(km[dom], 4),
+               kernel_ms={KERNEL_NAMES[x]: round(km[x], 4) for x in cands})
+    out["bytes_per_launch" if bound == "hbm" else "flops_per_launch"] = amount
+    return out
 
 
 def serving_leg(torch, dev, args, precision):
@@ -223,7 +233,8 @@ def main():
         for _ in range(args.warmup):
             step(eng, xa, axa)
         torch.cuda.synchronize()
-        k_ms = dict(select=0.0, products=0.0, transform=0.0, tridiag=0.0, ql=0.0, apply=0.0)
+        k_ms = dict(select=0.0, products=0.0, transform=0.0, tridiag=0.0, ql=0.0, apply=0.0,
+                    update=0.0)
         ts = time.perf_counter()
         flagged = 0
         for _ in range(args.steps):
@@ -235,6 +246,7 @@ def main():
             k_ms["tridiag"] += st.ms_tridiag
             k_ms["ql"] += st.ms_ql
             k_ms["apply"] += st.ms_apply
+            k_ms["update"] += st.ms_update
         torch.cuda.synchronize()
         el = time.perf_counter() - ts
         stats = eng.stats.as_dict()

@@ -167,12 +167,29 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
   for (int l = 0; l < k && status == 0; ++l) {
     int iter = 0;
     for (;;) {
+      // first m >= l with a negligible e[m] (or k - 1); four entries are
+      // read per step so the scan pays one LDS latency per four elements
       int m = l;
-      for (; m < k - 1; ++m) {
-        const double dd = fabs(sd[m][lane]) + fabs(sd[m + 1][lane]);
-        const double em = fabs(se[m][lane]);
-        if (em <= eps * dd || em <= eps * tnorm) break;
+      for (;;) {
+        double dv[5], ev[4];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) dv[t] = (m + t < KMAX) ? sd[m + t][lane] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ev[t] = (m + t < KMAX) ? se[m + t][lane] : 0.0;
+        int hit = 4;
+#pragma unroll
+        for (int t = 3; t >= 0; --t) {
+          const double dd = fabs(dv[t]) + fabs(dv[t + 1]);
+          const double em = fabs(ev[t]);
+          if (m + t >= k - 1 || em <= eps * dd || em <= eps * tnorm) hit = t;
+        }
+        if (hit < 4) {
+          m += hit;
+          break;
+        }
+        m += 4;
       }
+      if (m > k - 1) m = k - 1;
       if (m == l) break;
       if (++iter > 30) { status = l + 1; break; }
       if (nhdr >= p.hdr_cap || nrot + (m - l) > p.rot_cap) { status = -1; break; }
@@ -565,6 +582,375 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// k_vec + k_update: the k_apply work in two kernels sized for two wavefronts
+// per SIMD.  k_vec forms V (Q accumulation, logged rotations), overline_w and
+// trans = T + overline_w 1^T with LDS staged in small chunks, and writes
+// trans in the register layout of the update's MFMA B operand over the
+// analysis' reflector block (consumed by then).  k_update streams the state
+// vectors of the analysis through that operand.
+// ---------------------------------------------------------------------------
+template <int KMAX>
+struct VecLds {
+  static constexpr int LDV = KMAX + 1;
+  static constexpr int RING = 8 * KMAX;              // rotation ring (entries)
+  static constexpr int REFILL = RING / 2;
+  static constexpr int A = 16 * LDV;                 // 16 reflectors
+  static constexpr int B = 2 * RING + 2 * KMAX;      // ring + dense chase table
+  static constexpr int C = 16 * LDV;                 // 16 columns of V
+  static constexpr int N = (A > B ? (A > C ? A : C) : (B > C ? B : C));
+};
+
+// offset (in doubles) of tile (I, J) register r of lane l in a trans block
+template <int KMAX>
+__device__ __forceinline__ int tile_off(int I, int J, int r, int l) {
+  return ((I * (KMAX / 16) + J) * 4 + r) * 64 + l;
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_vec(EigParams p) {
+  constexpr int NB = KMAX / 8;
+  constexpr int TT = KMAX / 16;
+  constexpr int IPP = TT < 2 ? TT : 2;               // tile rows per pass
+  using L = VecLds<KMAX>;
+  constexpr int LDV = L::LDV, RING = L::RING, REFILL = L::REFILL;
+  static_assert(REFILL >= KMAX && REFILL % 64 == 0, "a refill covers any chase");
+  __shared__ __align__(16) double sbuf[L::N];
+  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
+  const int lane = threadIdx.x;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  if (p.status[a] != 0) return;          // rerun by the single-kernel transform
+  const int k = p.k, kp = p.kp;
+  const bool in_r = lane < k;
+  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + al * kp;
+  const int nref = k > 2 ? k - 2 : 0;
+
+  // ---- A. Z <- Z H_j, reflectors staged 16 at a time ----
+  if (lane < nref) stau[lane] = tg[lane];
+  double z[KMAX];
+#pragma unroll
+  for (int c = 0; c < KMAX; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
+  for (int j0 = 0; j0 < nref; j0 += 16) {
+    __syncthreads();
+    {
+      double v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        v[t] = (j0 + t < nref && lane < KMAX) ? refl[(long long)(j0 + t) * kp + lane] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (lane < KMAX) sbuf[t * LDV + lane] = v[t];
+    }
+    __syncthreads();
+    const int j1 = min(j0 + 16, nref);
+    for (int j = j0; j < j1; ++j) {
+      const double tau = uniform(stau[j]);
+      if (tau == 0.0) continue;
+      const double* q = sbuf + (j - j0) * LDV;
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+      for (int B = 0; B < NB; ++B) {
+        if (8 * B + 7 <= j) continue;
+        d0 = fma(z[8 * B + 0], q[8 * B + 0], d0);
+        d1 = fma(z[8 * B + 1], q[8 * B + 1], d1);
+        d2 = fma(z[8 * B + 2], q[8 * B + 2], d2);
+        d3 = fma(z[8 * B + 3], q[8 * B + 3], d3);
+        d0 = fma(z[8 * B + 4], q[8 * B + 4], d0);
+        d1 = fma(z[8 * B + 5], q[8 * B + 5], d1);
+        d2 = fma(z[8 * B + 6], q[8 * B + 6], d2);
+        d3 = fma(z[8 * B + 7], q[8 * B + 7], d3);
+      }
+      const double dot = ((d0 + d1) + (d2 + d3)) * tau;
+#pragma unroll
+      for (int B = 0; B < NB; ++B) {
+        if (8 * B + 7 <= j) continue;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) z[8 * B + t] = fma(-dot, q[8 * B + t], z[8 * B + t]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- B. logged QL rotations: a ring of REFILL-entry refills, dense chase tables ----
+  {
+    double2* sring = reinterpret_cast<double2*>(sbuf);
+    double2* stab = reinterpret_cast<double2*>(sbuf + 2 * RING);
+    const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
+    const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
+    const int nch = uniform(min(p.nchase[al], p.hdr_cap));
+    int nrot = 0;
+    if (nch > 0) {
+      const int4 hl = hdr[nch - 1];
+      nrot = uniform(hl.z + (hl.y - hl.x));
+    }
+    int committed = 0;
+    int4 hn = nch > 0 ? hdr[0] : make_int4(0, 0, 0, 0);
+    for (int ch = 0; ch < nch; ++ch) {
+      int4 h = hn;
+      h.x = uniform(h.x);
+      h.y = uniform(h.y);
+      h.z = uniform(h.z);
+      if (ch + 1 < nch) hn = hdr[ch + 1];
+      const int lo = h.x, hi = h.y;
+      if (h.z + (hi - lo) > committed) {
+        double2 v[REFILL / 64];
+#pragma unroll
+        for (int t = 0; t < REFILL / 64; ++t) {
+          const int e = committed + 64 * t + lane;
+          v[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
+        }
+#pragma unroll
+        for (int t = 0; t < REFILL / 64; ++t)
+          sring[(committed + 64 * t + lane) & (RING - 1)] = v[t];
+        committed += REFILL;
+        __syncthreads();
+      }
+      const int bhi = (hi - 1) >> 3, blo = lo >> 3;
+      const int base = h.z + hi - 1;
+      if (lane < KMAX) {
+        const bool in = lane >= lo && lane < hi;
+        const double2 g = sring[(base - lane) & (RING - 1)];
+        stab[lane] = in ? g : make_double2(1.0, 0.0);
+      }
+      __syncthreads();
+      double carry = 0.0;
+#pragma unroll
+      for (int B = NB - 1; B >= 0; --B) {
+        if (B > bhi || B < blo) continue;
+        const int top = (8 * B + 7 < KMAX - 1) ? 8 * B + 7 : KMAX - 2;
+        if (B == bhi) carry = z[top + 1];
+        double2 cs[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cs[t] = stab[8 * B + t];
+#pragma unroll
+        for (int t = 7; t >= 0; --t) {
+          const int i = 8 * B + t;
+          if (i > KMAX - 2) continue;
+          const double zi = z[i];
+          z[i + 1] = cs[t].y * zi + cs[t].x * carry;
+          carry = cs[t].x * zi - cs[t].y * carry;
+        }
+        if (B == blo) z[8 * B] = carry;
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- C. eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
+  double lam = in_r ? p.lam[al * kp + lane] : 0.0;
+  if (p.guard) {
+    const double mn = uniform(wave_min(in_r ? lam : 1e300));
+    if (mn < 0.0) lam = lam - mn;
+  }
+  lam = lam * 1.0 + (double)(k - 1) / p.inflation;
+  if (lane < KMAX) {
+    sf[lane] = in_r ? sqrt((double)(k - 1) / lam) : 0.0;
+    sg[lane] = in_r ? 1.0 / lam : 0.0;
+    sb[lane] = in_r ? p.b[al * kp + lane] : 0.0;
+  }
+  // T = V diag(f) V^T on f64 MFMA in IPP tile rows per pass; V is staged
+  // 16 columns at a time ([column][row] in LDS).  The first pass also forms
+  // st = diag(1/lambda) V^T b (letkf_core.F90:129-195 regrouped).
+  const int i16 = lane & 15, q = lane >> 4;
+  double* sVc = sbuf;
+  double wbar = 0.0;
+#pragma unroll
+  for (int pass = 0; pass < TT / IPP; ++pass) {
+    d4 acc[IPP][TT];
+#pragma unroll
+    for (int I = 0; I < IPP; ++I)
+#pragma unroll
+      for (int J = 0; J < TT; ++J) acc[I][J] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < TT; ++c) {
+      __syncthreads();
+      if (lane < KMAX) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) sVc[t * LDV + lane] = z[16 * c + t];
+      }
+      __syncthreads();
+      if (pass == 0 && lane < 16) {
+        double tb = 0.0;
+        for (int r = 0; r < KMAX; ++r) tb = fma(sVc[lane * LDV + r], sb[r], tb);
+        st[16 * c + lane] = tb * sg[16 * c + lane];
+      }
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int t = 4 * s4 + q;                      // column within the chunk
+        const double fc = sf[16 * c + t];
+        double fr[TT];
+#pragma unroll
+        for (int J = 0; J < TT; ++J) fr[J] = sVc[t * LDV + 16 * J + i16];
+#pragma unroll
+        for (int I = 0; I < IPP; ++I)
+#pragma unroll
+          for (int J = 0; J < TT; ++J)
+            acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[pass * IPP + I], fc * fr[J],
+                                                             acc[I][J], 0, 0, 0);
+      }
+    }
+    if (pass == 0) {
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < KMAX; ++c) wbar = fma(z[c], st[c], wbar);
+      if (lane < KMAX) sw[lane] = in_r ? wbar : 0.0;
+      __syncthreads();
+    }
+    // trans(f, e) = T(f, e) + overline_w(f)  (letkf_core.F90:221-225)
+#pragma unroll
+    for (int I = 0; I < IPP; ++I)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double wf = sw[16 * (pass * IPP + I) + q + 4 * r];
+#pragma unroll
+        for (int J = 0; J < TT; ++J) acc[I][J][r] += wf;
+      }
+    if (p.trans_out) {
+      double* tr = p.trans_out + a * (long long)k * k;
+#pragma unroll
+      for (int I = 0; I < IPP; ++I)
+#pragma unroll
+        for (int J = 0; J < TT; ++J)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int f = 16 * (pass * IPP + I) + q + 4 * r, e = 16 * J + i16;
+            if (f < k && e < k) tr[(long long)e * k + f] = acc[I][J][r];
+          }
+    }
+    if (p.xb) {
+      // the B operand of the update, over the (consumed) reflector block
+#pragma unroll
+      for (int I = 0; I < IPP; ++I)
+#pragma unroll
+        for (int J = 0; J < TT; ++J)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) refl[tile_off<KMAX>(pass * IPP + I, J, r, lane)] = acc[I][J][r];
+    }
+  }
+  if (p.w_out && in_r) p.w_out[a * (long long)k + lane] = wbar;
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_update(EigParams p) {
+  constexpr int TT = KMAX / 16;
+  constexpr int LDV = KMAX + 1;
+  __shared__ double sV[16 * LDV];
+  const int lane = threadIdx.x;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  if (p.status[a] != 0) return;
+  const int k = p.k, kp = p.kp;
+  const bool in_r = lane < k;
+  const int i16 = lane & 15, q = lane >> 4;
+  const double* __restrict__ tb = p.refl + al * (long long)kp * kp;
+  d4 acc[TT][TT];
+#pragma unroll
+  for (int I = 0; I < TT; ++I)
+#pragma unroll
+    for (int J = 0; J < TT; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[I][J][r] = tb[tile_off<KMAX>(I, J, r, lane)];
+
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
+  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  const long long lat_ext = p.g.nlat + 2 * p.halo;
+  int sk = 1, ek = p.nz;
+  if (p.vertical) {
+    sk = kc * p.cube_nz + 1;
+    ek = min(sk + p.cube_nz - 1, p.nz);
+  }
+  int rs[kMaxVars], rl[kMaxVars];
+  int nrun = 0, nvec = 0;
+  if (!p.vertical) {
+    rs[0] = 0; rl[0] = p.n; nrun = 1;
+  } else {
+    for (int vv = 0; vv < p.n_var; ++vv) {
+      const int start = p.var_start[vv] - 1;
+      if (p.var_lev[vv] == 1) {
+        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
+      } else {
+        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
+      }
+    }
+  }
+  for (int t = 0; t < nrun; ++t) nvec += rl[t];
+  auto sidx = [&](int v) {
+    int rem = v;
+    for (int t = 0; t < nrun; ++t) {
+      if (rem < rl[t]) return rs[t] + rem;
+      rem -= rl[t];
+    }
+    return -1;
+  };
+  // two wavefronts per SIMD hide the loads of one group behind the other
+  // wavefront's MFMAs, so there is no register prefetch here
+  for (int lo = si; lo <= ei; ++lo) {
+    for (int la = sj; la <= ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int v0 = 0; v0 < nvec; v0 += 16) {
+        __syncthreads();
+        for (int u = 0; u < 16; ++u) {
+          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+          if (lane < KMAX) sV[u * LDV + lane] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + lane] : 0.0;
+        }
+        double ab4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int v = v0 + q + 4 * r;
+          const int s_i = (v < nvec) ? sidx(v) : -1;
+          ab4[r] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
+        }
+        __syncthreads();
+        d4 out[TT];
+#pragma unroll
+        for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int sK = 0; sK < KMAX / 4; ++sK) {
+          const double xa_frag = sV[i16 * LDV + 4 * sK + q];
+#pragma unroll
+          for (int J = 0; J < TT; ++J)
+            out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa_frag, acc[sK >> 2][J][sK & 3],
+                                                          out[J], 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int v = v0 + q + 4 * r;
+          const int st_i = (v < nvec) ? sidx(v) : -1;
+          const double ab = ab4[r];
+          double sum = 0.0;
+          double xe[TT];
+#pragma unroll
+          for (int J = 0; J < TT; ++J) {
+            const int e = 16 * J + i16;
+            xe[J] = (e < k) ? out[J][r] + ab : 0.0;
+            sum += xe[J];
+          }
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
+          const double mean = sum / k;
+          if (st_i >= 0) {
+            const long long o = off + st_i;
+#pragma unroll
+            for (int J = 0; J < TT; ++J) {
+              const int e = 16 * J + i16;
+              if (e < k) p.xa[o * k + e] = xe[J] - mean;
+            }
+            if (i16 == 0) p.axa[o] = mean;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k = 65..128: the matrix does not fit a wavefront's registers, so the two
 // per-matrix phases keep it in LDS (one 128-thread workgroup per matrix,
 // thread r owns row r); the QL recurrence is k_ql_log<128>.
@@ -898,25 +1284,40 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
+    if (p.apply_fused) {
+      hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+      if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(64), 0, st, p);
+    }
   } else if (p.k <= 64) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
+    if (p.apply_fused) {
+      hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+      if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(64), 0, st, p);
+    }
   } else if (p.k <= 128) {
     hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);
+    if ((e = mark(3)) != hipSuccess) return e;
   } else {
     return hipErrorInvalidValue;
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  return mark(3);
+  return mark(4);
 }
 
 }  // namespace dida

@@ -111,6 +111,16 @@ int tridiag_variant() {
   return v;
 }
 
+// Eigenvector/update variant for k <= 64: DIDA_APPLY=fused selects the
+// single k_apply kernel, anything else k_vec + k_update.
+int apply_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("DIDA_APPLY");
+    return (e && std::strcmp(e, "fused") == 0) ? 1 : 0;
+  }();
+  return v;
+}
+
 // bytes of split-eigensolver scratch per analysis
 size_t eig_bytes_per_analysis(int k) {
   const int kp = kp_of(k);
@@ -162,6 +172,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.rot_cap = rot_cap;
   ep.hdr_cap = hdr_cap;
   ep.tridiag_lds = tridiag_variant();
+  ep.apply_fused = apply_variant();
   ep.nchase = s.nchase.as<int>();
   ep.status = xp.status;
   ep.nzc = xp.nzc;
@@ -540,15 +551,15 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
   hipStream_t st = h->stream;
   if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
   if (!h->have_obs) return set_error(DIDA_ERR_STATE, "set_obs was not called");
-  hipEvent_t ev[8];
+  hipEvent_t ev[9];
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   struct EvGuard {
     hipEvent_t* e;
     ~EvGuard() {
-      for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
+      for (int i = 0; i < 9; ++i) (void)hipEventDestroy(e[i]);
     }
   } guard{ev};
-  float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[3] = {0, 0, 0};
+  float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[4] = {0, 0, 0, 0};
   HIP_TRY(hipEventRecord(ev[0], st));
 
   Geom g;
@@ -703,7 +714,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipEventSynchronize(ev[3]));
     if (split) {
-      for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < 4; ++i) {
         float t = 0;
         HIP_TRY(hipEventElapsedTime(&t, ev[4 + i], ev[5 + i]));
         ms_k[i] += t;
@@ -737,6 +748,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     stats->ms_tridiag = ms_k[0];
     stats->ms_ql = ms_k[1];
     stats->ms_apply = ms_k[2];
+    stats->ms_update = ms_k[3];
   }
   return flagged;
 }

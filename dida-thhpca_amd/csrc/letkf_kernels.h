@@ -109,6 +109,7 @@ struct EigParams {
   int4* hdr;                // batch-local, hdr_cap chases (lo, hi, off) per analysis
   int rot_cap, hdr_cap;
   int tridiag_lds;          // k <= 64: tridiagonalise in LDS (1) or in VGPRs (0)
+  int apply_fused;          // k <= 64: k_apply (1) or k_vec + k_update (0)
   int* nchase;              // batch-local
   int* status;              // global index
   int nzc, vertical, cube_nz, nz, halo, n, n_var;
@@ -168,7 +169,8 @@ hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
-// ev: null, or 4 events recorded before k_tridiag, k_ql_log, k_apply and after
+// ev: null, or 5 events recorded before the tridiagonalisation, the QL, the
+// eigenvector/transform kernel and the update kernel, and after
 hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p);
 // halo of a [lon + 2h][lat + 2h][n] field with `inner` values per element
 hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);

@@ -75,7 +75,8 @@ class DidaStats(C.Structure):
                 ("max_rows", C.c_int), ("n_flagged", C.c_int), ("rows_over_10000", C.c_int),
                 ("ms_select", C.c_double), ("ms_products", C.c_double),
                 ("ms_transform", C.c_double), ("ms_total", C.c_double),
-                ("ms_tridiag", C.c_double), ("ms_ql", C.c_double), ("ms_apply", C.c_double)]
+                ("ms_tridiag", C.c_double), ("ms_ql", C.c_double), ("ms_apply", C.c_double),
+                ("ms_update", C.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -106,7 +106,9 @@ typedef struct dida_letkf_stats {
     double ms_total;        /* device time of the whole call */
     double ms_tridiag;      /* of ms_transform: Householder tridiagonalisation kernel */
     double ms_ql;           /* of ms_transform: QL recurrence kernel */
-    double ms_apply;        /* of ms_transform: eigenvector, transform and update kernel */
+    double ms_apply;        /* of ms_transform: eigenvector and transform kernel (k <= 64
+                               with the fused kernel: also the update) */
+    double ms_update;       /* of ms_transform: ensemble update kernel (0 when fused) */
 } dida_letkf_stats;
 
 typedef struct dida_letkf_handle dida_letkf_handle;

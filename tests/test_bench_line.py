@@ -14,8 +14,8 @@ def test_line_has_contract_keys():
     from dida_letkf import synth
     cfg = synth.make_config("c2")
     args = argparse.Namespace(mixed=2, steps=5, warmup=1, config="c2")
-    km = dict(select=2.8, products=9.6, transform=41.9, tridiag=8.5, ql=9.0, apply=24.0)
-    st = dict(total_rows=72_613_192, max_rows=2656)
+    km = dict(select=2.8, products=9.6, transform=41.9, tridiag=8.5, ql=9.0, apply=24.0, update=1.5)
+    st = dict(total_rows=72_613_192, max_rows=2656, ms_update=1.5)
     res = {2: dict(value=1.18e6, ms_step=54.9, kernel_ms=km, stats=st),
            0: dict(value=1.0e6, ms_step=64.8, kernel_ms=dict(km, products=19.9), stats=st)}
     line = bench.make_line(args, cfg, cfg.n_analyses(), 500_000, res, {}, None)
@@ -24,10 +24,14 @@ def test_line_has_contract_keys():
                 "roofline", "cpu_baseline"):
         assert key in line
     r = line["roofline"]
-    assert r["kernel"] == "k_apply" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
+    assert r["kernel"] == "k_vec" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-4
     k, m, n_a = cfg.k, cfg.n, cfg.n_analyses()
-    fl = n_a * ((4 / 3) * k**3 + 6 * k**3 + 2 * k**3 + 4 * k**2 + 2 * k**2 * m)
+    fl = n_a * ((4 / 3) * k**3 + 6 * k**3 + 2 * k**3 + 4 * k**2)
     assert abs(r["achieved"] - fl / 24.0e-3 / 1e12) < 1e-3
+    # an HBM-bound update, when it dominates, is priced in bytes
+    r2 = bench.roofline(dict(update=3.0, apply=1.0), st, cfg, n_a, 2)
+    assert r2["bound"] == "hbm" and r2["unit"] == "TB/s"
+    assert abs(r2["achieved"] - n_a * (16 * k * m + 16 * m) / 3.0e-3 / 1e12) < 1e-3
     assert line["mixed_over_fp64"] == round(1.18e6 / 1.0e6, 3)
     json.dumps(line)

@@ -167,29 +167,12 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
   for (int l = 0; l < k && status == 0; ++l) {
     int iter = 0;
     for (;;) {
-      // first m >= l with a negligible e[m] (or k - 1); four entries are
-      // read per step so the scan pays one LDS latency per four elements
       int m = l;
-      for (;;) {
-        double dv[5], ev[4];
-#pragma unroll
-        for (int t = 0; t < 5; ++t) dv[t] = (m + t < KMAX) ? sd[m + t][lane] : 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) ev[t] = (m + t < KMAX) ? se[m + t][lane] : 0.0;
-        int hit = 4;
-#pragma unroll
-        for (int t = 3; t >= 0; --t) {
-          const double dd = fabs(dv[t]) + fabs(dv[t + 1]);
-          const double em = fabs(ev[t]);
-          if (m + t >= k - 1 || em <= eps * dd || em <= eps * tnorm) hit = t;
-        }
-        if (hit < 4) {
-          m += hit;
-          break;
-        }
-        m += 4;
+      for (; m < k - 1; ++m) {
+        const double dd = fabs(sd[m][lane]) + fabs(sd[m + 1][lane]);
+        const double em = fabs(se[m][lane]);
+        if (em <= eps * dd || em <= eps * tnorm) break;
       }
-      if (m > k - 1) m = k - 1;
       if (m == l) break;
       if (++iter > 30) { status = l + 1; break; }
       if (nhdr >= p.hdr_cap || nrot + (m - l) > p.rot_cap) { status = -1; break; }

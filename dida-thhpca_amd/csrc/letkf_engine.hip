@@ -111,6 +111,18 @@ int tridiag_variant() {
   return v;
 }
 
+// Products variant for the horizontal loop with k <= 64: DIDA_PRODUCTS=column
+// selects one selection list per analysis, anything else tiles of 8
+// analyses sharing their staged union of rows.
+int products_tiled() {
+  static const int v = [] {
+    const char* e = std::getenv("DIDA_PRODUCTS");
+    return (e && std::strcmp(e, "column") == 0) ? 0 : 1;
+  }();
+  return v;
+}
+constexpr int kTile = 8;
+
 // Eigenvector/update variant for k <= 64: DIDA_APPLY=fused selects the
 // single k_apply kernel, anything else k_vec + k_update.
 int apply_variant() {
@@ -258,6 +270,7 @@ struct dida_letkf_handle {
   DevBuf s_cos, s_sin, s_lonr, s_row0, s_nrow;
   DevBuf yb32, rinv32;
   DevBuf d_hyai, d_hybi;
+  DevBuf row_site, nobsl_dev;         // tile-mode selection and row counts
   // work
   DevBuf cnt, off, scan_tmp, row_idx, row_gc, Abuf, bbuf, status, trans, wout;
   DevBuf x_xb, x_axb, x_xa, x_axa;   // host-entry staging
@@ -362,6 +375,10 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->bdeg_lat = 180.0 / h->nb_lat;
   h->bdeg_lon = 360.0 / h->nb_lon;
   h->max_batch = max_batch_columns(c);
+  if (h->nzc == 1 && h->kp <= 64 && products_tiled()) {
+    // tiles never straddle batches, so results do not depend on the batching
+    h->max_batch = std::max<long long>(kTile, (h->max_batch / kTile) * kTile);
+  }
   *out = h;
   return DIDA_OK;
 }
@@ -374,6 +391,7 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->val_out, &h->hist, &h->bin_start, &h->sort_tmp, &h->s_cos, &h->s_sin,
                     &h->s_lonr, &h->s_row0, &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai,
                     &h->d_hybi, &h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc,
+                    &h->row_site, &h->nobsl_dev,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
                     &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
@@ -589,6 +607,9 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
   long long total_rows = 0;
   int max_rows = 0, over_cap = 0;
   std::vector<long long> cnt_host;
+  std::vector<int> nob_host;
+  // tiles of analyses share their staged obs rows (horizontal loop, k <= 64)
+  const bool tiled = h->nzc == 1 && h->kp <= 64 && products_tiled();
 
   for (long long h0 = 0; h0 < h->n_h; h0 += mb) {
     const long long h1 = std::min(h->n_h, h0 + mb);
@@ -615,29 +636,43 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     sp.row_idx = nullptr;
     sp.row_gc = nullptr;
     sp.cap = 0;
-    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, sizeof(long long) * (ncol + 1), st));
-    HIP_TRY(launch_select(st, sp, false));
+    sp.tile = tiled ? kTile : 0;
+    sp.row_site = nullptr;
+    // count pass -> offsets -> fill pass; per analysis (column mode) or per
+    // tile of kTile analyses (tile mode: the union of their candidate rows)
+    const int nlist = tiled ? (ncol + kTile - 1) / kTile : ncol;
+    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, sizeof(long long) * (nlist + 1), st));
+    if (tiled) HIP_TRY(launch_select_tile(st, sp, false));
+    else HIP_TRY(launch_select(st, sp, false));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(h->scan_tmp.p, scan_bytes, h->cnt.as<long long>(),
-                                             h->off.as<long long>(), ncol + 1, st));
-    cnt_host.resize(ncol + 1);
-    HIP_TRY(hipMemcpyAsync(cnt_host.data(), h->off.p, sizeof(long long) * (ncol + 1),
+                                             h->off.as<long long>(), nlist + 1, st));
+    cnt_host.resize(nlist + 1);
+    HIP_TRY(hipMemcpyAsync(cnt_host.data(), h->off.p, sizeof(long long) * (nlist + 1),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    const long long rows = cnt_host[ncol];
-    for (int i = 0; i < ncol; ++i) {
-      long long r = cnt_host[i + 1] - cnt_host[i];
-      max_rows = std::max<long long>(max_rows, r);
-      if (r > 10000) ++over_cap;
-      if (c.keep_transforms)
-        for (int kc = 0; kc < h->nzc; ++kc) h->nobsl_host[(h0 + i) * h->nzc + kc] = (int)r;
+    const long long rows = cnt_host[nlist];
+    if (!tiled) {
+      for (int i = 0; i < ncol; ++i) {
+        long long r = cnt_host[i + 1] - cnt_host[i];
+        max_rows = std::max<long long>(max_rows, r);
+        if (r > 10000) ++over_cap;
+        if (c.keep_transforms)
+          for (int kc = 0; kc < h->nzc; ++kc) h->nobsl_host[(h0 + i) * h->nzc + kc] = (int)r;
+      }
+      total_rows += rows * h->nzc;
     }
-    total_rows += rows * h->nzc;
     RC_TRY(h->row_idx.ensure(sizeof(int) * std::max(rows, 1LL)));
-    RC_TRY(h->row_gc.ensure(sizeof(double) * std::max(rows, 1LL)));
     sp.row_idx = h->row_idx.as<int>();
-    sp.row_gc = h->row_gc.as<double>();
     sp.cap = rows;
-    HIP_TRY(launch_select(st, sp, true));
+    if (tiled) {
+      RC_TRY(h->row_site.ensure(sizeof(int) * std::max(rows, 1LL)));
+      sp.row_site = h->row_site.as<int>();
+      HIP_TRY(launch_select_tile(st, sp, true));
+    } else {
+      RC_TRY(h->row_gc.ensure(sizeof(double) * std::max(rows, 1LL)));
+      sp.row_gc = h->row_gc.as<double>();
+      HIP_TRY(launch_select(st, sp, true));
+    }
     HIP_TRY(hipEventRecord(ev[1], st));
 
     ProdParams pp{};
@@ -672,7 +707,23 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     pp.h0 = (int)h0;
     pp.A = h->Abuf.as<double>();
     pp.b = h->bbuf.as<double>();
-    HIP_TRY(launch_products(st, pp, c.precision));
+    if (tiled) {
+      pp.tile = kTile;
+      pp.row_site = h->row_site.as<int>();
+      pp.s_cos = h->s_cos.as<double>();
+      pp.s_sin = h->s_sin.as<double>();
+      pp.s_lonr = h->s_lonr.as<double>();
+      pp.local_dist = c.local_dist;
+      pp.gc_filter = c.gc_filter;
+      RC_TRY(h->nobsl_dev.ensure(sizeof(int) * std::max(ncol, 1)));
+      pp.nobsl = h->nobsl_dev.as<int>();
+      HIP_TRY(launch_products_tile(st, pp, c.precision));
+      nob_host.resize(ncol);
+      HIP_TRY(hipMemcpyAsync(nob_host.data(), h->nobsl_dev.p, sizeof(int) * ncol,
+                             hipMemcpyDeviceToHost, st));
+    } else {
+      HIP_TRY(launch_products(st, pp, c.precision));
+    }
     HIP_TRY(hipEventRecord(ev[2], st));
 
     XformParams xp{};
@@ -713,6 +764,16 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     RC_TRY(run_transform(st, h->eig, xp, c.transform_kernel, ev + 4, &split));
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipEventSynchronize(ev[3]));
+    if (tiled) {
+      // local obs counts of this batch (copied after the products kernel)
+      for (int i = 0; i < ncol; ++i) {
+        const long long r = nob_host[i];
+        max_rows = std::max<long long>(max_rows, r);
+        if (r > 10000) ++over_cap;
+        total_rows += r;
+        if (c.keep_transforms) h->nobsl_host[h0 + i] = (int)r;
+      }
+    }
     if (split) {
       for (int i = 0; i < 4; ++i) {
         float t = 0;

@@ -43,6 +43,10 @@ struct SelectParams {
   int* row_idx;
   double* row_gc;
   long long cap;          // fill pass: capacity of row_idx / row_gc
+  // tile mode (k_select_tile): tiles of `tile` consecutive analyses of
+  // [h0, h1); the union of their candidate rows, with the sorted site index
+  int tile;
+  int* row_site;
 };
 
 struct ProdParams {
@@ -69,6 +73,16 @@ struct ProdParams {
   double* b;                 // batch-local, kp per analysis
   const int* order;          // optional processing order (batch-local analysis offsets)
   int n_blocks;              // set by the launcher
+  // tile mode (horizontal loop, k <= 64): row_off indexes tiles of `tile`
+  // consecutive analyses; each union row carries its sorted site index
+  int tile;
+  const int* row_site;
+  const double* s_cos;
+  const double* s_sin;
+  const double* s_lonr;
+  double local_dist;
+  int gc_filter;
+  int* nobsl;                // tile mode: local obs rows per analysis (batch-local)
 };
 
 struct XformParams {
@@ -166,7 +180,9 @@ hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, con
                               const double* lon, const int* row_ptr, double* s_cos, double* s_sin,
                               double* s_lonr, int* s_row0, int* s_nrow);
 hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
+hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
+hipError_t launch_products_tile(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
 // ev: null, or 5 events recorded before the tridiagonalisation, the QL, the

@@ -16,6 +16,8 @@
 // Operands: A[i][k] at lane i + 16*(k/K0), B[k][j] at lane j + 16*(k/K0),
 // with K0 = 1 (x4 shapes) or 8 consecutive k per lane (16x16x32 bf16).
 #include "letkf_kernels.h"
+
+#include <type_traits>
 #include "letkf_device.h"
 
 namespace dida {
@@ -77,6 +79,48 @@ __device__ __forceinline__ void column_center(const Geom& g, int ah, int* sj, in
   *ei = min(*si + g.cube_n - 1, g.nlon);
 }
 
+// Bins a disc of half-angle th (degrees) around (clat, clon) touches: latitude
+// bands [b0, b1] and one or two longitude bin ranges [ja[r], jb[r]].
+struct BinRange {
+  int b0, b1, nr;
+  int ja[2], jb[2];
+};
+
+__device__ __forceinline__ BinRange bin_range(const SelectParams& p, double clat, double clon,
+                                              double th) {
+  BinRange R;
+  R.b0 = (int)floor((clat - th + 90.0) / p.bdeg_lat);
+  R.b1 = (int)floor((clat + th + 90.0) / p.bdeg_lat);
+  R.b0 = R.b0 < 0 ? 0 : R.b0;
+  R.b1 = R.b1 >= p.nb_lat ? p.nb_lat - 1 : R.b1;
+  bool full = fabs(clat) + th >= 89.99;
+  double dlam = 360.0;
+  if (!full) {
+    double sr = sin(th * (M_PI / 180.0)) / cos(fabs(clat) * (M_PI / 180.0));
+    if (sr >= 1.0) full = true;
+    else dlam = asin(sr) * (180.0 / M_PI) + 1e-3;
+  }
+  double clw = fmod(clon, 360.0);
+  if (clw < 0.0) clw += 360.0;
+  R.nr = 1;
+  R.ja[0] = 0;
+  R.jb[0] = p.nb_lon - 1;
+  if (!full) {
+    int j0 = (int)floor((clw - dlam) / p.bdeg_lon);
+    int j1 = (int)floor((clw + dlam) / p.bdeg_lon);
+    if (j1 - j0 + 1 < p.nb_lon) {
+      if (j0 < 0) {
+        R.ja[0] = j0 + p.nb_lon; R.jb[0] = p.nb_lon - 1; R.ja[1] = 0; R.jb[1] = j1; R.nr = 2;
+      } else if (j1 >= p.nb_lon) {
+        R.ja[0] = j0; R.jb[0] = p.nb_lon - 1; R.ja[1] = 0; R.jb[1] = j1 - p.nb_lon; R.nr = 2;
+      } else {
+        R.ja[0] = j0; R.jb[0] = j1;
+      }
+    }
+  }
+  return R;
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   const int lane = threadIdx.x & 63;
@@ -89,36 +133,10 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   const double clon = (double)p.g.glon[(si + ei) / 2 - 1];
   const double c2 = cos(deg2rad_ref(clat)), s2 = sin(deg2rad_ref(clat));
   const double l2 = deg2rad_ref(clon);
-  const double th = p.theta_deg;
-  int b0 = (int)floor((clat - th + 90.0) / p.bdeg_lat);
-  int b1 = (int)floor((clat + th + 90.0) / p.bdeg_lat);
-  b0 = b0 < 0 ? 0 : b0;
-  b1 = b1 >= p.nb_lat ? p.nb_lat - 1 : b1;
-  bool full = fabs(clat) + th >= 89.99;
-  double dlam = 360.0;
-  if (!full) {
-    double sr = sin(th * (M_PI / 180.0)) / cos(fabs(clat) * (M_PI / 180.0));
-    if (sr >= 1.0) full = true;
-    else dlam = asin(sr) * (180.0 / M_PI) + 1e-3;
-  }
-  double clw = fmod(clon, 360.0);
-  if (clw < 0.0) clw += 360.0;
-  int ja[2], jb[2], nr = 1;
-  ja[0] = 0;
-  jb[0] = p.nb_lon - 1;
-  if (!full) {
-    int j0 = (int)floor((clw - dlam) / p.bdeg_lon);
-    int j1 = (int)floor((clw + dlam) / p.bdeg_lon);
-    if (j1 - j0 + 1 < p.nb_lon) {
-      if (j0 < 0) {
-        ja[0] = j0 + p.nb_lon; jb[0] = p.nb_lon - 1; ja[1] = 0; jb[1] = j1; nr = 2;
-      } else if (j1 >= p.nb_lon) {
-        ja[0] = j0; jb[0] = p.nb_lon - 1; ja[1] = 0; jb[1] = j1 - p.nb_lon; nr = 2;
-      } else {
-        ja[0] = j0; jb[0] = j1;
-      }
-    }
-  }
+  const BinRange R = bin_range(p, clat, clon, p.theta_deg);
+  const int b0 = R.b0, b1 = R.b1, nr = R.nr;
+  const int* ja = R.ja;
+  const int* jb = R.jb;
   long long count = 0;
   long long base = FILL ? p.off[wid] : 0;
   for (int b = b0; b <= b1; ++b) {
@@ -157,6 +175,77 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   if (!FILL) {
     count = wave_sum_ll(count);
     if (lane == 0) p.cnt[wid] = count;
+  }
+}
+
+// Tile mode: one wavefront per tile of p.tile consecutive analyses of
+// [h0, h1).  Lists every row of every site within R_tile of the tile's
+// middle centre, R_tile = 2c + the largest distance from the middle centre to
+// a centre of the tile (so the union of the analyses' supports is covered, by
+// the triangle inequality), in bin order, with the sorted site index.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_select_tile(SelectParams p) {
+  const int lane = threadIdx.x & 63;
+  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int a0 = p.h0 + t * p.tile;
+  if (a0 >= p.h1) return;
+  const int a1 = min(a0 + p.tile, p.h1);
+  auto centre = [&](int ah, double* la, double* lo) {
+    int sj, ej, si, ei;
+    column_center(p.g, ah, &sj, &ej, &si, &ei);
+    *la = (double)p.g.glat[(sj + ej) / 2 - 1];
+    *lo = (double)p.g.glon[(si + ei) / 2 - 1];
+  };
+  double clat, clon;
+  centre((a0 + a1 - 1) / 2, &clat, &clon);
+  const double c2 = cos(deg2rad_ref(clat)), s2 = sin(deg2rad_ref(clat));
+  const double l2 = deg2rad_ref(clon);
+  double spread = 0.0;
+  for (int ah = a0; ah < a1; ++ah) {
+    double la, lo;
+    centre(ah, &la, &lo);
+    spread = fmax(spread, latlon_dist_pre(cos(deg2rad_ref(la)), sin(deg2rad_ref(la)),
+                                          deg2rad_ref(lo), c2, s2, l2));
+  }
+  const double r_tile = 2.0 * p.local_dist + spread * (1.0 + 1e-9) + 1e-3;
+  const double th = (r_tile / 6371.39306640625) * (180.0 / M_PI) * (1.0 + 1e-6) + 1e-2;
+  const BinRange R = bin_range(p, clat, clon, th);
+  long long count = 0;
+  long long base = FILL ? p.off[t] : 0;
+  for (int b = R.b0; b <= R.b1; ++b) {
+    for (int r = 0; r < R.nr; ++r) {
+      const int s0 = p.bin_start[b * p.nb_lon + R.ja[r]];
+      const int s1 = p.bin_start[b * p.nb_lon + R.jb[r] + 1];
+      for (int cs = s0; cs < s1; cs += 64) {
+        const int s = cs + lane;
+        int rc = 0;
+        if (s < s1) {
+          const double d = latlon_dist_pre(p.s_cos[s], p.s_sin[s], p.s_lonr[s], c2, s2, l2);
+          if (d <= r_tile) rc = p.s_nrow[s];
+        }
+        if (!FILL) {
+          count += rc;
+        } else {
+          const int pre = wave_excl_scan(rc, lane);
+          const int tot = __shfl(pre + rc, 63, 64);
+          if (rc) {
+            const long long pos = base + pre;
+            const int r0 = p.s_row0[s];
+            for (int u = 0; u < rc; ++u) {
+              if (pos + u < p.cap) {
+                p.row_idx[pos + u] = r0 + u;
+                p.row_site[pos + u] = s;
+              }
+            }
+          }
+          base += tot;
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    count = wave_sum_ll(count);
+    if (lane == 0) p.cnt[t] = count;
   }
 }
 
@@ -838,12 +927,229 @@ hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, con
   return hipGetLastError();
 }
 
+hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill) {
+  const int nt = (p.h1 - p.h0 + p.tile - 1) / p.tile;
+  if (nt <= 0) return hipSuccess;
+  const int blocks = (nt + 3) / 4;
+  if (fill) hipLaunchKernelGGL(k_select_tile<true>, dim3(blocks), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL(k_select_tile<false>, dim3(blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
 hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill) {
   const int ncol = p.h1 - p.h0;
   if (ncol <= 0) return hipSuccess;
   const int blocks = (ncol + 3) / 4;
   if (fill) hipLaunchKernelGGL(k_select<true>, dim3(blocks), dim3(256), 0, st, p);
   else hipLaunchKernelGGL(k_select<false>, dim3(blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tile-union products (horizontal loop, k <= 64).  A workgroup takes a tile
+// of NA consecutive analyses; the union of their candidate rows (k_select_tile)
+// is staged through LDS CH rows at a time, once for the whole tile, and every
+// wavefront keeps its own analysis' rows: it evaluates the same GC weight as
+// k_select for each staged row and compacts the rows with weight > 0 (ballot
+// + prefix count) into a per-wavefront list, in union (bin) order.  The MFMA
+// accumulation is that of the per-analysis kernels, over the listed rows.
+// ---------------------------------------------------------------------------
+template <int KP, int PREC>
+struct TileTypes {
+  using Y = typename std::conditional<PREC == 0, double, float>::type;
+};
+
+template <int KP, int PREC, int NA, int CH>
+__global__ __launch_bounds__(NA * 64) void k_products_tile(ProdParams p) {
+  using Y = typename TileTypes<KP, PREC>::Y;
+  constexpr int T = KP / 16;
+  using PT = PairTable<T>;
+  constexpr int NP = PT::N;
+  __shared__ __align__(16) Y sY[CH * KP];
+  __shared__ int sRow[CH], sSite[CH];
+  __shared__ int sLi[NA][CH];
+  __shared__ double sW[NA][CH], sWd[NA][CH];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, q = lane >> 4;
+  const int tile = blockIdx.x;
+  const long long ta0 = (long long)tile * p.tile;          // batch-local first analysis
+  const long long a_loc = ta0 + wave;
+  const bool act = wave < p.tile && a_loc < p.n_an;
+  const long long u0 = p.row_off[tile], u1 = p.row_off[tile + 1];
+  // this wavefront's analysis centre
+  double c2 = 0.0, s2 = 0.0, l2 = 0.0;
+  if (act) {
+    int sj, ej, si, ei;
+    column_center(p.g, p.h0 + (int)a_loc, &sj, &ej, &si, &ei);
+    const double clat = (double)p.g.glat[(sj + ej) / 2 - 1];
+    const double clon = (double)p.g.glon[(si + ei) / 2 - 1];
+    c2 = cos(deg2rad_ref(clat));
+    s2 = sin(deg2rad_ref(clat));
+    l2 = deg2rad_ref(clon);
+  }
+  using Acc = typename std::conditional<PREC == 0, d4, f4>::type;
+  Acc acc[NP];
+#pragma unroll
+  for (int x = 0; x < NP; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[x][r] = 0;
+  double bacc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  int nsel = 0;
+  const int k = p.k;
+  for (long long c0 = u0; c0 < u1; c0 += CH) {
+    const int nch = (int)min((long long)CH, u1 - c0);
+    __syncthreads();
+    // ---- stage the chunk's rows (y in the storage precision of the mode) ----
+    for (int i = threadIdx.x; i < CH; i += NA * 64) {
+      sRow[i] = (i < nch) ? p.row_idx[c0 + i] : 0;
+      sSite[i] = (i < nch) ? p.row_site[c0 + i] : 0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < CH * KP; e += NA * 64) {
+      const int r = e / KP, m = e % KP;
+      Y v = 0;
+      if (r < nch && m < k) {
+        if constexpr (PREC == 0) v = p.yb64[(long long)sRow[r] * k + m];
+        else v = p.yb32[(long long)sRow[r] * k + m];
+      }
+      sY[e] = v;
+    }
+    // ---- this analysis' rows of the chunk: the weight of k_select / row_weight ----
+    int cnt = 0;
+    if (act) {
+      double w = 0.0, wd = 0.0;
+      bool sel = false;
+      if (lane < nch) {
+        const int st = sSite[lane];
+        const double d = latlon_dist_pre(p.s_cos[st], p.s_sin[st], p.s_lonr[st], c2, s2, l2);
+        const double gc = gc_weight(d * 1.0 / p.local_dist, p.gc_filter);
+        if (gc > 0.0) {
+          const int row = sRow[lane];
+          w = row_weight(p, row, gc, 0.0);
+          wd = w * p.dep[row];
+          sel = true;
+        }
+      }
+      const unsigned long long m = __ballot(sel);
+      cnt = __popcll(m);
+      if (sel) {
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        sLi[wave][rank] = lane;
+        sW[wave][rank] = w;
+        sWd[wave][rank] = wd;
+      }
+    }
+    __syncthreads();
+    if (!act) continue;
+    nsel += cnt;
+    if constexpr (PREC == 2) {
+      // BF16 hi/lo split of sqrt(w) y, 32 rows per MFMA (8 per lane)
+      for (int g = 0; g < cnt; g += 32) {
+        bf8 hi[T], lo[T];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int idx = g + 8 * q + u;
+          float sq = 0.f;
+          double wd = 0.0;
+          int li = 0;
+          if (idx < cnt) {
+            li = sLi[wave][idx];
+            sq = sqrtf((float)sW[wave][idx]);
+            wd = sWd[wave][idx];
+          }
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+            const float yv = (idx < cnt) ? (float)sY[li * KP + 16 * t + i16] : 0.f;
+            const float yh = sq * yv;
+            const __bf16 h = (__bf16)yh;
+            hi[t][u] = h;
+            lo[t][u] = (__bf16)(yh - (float)h);
+            bacc[t] = fma(wd, (double)yv, bacc[t]);
+          }
+        }
+#pragma unroll
+        for (int x = 0; x < NP; ++x) {
+          const int I = PT::I(x), J = PT::J(x);
+          acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], hi[J], acc[x], 0, 0, 0);
+          acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], lo[J], acc[x], 0, 0, 0);
+          acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[I], hi[J], acc[x], 0, 0, 0);
+        }
+      }
+    } else {
+      // FP64 / FP32: 4 rows per MFMA (one per lane quarter)
+      for (int g = 0; g < cnt; g += 4) {
+        const int idx = g + q;
+        Y y[T];
+        double w = 0.0, wd = 0.0;
+        if (idx < cnt) {
+          const int li = sLi[wave][idx];
+          w = sW[wave][idx];
+          wd = sWd[wave][idx];
+#pragma unroll
+          for (int t = 0; t < T; ++t) y[t] = sY[li * KP + 16 * t + i16];
+        } else {
+#pragma unroll
+          for (int t = 0; t < T; ++t) y[t] = 0;
+        }
+#pragma unroll
+        for (int x = 0; x < NP; ++x) {
+          const int I = PT::I(x), J = PT::J(x);
+          if constexpr (PREC == 0)
+            acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[I], w * y[J], acc[x], 0, 0, 0);
+          else
+            acc[x] = __builtin_amdgcn_mfma_f32_16x16x4f32(y[I], (float)w * y[J], acc[x], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t) bacc[t] = fma(wd, (double)y[t], bacc[t]);
+      }
+    }
+  }
+  if (!act) return;
+  const long long al = a_loc;
+  double* A = p.A + al * (long long)KP * KP;
+#pragma unroll
+  for (int x = 0; x < NP; ++x) {
+    const int I = PT::I(x), J = PT::J(x);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // f64 16x16x4 output: D[l/16 + 4r][l%16]; f32 16x16x4 / bf16 16x16x32: D[4(l/16) + r][l%16]
+      const int row = (PREC == 0) ? 16 * I + q + 4 * r : 16 * I + 4 * q + r;
+      const int col = 16 * J + i16;
+      const double v = (double)acc[x][r];
+      A[(long long)col * KP + row] = v;
+      if (I != J) A[(long long)row * KP + col] = v;
+    }
+  }
+  double* b = p.b + al * KP;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    double v = bacc[t];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (q == 0) b[16 * t + i16] = v;
+  }
+  if (lane == 0) p.nobsl[al] = nsel;
+}
+
+hipError_t launch_products_tile(hipStream_t st, const ProdParams& p, int precision) {
+  constexpr int NA = 8, CH = 64;
+  if (p.tile != NA) return hipErrorInvalidValue;
+  const long long nt = (p.n_an + NA - 1) / NA;
+  if (nt <= 0) return hipSuccess;
+  const dim3 grid((unsigned)nt), block(NA * 64);
+  if (p.kp == 32) {
+    if (precision == 0) hipLaunchKernelGGL((k_products_tile<32, 0, NA, CH>), grid, block, 0, st, p);
+    else if (precision == 1) hipLaunchKernelGGL((k_products_tile<32, 1, NA, CH>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_products_tile<32, 2, NA, CH>), grid, block, 0, st, p);
+  } else if (p.kp == 64) {
+    if (precision == 0) hipLaunchKernelGGL((k_products_tile<64, 0, NA, CH>), grid, block, 0, st, p);
+    else if (precision == 1) hipLaunchKernelGGL((k_products_tile<64, 1, NA, CH>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_products_tile<64, 2, NA, CH>), grid, block, 0, st, p);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -166,6 +166,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--k", type=int, default=0, help="ensemble size (0: the config's)")
     ap.add_argument("--mixed", type=int, default=2, help="precision of the headline mode (1 fp32, 2 bf16x3)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -198,7 +199,7 @@ def main():
         print(json.dumps(dict(metric="LETKF serving job latency (p50/p99)", serving=out)), flush=True)
         return
     t0 = time.time()
-    inst = synth.generate(args.config, seed=args.seed)
+    inst = synth.generate(args.config, seed=args.seed, k=args.k or None)
     cfg = inst.cfg
     log(f"generated {args.config}: grid {cfg.nlat}x{cfg.nlon}x{cfg.nz} n={cfg.n} k={cfg.k} "
         f"obs rows={inst.obs.n_rows} sites={inst.obs.n_sites} in {time.time() - t0:.1f}s")

@@ -111,16 +111,8 @@ int tridiag_variant() {
   return v;
 }
 
-// Products variant for the horizontal loop with k <= 64: DIDA_PRODUCTS=column
-// selects one selection list per analysis, anything else tiles of 8
-// analyses sharing their staged union of rows.
-int products_tiled() {
-  static const int v = [] {
-    const char* e = std::getenv("DIDA_PRODUCTS");
-    return (e && std::strcmp(e, "column") == 0) ? 0 : 1;
-  }();
-  return v;
-}
+// Analyses per tile of the tile-union products (config products_kernel = 0,
+// horizontal loop, k <= 64).
 constexpr int kTile = 8;
 
 // Eigenvector/update variant for k <= 64: DIDA_APPLY=fused selects the
@@ -277,6 +269,7 @@ struct dida_letkf_handle {
   EigScratch eig;
   std::vector<int> nobsl_host;
   long long max_batch = 0;
+  bool tiled = false;                 // tile-union products
 };
 
 extern "C" {
@@ -333,6 +326,8 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   if (!(c.inflation_factor > 0.0)) return set_error(DIDA_ERR_ARG, "inflation_factor must be > 0");
   if (c.precision < DIDA_PREC_FP64 || c.precision > DIDA_PREC_BF16X3)
     return set_error(DIDA_ERR_ARG, "bad precision %d", c.precision);
+  if (c.products_kernel < 0 || c.products_kernel > 1)
+    return set_error(DIDA_ERR_ARG, "bad products_kernel %d", c.products_kernel);
   if (c.vert_dist > 0.0) {
     if (c.vcoord_mode == DIDA_VCOORD_LNP && (!c.hyai || !c.hybi))
       return set_error(DIDA_ERR_ARG, "LNP vertical mode needs hyai/hybi");
@@ -375,7 +370,8 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->bdeg_lat = 180.0 / h->nb_lat;
   h->bdeg_lon = 360.0 / h->nb_lon;
   h->max_batch = max_batch_columns(c);
-  if (h->nzc == 1 && h->kp <= 64 && products_tiled()) {
+  h->tiled = h->nzc == 1 && h->kp <= 64 && c.products_kernel == 0;
+  if (h->tiled) {
     // tiles never straddle batches, so results do not depend on the batching
     h->max_batch = std::max<long long>(kTile, (h->max_batch / kTile) * kTile);
   }
@@ -609,7 +605,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
   std::vector<long long> cnt_host;
   std::vector<int> nob_host;
   // tiles of analyses share their staged obs rows (horizontal loop, k <= 64)
-  const bool tiled = h->nzc == 1 && h->kp <= 64 && products_tiled();
+  const bool tiled = h->tiled;
 
   for (long long h0 = 0; h0 < h->n_h; h0 += mb) {
     const long long h1 = std::min(h->n_h, h0 + mb);

@@ -38,7 +38,8 @@ class DidaConfig(C.Structure):
                 ("gc_filter", C.c_int), ("gc_filter_vertical", C.c_int), ("precision", C.c_int),
                 ("vcoord_mode", C.c_int), ("ps_index", C.c_int), ("hyai", _dp), ("hybi", _dp),
                 ("p0", C.c_double), ("ptop", C.c_double), ("keep_transforms", C.c_int),
-                ("max_batch_columns", C.c_int), ("transform_kernel", C.c_int)]
+                ("max_batch_columns", C.c_int), ("transform_kernel", C.c_int),
+                ("products_kernel", C.c_int)]
 
 
 class ServeConfig(C.Structure):
@@ -155,7 +156,8 @@ def _p(a, t):
 
 
 def to_c_config(cfg: LetkfConfig, keep_transforms: bool = False,
-                max_batch_columns: int = 0, transform_kernel: int = 0) -> DidaConfig:
+                max_batch_columns: int = 0, transform_kernel: int = 0,
+                products_kernel: int = 0) -> DidaConfig:
     c = DidaConfig()
     lib().dida_letkf_config_default(C.byref(c))
     c.k, c.nlat, c.nlon, c.nz = cfg.k, cfg.nlat, cfg.nlon, cfg.nz
@@ -169,6 +171,7 @@ def to_c_config(cfg: LetkfConfig, keep_transforms: bool = False,
     c.keep_transforms = int(keep_transforms)
     c.max_batch_columns = int(max_batch_columns)
     c.transform_kernel = int(transform_kernel)
+    c.products_kernel = int(products_kernel)
     return c
 
 
@@ -233,10 +236,12 @@ class LetkfEngine:
     """One handle = one letkf_init'ed block (letkf_init.F90:2)."""
 
     def __init__(self, cfg: LetkfConfig, keep_transforms: bool = False,
-                 max_batch_columns: int = 0, transform_kernel: int = 0):
+                 max_batch_columns: int = 0, transform_kernel: int = 0,
+                 products_kernel: int = 0):
         self.cfg = cfg
         self._keep = []
-        c = to_c_config(cfg, keep_transforms, max_batch_columns, transform_kernel)
+        c = to_c_config(cfg, keep_transforms, max_batch_columns, transform_kernel,
+                        products_kernel)
         if cfg.hyai is not None:
             hy = np.ascontiguousarray(cfg.hyai, np.float64)
             hb = np.ascontiguousarray(cfg.hybi, np.float64)
@@ -366,15 +371,15 @@ def obs_operator(case):
 
 
 def letkf_compute_nooverlap(inst, precision: Optional[int] = None, keep_transforms=False,
-                            transform_kernel: int = 0):
+                            transform_kernel: int = 0, products_kernel: int = 0):
     """Convenience: run one analysis cycle of a synth.Instance on the GPU."""
     cfg = inst.cfg
     if precision is not None:
         import copy
         cfg = copy.copy(cfg)
         cfg.precision = precision
-    with LetkfEngine(cfg, keep_transforms=keep_transforms,
-                     transform_kernel=transform_kernel) as eng:
+    with LetkfEngine(cfg, keep_transforms=keep_transforms, transform_kernel=transform_kernel,
+                     products_kernel=products_kernel) as eng:
         eng.set_grid(inst.grid_lat, inst.grid_lon)
         eng.set_obs(inst.obs)
         xa, axa, nflag = eng.letkf_compute(inst.xb, inst.axb)

@@ -92,6 +92,9 @@ typedef struct dida_letkf_config {
     int transform_kernel;   /* 0: split eigensolver (tridiagonalisation,
                                lane-per-matrix QL, rotation apply); 1: the
                                single-kernel transform */
+    int products_kernel;    /* 0: automatic (tiles of 8 analyses sharing their
+                               staged obs rows where the loop is horizontal and
+                               k <= 64); 1: one selection list per analysis */
 } dida_letkf_config;
 
 typedef struct dida_letkf_stats {

@@ -302,3 +302,28 @@ def test_reduced_precision_k128(prec, tol):
     err = increment_error(inst, gpu["xa"], gpu["axa"], ref["xa"], ref["axa"])
     print("k=128 precision", prec, "increment rel RMS err per var", err)
     assert max(err.values()) < tol
+
+
+@pytest.mark.parametrize("pk", [0, 1])
+@pytest.mark.parametrize("prec", [0, 2])
+def test_products_paths_match(pk, prec):
+    """Tile-union products (0) and per-analysis products (1) against the
+    oracle (FP64) or each other (BF16x3, within the mode's rounding)."""
+    import orc
+    from dida_letkf import synth, engine
+    inst = synth.generate("c2", seed=27, nlat=20, nlon=40, nz=3, n_obs=15000, local_dist=700.0)
+    gpu = engine.letkf_compute_nooverlap(inst, precision=prec, keep_transforms=True,
+                                         products_kernel=pk)
+    if prec == 0:
+        ref = orc.letkf_compute(inst, want_trans=True)
+        _assert_parity(gpu, ref, what=f"products_kernel={pk}")
+    else:
+        # the two paths sum the rows in different groupings: FP32-accumulation
+        # rounding apart (~1e-7), each within the mode's bar of the FP64 oracle
+        other = engine.letkf_compute_nooverlap(inst, precision=prec, keep_transforms=True,
+                                               products_kernel=1 - pk)
+        assert np.array_equal(gpu["nobsl"], other["nobsl"])
+        assert rel(gpu["xa"] + gpu["axa"][..., None], other["xa"] + other["axa"][..., None]) < 1e-5
+        ref = orc.letkf_compute(inst)
+        err = increment_error(inst, gpu["xa"], gpu["axa"], ref["xa"], ref["axa"])
+        assert max(err.values()) < 1e-4, err

@@ -111,7 +111,7 @@ int tridiag_variant() {
   return v;
 }
 
-// Analyses per tile of the tile-union products (config products_kernel = 0,
+// Analyses per tile of the tile-union products (config products_kernel = 2,
 // horizontal loop, k <= 64).
 constexpr int kTile = 8;
 
@@ -326,7 +326,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   if (!(c.inflation_factor > 0.0)) return set_error(DIDA_ERR_ARG, "inflation_factor must be > 0");
   if (c.precision < DIDA_PREC_FP64 || c.precision > DIDA_PREC_BF16X3)
     return set_error(DIDA_ERR_ARG, "bad precision %d", c.precision);
-  if (c.products_kernel < 0 || c.products_kernel > 1)
+  if (c.products_kernel < 0 || c.products_kernel > 2)
     return set_error(DIDA_ERR_ARG, "bad products_kernel %d", c.products_kernel);
   if (c.vert_dist > 0.0) {
     if (c.vcoord_mode == DIDA_VCOORD_LNP && (!c.hyai || !c.hybi))
@@ -370,7 +370,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   h->bdeg_lat = 180.0 / h->nb_lat;
   h->bdeg_lon = 360.0 / h->nb_lon;
   h->max_batch = max_batch_columns(c);
-  h->tiled = h->nzc == 1 && h->kp <= 64 && c.products_kernel == 0;
+  h->tiled = h->nzc == 1 && h->kp <= 64 && c.products_kernel == 2;
   if (h->tiled) {
     // tiles never straddle batches, so results do not depend on the batching
     h->max_batch = std::max<long long>(kTile, (h->max_batch / kTile) * kTile);

@@ -92,9 +92,10 @@ typedef struct dida_letkf_config {
     int transform_kernel;   /* 0: split eigensolver (tridiagonalisation,
                                lane-per-matrix QL, rotation apply); 1: the
                                single-kernel transform */
-    int products_kernel;    /* 0: automatic (tiles of 8 analyses sharing their
-                               staged obs rows where the loop is horizontal and
-                               k <= 64); 1: one selection list per analysis */
+    int products_kernel;    /* 0: automatic (= 1); 1: one selection list per
+                               analysis; 2: tiles of 8 analyses sharing their
+                               LDS-staged union of obs rows (horizontal loop,
+                               k <= 64; other loops use 1) */
 } dida_letkf_config;
 
 typedef struct dida_letkf_stats {

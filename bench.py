@@ -97,7 +97,7 @@ def serving_leg(torch, dev, args, precision):
         pools.obs_pool(j.k, serving.GRIDS[j.grid][2])
     torch.cuda.synchronize()
     log(f"serving: pools ready, {torch.cuda.memory_allocated() / 2**30:.1f} GiB allocated")
-    sat = serving.run(torch, sat_jobs, pools, precision=precision, seed=args.seed)
+    sat = serving.run(torch, sat_jobs, pools, precision=precision, seed=args.seed, log=log)
     sat_rate = sat.n_jobs / sat.makespan
     out = dict(saturation=sat.summary(), runs=[])
     log(f"serving saturation: {out['saturation']}")
@@ -107,7 +107,7 @@ def serving_leg(torch, dev, args, precision):
             pools.background(j.grid, j.k)
             pools.obs_pool(j.k, serving.GRIDS[j.grid][2])
         torch.cuda.synchronize()
-        r = serving.run(torch, jobs, pools, precision=precision, seed=args.seed + 3)
+        r = serving.run(torch, jobs, pools, precision=precision, seed=args.seed + 3, log=log)
         r.rate = frac * sat_rate
         summ = r.summary()
         summ["load"] = frac

@@ -208,6 +208,11 @@ def run(torch, jobs: List[JobSpec], pools: Pools, precision: int = PREC_BF16X3,
             res.queue.append(st["t_start"] - st["t_submit"])
             res.analyses += int(st["n_analyses"])
             res.makespan = max(res.makespan, st["t_end"] - t0)
+            if log:
+                sp = rec["spec"]
+                log(f"job {sp.i} done: grid={pools.grids[sp.grid]} k={sp.k} obs={sp.n_obs} "
+                    f"latency={st['t_end'] - st['t_submit']:.2f}s queue={st['t_start'] - st['t_submit']:.2f}s "
+                    f"state={st['state']}")
         return len(done)
 
     with Server(hbm_budget, max_active) as srv:
@@ -236,9 +241,6 @@ def run(torch, jobs: List[JobSpec], pools: Pools, precision: int = PREC_BF16X3,
                              axa.data_ptr())
             outstanding[jid] = dict(keep=keep, bytes=need, spec=j)
             out_bytes += need
-            if log:
-                log(f"submitted job {j.i} grid={grid} k={j.k} obs={j.n_obs} "
-                    f"t={srv.now() - t0:.2f}s outstanding={len(outstanding)}")
         while outstanding:
             if not reap(srv):
                 time.sleep(0.002)

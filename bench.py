@@ -102,7 +102,7 @@ def serving_leg(torch, dev, args, precision):
     out = dict(saturation=sat.summary(), runs=[])
     log(f"serving saturation: {out['saturation']}")
     for frac in [float(x) for x in args.serve_rates.split(",") if x]:
-        jobs = serving.draw_jobs(args.serve_jobs, frac * sat_rate, args.seed + 2)
+        jobs = serving.draw_jobs(args.serve_jobs, frac * sat_rate, args.seed + 1)
         for j in jobs:
             pools.background(j.grid, j.k)
             pools.obs_pool(j.k, serving.GRIDS[j.grid][2])

@@ -264,7 +264,7 @@ struct dida_letkf_handle {
   DevBuf d_hyai, d_hybi;
   DevBuf row_site, nobsl_dev;         // tile-mode selection and row counts
   // work
-  DevBuf cnt, off, scan_tmp, row_idx, row_gc, Abuf, bbuf, status, trans, wout;
+  DevBuf cnt, off, scan_tmp, row_idx, row_gc, row_wd, Abuf, bbuf, status, trans, wout;
   DevBuf x_xb, x_axb, x_xa, x_axa;   // host-entry staging
   EigScratch eig;
   std::vector<int> nobsl_host;
@@ -387,7 +387,7 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->val_out, &h->hist, &h->bin_start, &h->sort_tmp, &h->s_cos, &h->s_sin,
                     &h->s_lonr, &h->s_row0, &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai,
                     &h->d_hybi, &h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc,
-                    &h->row_site, &h->nobsl_dev,
+                    &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
                     &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
@@ -634,6 +634,11 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     sp.cap = 0;
     sp.tile = tiled ? kTile : 0;
     sp.row_site = nullptr;
+    // horizontal loop: weights precomputed by the fill pass
+    sp.weights = h->nzc == 1 ? 1 : 0;
+    sp.rinv32 = h->rinv32.as<float>();
+    sp.dep = h->o_dep.as<double>();
+    sp.row_wd = nullptr;
     // count pass -> offsets -> fill pass; per analysis (column mode) or per
     // tile of kTile analyses (tile mode: the union of their candidate rows)
     const int nlist = tiled ? (ncol + kTile - 1) / kTile : ncol;
@@ -667,6 +672,10 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     } else {
       RC_TRY(h->row_gc.ensure(sizeof(double) * std::max(rows, 1LL)));
       sp.row_gc = h->row_gc.as<double>();
+      if (sp.weights) {
+        RC_TRY(h->row_wd.ensure(sizeof(double) * std::max(rows, 1LL)));
+        sp.row_wd = h->row_wd.as<double>();
+      }
       HIP_TRY(launch_select(st, sp, true));
     }
     HIP_TRY(hipEventRecord(ev[1], st));
@@ -692,6 +701,8 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     pp.axb = axb;
     pp.row_idx = h->row_idx.as<int>();
     pp.row_gc = h->row_gc.as<double>();
+    pp.row_wd = h->row_wd.as<double>();
+    pp.pre_weighted = (!tiled && h->nzc == 1) ? 1 : 0;
     pp.row_off = h->off.as<long long>();
     pp.yb64 = h->o_yb64.as<double>();
     pp.yb32 = h->yb32.as<float>();

@@ -43,6 +43,12 @@ struct SelectParams {
   int* row_idx;
   double* row_gc;
   long long cap;          // fill pass: capacity of row_idx / row_gc
+  // horizontal loop: the fill pass stores w = rinv * rho in row_gc and
+  // w * dep in row_wd, so the products read them with the row index
+  int weights;
+  const float* rinv32;
+  const double* dep;
+  double* row_wd;
   // tile mode (k_select_tile): tiles of `tile` consecutive analyses of
   // [h0, h1); the union of their candidate rows, with the sorted site index
   int tile;
@@ -60,7 +66,9 @@ struct ProdParams {
   double p0, ptop;
   const double* axb;
   const int* row_idx;
-  const double* row_gc;
+  const double* row_gc;      // GC weight, or w = rinv * rho when pre_weighted
+  const double* row_wd;      // w * dep when pre_weighted
+  int pre_weighted;
   const long long* row_off;  // per batch column, n_cols + 1 entries
   const double* yb64;
   const float* yb32;

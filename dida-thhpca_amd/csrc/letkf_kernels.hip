@@ -163,7 +163,14 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
             for (int t = 0; t < rc; ++t) {
               if (pos + t < p.cap) {
                 p.row_idx[pos + t] = r0 + t;
-                p.row_gc[pos + t] = gc;
+                if (p.weights) {
+#pragma clang fp contract(off)
+                  const double w = (double)p.rinv32[r0 + t] * gc;
+                  p.row_gc[pos + t] = w;
+                  p.row_wd[pos + t] = w * p.dep[r0 + t];
+                } else {
+                  p.row_gc[pos + t] = gc;
+                }
               }
             }
           }
@@ -326,8 +333,13 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
     wd = 0.0;
     if (rr < r1) {
       const int row = p.row_idx[rr];
-      w = row_weight(p, row, p.row_gc[rr], vc);
-      wd = w * p.dep[row];
+      if (p.pre_weighted) {
+        w = p.row_gc[rr];
+        wd = p.row_wd[rr];
+      } else {
+        w = row_weight(p, row, p.row_gc[rr], vc);
+        wd = w * p.dep[row];
+      }
       const double* yr = p.yb64 + (long long)row * p.k;
 #pragma unroll
       for (int t = 0; t < T; ++t) {
@@ -424,9 +436,15 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
     wd = 0.0;
     if (rr < r1) {
       const int row = p.row_idx[rr];
-      const double w = row_weight(p, row, p.row_gc[rr], vc);
+      double w;
+      if (p.pre_weighted) {
+        w = p.row_gc[rr];
+        wd = p.row_wd[rr];
+      } else {
+        w = row_weight(p, row, p.row_gc[rr], vc);
+        wd = w * p.dep[row];
+      }
       wf = (float)w;
-      wd = w * p.dep[row];
       const float* yr = p.yb32 + (long long)row * p.k;
 #pragma unroll
       for (int t = 0; t < T; ++t) {
@@ -532,9 +550,15 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
       const float* yr = nullptr;
       if (rr < r1) {
         const int row = p.row_idx[rr];
-        const double w = row_weight(p, row, p.row_gc[rr], vc);
+        double w;
+        if (p.pre_weighted) {
+          w = p.row_gc[rr];
+          wd = p.row_wd[rr];
+        } else {
+          w = row_weight(p, row, p.row_gc[rr], vc);
+          wd = w * p.dep[row];
+        }
         s = sqrtf((float)w);
-        wd = w * p.dep[row];
         yr = p.yb32 + (long long)row * p.k;
       }
 #pragma unroll

@@ -4,8 +4,9 @@ ensemble size and obs count through the continuous-batching server
 
 Each job draws its grid from GRIDS, k from KS and its obs count log-uniform
 in OBS_RANGE, and runs CYCLES analysis cycles with its ensemble resident on
-the device.  Latency is submit -> end of the job's last cycle, on the
-server's clock; it includes queueing for admission.
+the device.  Latency is arrival -> end of the job's last cycle, on the
+server's clock: it includes queueing for admission and any wait of the
+client (whose output arrays for queued jobs must exist at submit).
 
 Inputs are synthetic and device-generated, because a job stream of this size
 cannot be built through the host-side observation operator in the time of a
@@ -76,7 +77,9 @@ class ServeResult:
 
 def draw_jobs(n_jobs: int, rate: Optional[float], seed: int, grids=None, ks=None,
               obs_range=OBS_RANGE) -> List[JobSpec]:
-    """Job shapes and Poisson arrival times (rate None: all arrive at t=0)."""
+    """Job shapes and Poisson arrival times (rate None: all arrive at t=0).
+    The shapes depend on the seed only, so the same seed gives the same jobs
+    at every rate."""
     grids = GRIDS if grids is None else grids
     ks = KS if ks is None else ks
     rng = np.random.Generator(np.random.Philox(np.random.SeedSequence([seed, 77])))
@@ -87,7 +90,8 @@ def draw_jobs(n_jobs: int, rate: Optional[float], seed: int, grids=None, ks=None
     if rate is None:
         t = np.zeros(n_jobs)
     else:
-        t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+        rt = np.random.Generator(np.random.Philox(np.random.SeedSequence([seed, 78])))
+        t = np.cumsum(rt.exponential(1.0 / rate, n_jobs))
         t -= t[0]
     return [JobSpec(i, float(t[i]), int(g[i]), int(ks[kk[i]]), int(nobs[i])) for i in range(n_jobs)]
 
@@ -204,14 +208,15 @@ def run(torch, jobs: List[JobSpec], pools: Pools, precision: int = PREC_BF16X3,
             out_bytes -= rec["bytes"]
             if st["state"] != 2 or st["rc"] < 0:
                 res.failed += 1
-            res.latency.append(st["t_end"] - st["t_submit"])
-            res.queue.append(st["t_start"] - st["t_submit"])
+            arr = t0 + rec["spec"].arrival
+            res.latency.append(st["t_end"] - arr)
+            res.queue.append(st["t_start"] - arr)
             res.analyses += int(st["n_analyses"])
             res.makespan = max(res.makespan, st["t_end"] - t0)
             if log:
                 sp = rec["spec"]
                 log(f"job {sp.i} done: grid={pools.grids[sp.grid]} k={sp.k} obs={sp.n_obs} "
-                    f"latency={st['t_end'] - st['t_submit']:.2f}s queue={st['t_start'] - st['t_submit']:.2f}s "
+                    f"latency={st['t_end'] - arr:.2f}s queue={st['t_start'] - arr:.2f}s "
                     f"state={st['state']}")
         return len(done)
 

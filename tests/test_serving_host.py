@@ -34,3 +34,9 @@ def test_job_config_matches_workload():
     cfg = serving.job_config(serving.GRIDS[1], 64, 2)
     assert (cfg.nlat, cfg.nlon, cfg.nz, cfg.k, cfg.n) == (180, 360, 32, 64, 97)
     assert cfg.local_dist == serving.LOCAL_DIST_KM and cfg.vert_dist < 0
+
+
+def test_same_seed_same_shapes_at_every_rate():
+    a = serving.draw_jobs(50, None, seed=3)
+    b = serving.draw_jobs(50, 0.7, seed=3)
+    assert [(j.grid, j.k, j.n_obs) for j in a] == [(j.grid, j.k, j.n_obs) for j in b]

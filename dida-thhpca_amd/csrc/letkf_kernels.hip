@@ -121,8 +121,18 @@ __device__ __forceinline__ BinRange bin_range(const SelectParams& p, double clat
   return R;
 }
 
+// orders a wavefront's LDS writes before its reads of other lanes' entries
+// (the wavefronts of k_select's workgroup never share LDS entries)
+__device__ __forceinline__ void __syncwarp_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_select(SelectParams p) {
+  __shared__ int s_pre[4][64], s_r0[4][64];
+  __shared__ double s_gc[4][64];
   const int lane = threadIdx.x & 63;
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int ah = p.h0 + wid;
@@ -155,25 +165,40 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
         if (!FILL) {
           count += rc;
         } else {
+          // the chunk's rows are written by all 64 lanes: entry e belongs to
+          // the lane whose [pre, pre + rc) holds it (binary search over the
+          // wavefront's prefix counts in LDS), so a station column's rows do
+          // not serialise on one lane
           const int pre = wave_excl_scan(rc, lane);
           const int tot = __shfl(pre + rc, 63, 64);
-          if (rc) {
-            const long long pos = base + pre;
-            const int r0 = p.s_row0[s];
-            for (int t = 0; t < rc; ++t) {
-              if (pos + t < p.cap) {
-                p.row_idx[pos + t] = r0 + t;
-                if (p.weights) {
+          const int wv = (threadIdx.x >> 6) & 3;
+          s_pre[wv][lane] = pre;
+          s_r0[wv][lane] = rc ? p.s_row0[s] : 0;
+          s_gc[wv][lane] = gc;
+          __syncwarp_lds();
+          for (int e = lane; e < tot; e += 64) {
+            int lo = 0, hi = 63;             // last lane with pre <= e
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (s_pre[wv][mid] <= e) lo = mid;
+              else hi = mid - 1;
+            }
+            const long long pos = base + e;
+            const int row = s_r0[wv][lo] + (e - s_pre[wv][lo]);
+            const double g = s_gc[wv][lo];
+            if (pos < p.cap) {
+              p.row_idx[pos] = row;
+              if (p.weights) {
 #pragma clang fp contract(off)
-                  const double w = (double)p.rinv32[r0 + t] * gc;
-                  p.row_gc[pos + t] = w;
-                  p.row_wd[pos + t] = w * p.dep[r0 + t];
-                } else {
-                  p.row_gc[pos + t] = gc;
-                }
+                const double w = (double)p.rinv32[row] * g;
+                p.row_gc[pos] = w;
+                p.row_wd[pos] = w * p.dep[row];
+              } else {
+                p.row_gc[pos] = g;
               }
             }
           }
+          __syncwarp_lds();
           base += tot;
         }
       }

@@ -934,6 +934,381 @@ void k_update(EigParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// k_vec128 + k_update128: k = 65..128 with the rows of Z in registers.
+// 256 threads per matrix: thread t keeps half h = t / 128 (columns
+// 64h .. 64h+63) of row r = t % 128.  A chase that crosses the middle (lo < 64
+// <= hi) runs on the upper half first; its carry goes to the lower half
+// through LDS, and the lower half's step at position 63 returns the new
+// column-64 entry the same way.
+// ---------------------------------------------------------------------------
+
+// Applies the positions il in [lo_l, hi_l) of a chase (dense table `tab`,
+// identity outside) to a half row z[0..63]; z[64] is `ztop` (in/out).  The
+// carry enters at the top block as z[top+1] and leaves as the return value;
+// with `final_write`, z[lo_l] takes it (the chase ends in this half).
+__device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int lo_l, int hi_l,
+                                             const double2* __restrict__ tab, bool final_write) {
+  const int bhi = (hi_l - 1) >> 3, blo = lo_l >> 3;
+  double carry = 0.0;
+#pragma unroll
+  for (int B = 7; B >= 0; --B) {
+    if (B > bhi || B < blo) continue;
+    if (B == bhi) carry = (B == 7) ? ztop : z[8 * B + 8];
+    double2 cs[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) cs[t] = tab[8 * B + t];
+#pragma unroll
+    for (int t = 7; t >= 0; --t) {
+      const int i = 8 * B + t;
+      const double zi = z[i];
+      const double up = cs[t].y * zi + cs[t].x * carry;
+      if (i == 63) ztop = up;
+      else z[i + 1] = up;
+      carry = cs[t].x * zi - cs[t].y * carry;
+    }
+    if (B == blo && final_write) z[8 * B] = carry;
+  }
+  return carry;
+}
+
+__global__ __launch_bounds__(256) void k_vec128(EigParams p) {
+  constexpr int KMAX = 128, LDV = KMAX + 1, NT = 256;
+  constexpr int RING = 1024, REFILL = 512;
+  __shared__ __align__(16) double sbuf[16 * LDV > 2 * RING + 2 * KMAX ? 16 * LDV : 2 * RING + 2 * KMAX];
+  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
+  __shared__ double sx[2][2][KMAX];      // partial dots [parity][half][row]
+  __shared__ double sc[KMAX], sz64[KMAX];
+  __shared__ double red[8];
+  const int tid = threadIdx.x;
+  const int r = tid & 127, h = tid >> 7;
+  const int lane = tid & 63, wv = tid >> 6;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  if (p.status[a] != 0) return;
+  const int k = p.k, kp = p.kp;
+  const bool in_r = r < k;
+  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + al * kp;
+  const int nref = k > 2 ? k - 2 : 0;
+
+  // ---- A. Z <- Z H_j: half dots exchanged through LDS ----
+  if (tid < nref) stau[tid] = tg[tid];
+  double z[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) z[c] = (64 * h + c == r) ? 1.0 : 0.0;
+  for (int j0 = 0; j0 < nref; j0 += 16) {
+    __syncthreads();
+    for (int e = tid; e < 16 * KMAX; e += NT) {
+      const int t = e / KMAX, c = e % KMAX;
+      sbuf[t * LDV + c] = (j0 + t < nref) ? refl[(long long)(j0 + t) * kp + c] : 0.0;
+    }
+    __syncthreads();
+    const int j1 = min(j0 + 16, nref);
+    for (int j = j0; j < j1; ++j) {
+      const double tau = uniform(stau[j]);
+      const double* q = sbuf + (j - j0) * LDV + 64 * h;
+      double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 64; c += 2) {
+        d0 = fma(z[c], q[c], d0);
+        d1 = fma(z[c + 1], q[c + 1], d1);
+      }
+      sx[j & 1][h][r] = d0 + d1;
+      __syncthreads();
+      if (tau == 0.0) continue;
+      const double dot = (sx[j & 1][0][r] + sx[j & 1][1][r]) * tau;
+#pragma unroll
+      for (int c = 0; c < 64; ++c) z[c] = fma(-dot, q[c], z[c]);
+    }
+  }
+  __syncthreads();
+
+  // ---- B. logged QL rotations ----
+  {
+    double2* sring = reinterpret_cast<double2*>(sbuf);
+    double2* stab = reinterpret_cast<double2*>(sbuf + 2 * RING);
+    const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
+    const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
+    const int nch = uniform(min(p.nchase[al], p.hdr_cap));
+    int nrot = 0;
+    if (nch > 0) {
+      const int4 hl = hdr[nch - 1];
+      nrot = uniform(hl.z + (hl.y - hl.x));
+    }
+    int committed = 0;
+    double ztop = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+      int4 hh = hdr[ch];
+      const int lo = uniform(hh.x), hi = uniform(hh.y), off = uniform(hh.z);
+      if (off + (hi - lo) > committed) {
+        __syncthreads();
+        for (int e = tid; e < REFILL; e += NT) {
+          const int g = committed + e;
+          sring[g & (RING - 1)] = (g < nrot) ? rot[g] : make_double2(1.0, 0.0);
+        }
+        committed += REFILL;
+      }
+      __syncthreads();
+      const int base = off + hi - 1;
+      if (tid < KMAX) {
+        const bool in = tid >= lo && tid < hi;
+        const double2 g = sring[(base - tid) & (RING - 1)];
+        stab[tid] = in ? g : make_double2(1.0, 0.0);
+      }
+      __syncthreads();
+      const bool cross = lo < 64 && hi >= 64;
+      if (!cross) {
+        // the whole chase is in one half
+        if ((hi <= 63 && h == 0) || (lo >= 64 && h == 1)) {
+          ztop = 0.0;
+          (void)chase_half(z, ztop, lo - 64 * h, hi - 64 * h, stab + 64 * h, true);
+        }
+      } else {
+        if (h == 1) {
+          // hi == 64: no position in this half, the carry is column 64 itself
+          ztop = 0.0;
+          const double cy = (hi > 64) ? chase_half(z, ztop, 0, hi - 64, stab + 64, false) : z[0];
+          sc[r] = cy;
+        }
+        __syncthreads();
+        if (h == 0) {
+          ztop = sc[r];
+          (void)chase_half(z, ztop, lo, 64, stab, true);
+          sz64[r] = ztop;
+        }
+        __syncthreads();
+        if (h == 1) z[0] = sz64[r];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- C. eigenvalue functions, T = V diag(f) V^T on MFMA, overline_w ----
+  if (tid < KMAX) {
+    double lam = tid < k ? p.lam[al * kp + tid] : 0.0;
+    sf[tid] = lam;
+  }
+  __syncthreads();
+  if (p.guard) {
+    double mn = 1e300;
+    for (int i = 0; i < k; ++i) mn = fmin(mn, sf[i]);
+    __syncthreads();
+    if (tid < k && mn < 0.0) sf[tid] -= mn;
+    __syncthreads();
+  }
+  if (tid < KMAX) {
+    const bool ok = tid < k;
+    const double lam = ok ? sf[tid] * 1.0 + (double)(k - 1) / p.inflation : 1.0;
+    sg[tid] = ok ? 1.0 / lam : 0.0;
+    sb[tid] = ok ? p.b[al * kp + tid] : 0.0;
+    sf[tid] = ok ? sqrt((double)(k - 1) / lam) : 0.0;
+  }
+  // st = diag(1/lambda) V^T b: column sums over rows, V staged 16 columns at
+  // a time ([column][row]); T tiles: wave w computes J in {2w, 2w+1}, in two
+  // passes over I (4 tile rows each)
+  const int i16 = lane & 15, q = lane >> 4;
+  double* sVc = sbuf;
+  double wpart = 0.0;
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    d4 acc[4][2];
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+      for (int J = 0; J < 2; ++J) acc[I][J] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {                // chunk c: columns 16c .. 16c+15
+      __syncthreads();
+      if (h == (c >> 2)) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) sVc[t * LDV + r] = z[16 * (c & 3) + t];
+      }
+      __syncthreads();
+      if (pass == 0 && tid < 16) {
+        double tb = 0.0;
+        for (int rr = 0; rr < KMAX; ++rr) tb = fma(sVc[tid * LDV + rr], sb[rr], tb);
+        st[16 * c + tid] = tb * sg[16 * c + tid];
+      }
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int t = 4 * s4 + q;
+        const double fc = sf[16 * c + t];
+        double fa[4], fb[2];
+#pragma unroll
+        for (int I = 0; I < 4; ++I) fa[I] = sVc[t * LDV + 16 * (4 * pass + I) + i16];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) fb[J] = fc * sVc[t * LDV + 16 * (2 * wv + J) + i16];
+#pragma unroll
+        for (int I = 0; I < 4; ++I)
+#pragma unroll
+          for (int J = 0; J < 2; ++J)
+            acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[I], fb[J], acc[I][J], 0, 0, 0);
+      }
+    }
+    if (pass == 0) {
+      __syncthreads();
+      // overline_w(r) = sum_c V[r][c] st_c, halves summed through LDS
+#pragma unroll
+      for (int c = 0; c < 64; ++c) wpart = fma(z[c], st[64 * h + c], wpart);
+      sx[0][h][r] = wpart;
+      __syncthreads();
+      if (tid < KMAX) sw[tid] = tid < k ? sx[0][0][tid] + sx[0][1][tid] : 0.0;
+      __syncthreads();
+    }
+    // trans(f, e) = T(f, e) + overline_w(f)
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const double wf = sw[16 * (4 * pass + I) + q + 4 * rr];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) acc[I][J][rr] += wf;
+      }
+    if (p.trans_out) {
+      double* tr = p.trans_out + a * (long long)k * k;
+#pragma unroll
+      for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = 0; J < 2; ++J)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int f = 16 * (4 * pass + I) + q + 4 * rr, e = 16 * (2 * wv + J) + i16;
+            if (f < k && e < k) tr[(long long)e * k + f] = acc[I][J][rr];
+          }
+    }
+    if (p.xb) {
+      __syncthreads();                             // every read of the reflector block is done
+#pragma unroll
+      for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = 0; J < 2; ++J)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr)
+            refl[tile_off<KMAX>(4 * pass + I, 2 * wv + J, rr, lane)] = acc[I][J][rr];
+    }
+  }
+  if (p.w_out && h == 0 && in_r) p.w_out[a * (long long)k + r] = sw[r];
+}
+
+// Xa = Xb trans for k = 65..128: wave w produces members 32w .. 32w+31 from the
+// trans tiles J in {2w, 2w+1}; 16 state vectors per MFMA group.
+__global__ __launch_bounds__(256) void k_update128(EigParams p) {
+  constexpr int KMAX = 128, LDV = KMAX + 1, NT = 256;
+  __shared__ double sV[16 * LDV];
+  __shared__ double spart[4][16];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  if (p.status[a] != 0) return;
+  const int k = p.k, kp = p.kp;
+  const int i16 = lane & 15, q = lane >> 4;
+  const double* __restrict__ tb = p.refl + al * (long long)kp * kp;
+  d4 bop[8][2];
+#pragma unroll
+  for (int I = 0; I < 8; ++I)
+#pragma unroll
+    for (int J = 0; J < 2; ++J)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) bop[I][J][rr] = tb[tile_off<KMAX>(I, 2 * wv + J, rr, lane)];
+
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
+  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  const long long lat_ext = p.g.nlat + 2 * p.halo;
+  int sk = 1, ek = p.nz;
+  if (p.vertical) {
+    sk = kc * p.cube_nz + 1;
+    ek = min(sk + p.cube_nz - 1, p.nz);
+  }
+  int rs[kMaxVars], rl[kMaxVars];
+  int nrun = 0, nvec = 0;
+  if (!p.vertical) {
+    rs[0] = 0; rl[0] = p.n; nrun = 1;
+  } else {
+    for (int vv = 0; vv < p.n_var; ++vv) {
+      const int start = p.var_start[vv] - 1;
+      if (p.var_lev[vv] == 1) {
+        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
+      } else {
+        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
+      }
+    }
+  }
+  for (int t = 0; t < nrun; ++t) nvec += rl[t];
+  auto sidx = [&](int v) {
+    int rem = v;
+    for (int t = 0; t < nrun; ++t) {
+      if (rem < rl[t]) return rs[t] + rem;
+      rem -= rl[t];
+    }
+    return -1;
+  };
+  for (int lo = si; lo <= ei; ++lo) {
+    for (int la = sj; la <= ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int v0 = 0; v0 < nvec; v0 += 16) {
+        __syncthreads();
+        for (int e = tid; e < 16 * KMAX; e += NT) {
+          const int u = e / KMAX, m = e % KMAX;
+          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
+          sV[u * LDV + m] = (s_i >= 0 && m < k) ? p.xb[(off + s_i) * k + m] : 0.0;
+        }
+        __syncthreads();
+        d4 out[2];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int sK = 0; sK < KMAX / 4; ++sK) {
+          const double xa_frag = sV[i16 * LDV + 4 * sK + q];
+#pragma unroll
+          for (int J = 0; J < 2; ++J)
+            out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa_frag, bop[sK >> 2][J][sK & 3],
+                                                          out[J], 0, 0, 0);
+        }
+        // lane holds Xa[vector q + 4rr][member 32w + 16J + i16]
+        double ab4[4], xe[4][2];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int v = v0 + q + 4 * rr;
+          const int s_i = (v < nvec) ? sidx(v) : -1;
+          ab4[rr] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
+          double sum = 0.0;
+#pragma unroll
+          for (int J = 0; J < 2; ++J) {
+            const int e = 32 * wv + 16 * J + i16;
+            xe[rr][J] = (e < k) ? out[J][rr] + ab4[rr] : 0.0;
+            sum += xe[rr][J];
+          }
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
+          if (i16 == 0) spart[wv][q + 4 * rr] = sum;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int v = v0 + q + 4 * rr;
+          const int s_i = (v < nvec) ? sidx(v) : -1;
+          if (s_i < 0) continue;
+          const int vi = q + 4 * rr;
+          const double mean = ((spart[0][vi] + spart[1][vi]) + (spart[2][vi] + spart[3][vi])) / k;
+          const long long o = off + s_i;
+#pragma unroll
+          for (int J = 0; J < 2; ++J) {
+            const int e = 32 * wv + 16 * J + i16;
+            if (e < k) p.xa[o * k + e] = xe[rr][J] - mean;
+          }
+          if (wv == 0 && i16 == 0) p.axa[o] = mean;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k = 65..128: the matrix does not fit a wavefront's registers, so the two
 // per-matrix phases keep it in LDS (one 128-thread workgroup per matrix,
 // thread r owns row r); the QL recurrence is k_ql_log<128>.
@@ -1294,8 +1669,14 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);
-    if ((e = mark(3)) != hipSuccess) return e;
+    if (p.apply_fused) {
+      hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+      if ((e = mark(3)) != hipSuccess) return e;
+      if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+    }
   } else {
     return hipErrorInvalidValue;
   }

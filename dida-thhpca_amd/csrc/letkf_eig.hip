@@ -564,6 +564,36 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   }
 }
 
+// Sum over the NW wavefronts of a workgroup; every thread gets the same value
+// (wave sums, then the partials in a fixed order).  `red` holds two slots of
+// NW partials so that back-to-back calls need one barrier each.
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double* red, int& slot) {
+  const double w = wave_sum(v);
+  if constexpr (NW == 1) {
+    return uniform(w);
+  } else {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[NW * slot + wv] = w;
+    __syncthreads();
+    double r = red[NW * slot];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r += red[NW * slot + i];
+    slot ^= 1;
+    return uniform(r);
+  }
+}
+
+__device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
+  const double w = wave_min(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[2 * slot + wv] = w;
+  __syncthreads();
+  const double r = fmin(red[2 * slot], red[2 * slot + 1]);
+  slot ^= 1;
+  return uniform(r);
+}
+
 // ---------------------------------------------------------------------------
 // k_vec + k_update: the k_apply work in two kernels sized for two wavefronts
 // per SIMD.  k_vec forms V (Q accumulation, logged rotations), overline_w and
@@ -942,6 +972,108 @@ void k_update(EigParams p) {
 // column-64 entry the same way.
 // ---------------------------------------------------------------------------
 
+// Householder tridiagonalisation for k = 65..128 with half rows of A in
+// registers (thread t: row t % 128, columns 64h .. 64h+63, h = t / 128);
+// same outputs as k_tridiag.  The halves' partial products are summed
+// through LDS in a fixed order; column j+1 after step j's update goes
+// through LDS under a uniform branch.
+__global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
+  constexpr int KMAX = 128;
+  __shared__ double sv[KMAX], sw[KMAX], sxn[KMAX], spr[2][KMAX];
+  __shared__ double se_s[KMAX + 1], st_s[KMAX + 1], sd[KMAX];
+  __shared__ double red[8];
+  const int tid = threadIdx.x;
+  const int r = tid & 127, h = uniform(tid >> 7);   // the half is one per wavefront
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  const int k = p.k, kp = p.kp;
+  const double* __restrict__ Ag = p.A + al * (long long)kp * kp;
+  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
+  const bool in_r = r < k;
+  double arow[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) {
+    const int col = 64 * h + c;
+    arow[c] = (in_r && col < k) ? Ag[(long long)col * kp + r] : 0.0;
+  }
+  if (h == 0) sxn[r] = arow[0];
+  int slot = 0;
+  __syncthreads();
+  for (int j = 0; j + 2 < k; ++j) {
+    const bool act = in_r && r > j;
+    const double x = act ? sxn[r] : 0.0;
+    const double x0 = uniform(sxn[j + 1]);
+    const double rest = block_sum<4>((h == 0 && act && r > j + 1) ? x * x : 0.0, red, slot);
+    if (rest == 0.0) {
+      // column already in tridiagonal form; pass column j+1 on
+#pragma unroll
+      for (int c = 0; c < 64; ++c)
+        if (64 * h + c == j + 1) sxn[r] = arow[c];
+      if (h == 0) refl[(long long)j * kp + r] = 0.0;
+      if (tid == 0) { se_s[j] = x0; st_s[j] = 0.0; }
+      __syncthreads();
+      continue;
+    }
+    const double nrm = sqrt(rest + x0 * x0);
+    const double alpha = (x0 >= 0.0) ? -nrm : nrm;
+    const double v0 = x0 - alpha;
+    const double tau = 2.0 / (rest + v0 * v0);
+    const double v = act ? ((r == j + 1) ? v0 : x) : 0.0;
+    if (h == 0) sv[r] = v;
+    __syncthreads();
+    double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+    for (int B = 0; B < 8; ++B) {
+#pragma unroll
+      for (int c = 8 * B; c < 8 * B + 8; c += 2) {
+        p0 = fma(arow[c], sv[64 * h + c], p0);
+        p1 = fma(arow[c + 1], sv[64 * h + c + 1], p1);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    spr[h][r] = p0 + p1;
+    __syncthreads();
+    const double pr = act ? (spr[0][r] + spr[1][r]) * tau : 0.0;
+    const double K = 0.5 * tau * block_sum<4>(h == 0 ? v * pr : 0.0, red, slot);
+    const double wr = act ? pr - K * v : 0.0;
+    if (h == 0) sw[r] = wr;
+    __syncthreads();
+#pragma unroll
+    for (int B = 0; B < 8; ++B) {
+#pragma unroll
+      for (int c = 8 * B; c < 8 * B + 8; ++c) {
+        if (act) arow[c] -= v * sw[64 * h + c] + wr * sv[64 * h + c];
+        if (64 * h + c == j + 1) sxn[r] = arow[c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (h == 0) refl[(long long)j * kp + r] = v;
+    if (tid == 0) { se_s[j] = alpha; st_s[j] = tau; }
+    __syncthreads();
+  }
+  // diagonal (thread of the half holding column r) and e[k-2] = A[k-1][k-2]
+#pragma unroll
+  for (int c = 0; c < 64; ++c)
+    if (64 * h + c == r) sd[r] = arow[c];
+  if (r == k - 1) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (64 * h + c == k - 2) se_s[KMAX] = arow[c];
+  }
+  __syncthreads();
+  if (h != 0) return;
+  double* __restrict__ dg = p.d + al * kp;
+  double* __restrict__ eg = p.e + al * kp;
+  double* __restrict__ tg = p.tau + al * kp;
+  dg[r] = in_r ? sd[r] : 0.0;
+  double ev = 0.0, tv = 0.0;
+  if (r + 2 < k) { ev = se_s[r]; tv = st_s[r]; }
+  else if (r == k - 2) ev = se_s[KMAX];
+  eg[r] = ev;
+  tg[r] = tv;
+}
+
 // Applies the positions il in [lo_l, hi_l) of a chase (dense table `tab`,
 // identity outside) to a half row z[0..63]; z[64] is `ztop` (in/out).  The
 // carry enters at the top block as z[top+1] and leaves as the return value;
@@ -980,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_vec128(EigParams p) {
   __shared__ double sc[KMAX], sz64[KMAX];
   __shared__ double red[8];
   const int tid = threadIdx.x;
-  const int r = tid & 127, h = tid >> 7;
-  const int lane = tid & 63, wv = tid >> 6;
+  const int r = tid & 127, h = uniform(tid >> 7);   // the half is one per wavefront
+  const int lane = tid & 63, wv = uniform(tid >> 6);
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
   const long long al = a - p.a0;
@@ -1314,36 +1446,6 @@ __global__ __launch_bounds__(256) void k_update128(EigParams p) {
 // thread r owns row r); the QL recurrence is k_ql_log<128>.
 // ---------------------------------------------------------------------------
 
-// Sum over the NW wavefronts of a workgroup; every thread gets the same value
-// (wave sums, then the partials in a fixed order).  `red` holds two slots of
-// NW partials so that back-to-back calls need one barrier each.
-template <int NW>
-__device__ __forceinline__ double block_sum(double v, double* red, int& slot) {
-  const double w = wave_sum(v);
-  if constexpr (NW == 1) {
-    return uniform(w);
-  } else {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) red[NW * slot + wv] = w;
-    __syncthreads();
-    double r = red[NW * slot];
-#pragma unroll
-    for (int i = 1; i < NW; ++i) r += red[NW * slot + i];
-    slot ^= 1;
-    return uniform(r);
-  }
-}
-
-__device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
-  const double w = wave_min(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[2 * slot + wv] = w;
-  __syncthreads();
-  const double r = fmin(red[2 * slot], red[2 * slot + 1]);
-  slot ^= 1;
-  return uniform(r);
-}
-
 // Householder tridiagonalisation with A in LDS, one thread per row (one
 // workgroup of max(KMAX, 64) threads per matrix); same outputs as k_tridiag.
 template <int KMAX>
@@ -1665,7 +1767,8 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
       if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(64), 0, st, p);
     }
   } else if (p.k <= 128) {
-    hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
+    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
+    else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;

@@ -139,11 +139,18 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
 // ---------------------------------------------------------------------------
 // k_ql_log: one lane per matrix
 // ---------------------------------------------------------------------------
-template <int KMAX>
+// GLOBAL: d and e live in the lane-interleaved scratch p.dq / p.eq (64
+// matrices per group, [i][lane]) instead of LDS, so the kernel's occupancy is
+// bound by registers only and a whole cycle's wavefronts are resident.
+template <int KMAX, bool GLOBAL>
 __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
-  __shared__ double sd[KMAX][64];
-  __shared__ double se[KMAX][64];
+  __shared__ double sd_l[GLOBAL ? 1 : KMAX][64];
+  __shared__ double se_l[GLOBAL ? 1 : KMAX][64];
   const int lane = threadIdx.x;
+  double* const dq = p.dq + (long long)blockIdx.x * KMAX * 64;
+  double* const eq = p.eq + (long long)blockIdx.x * KMAX * 64;
+#define SD(i) (*(GLOBAL ? &dq[(i) * 64 + lane] : &sd_l[GLOBAL ? 0 : (i)][lane]))
+#define SE(i) (*(GLOBAL ? &eq[(i) * 64 + lane] : &se_l[GLOBAL ? 0 : (i)][lane]))
   const long long a = p.a0 + (long long)blockIdx.x * 64 + lane;
   const bool valid = a < p.a0 + p.n_an;
   const long long al = a - p.a0;
@@ -155,8 +162,8 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       dv = p.d[al * kp + i];
       ev = p.e[al * kp + i];
     }
-    sd[i][lane] = dv;
-    se[i][lane] = ev;
+    SD(i) = dv;
+    SE(i) = ev;
     tnorm = fmax(tnorm, fabs(dv) + fabs(ev));
   }
   if (!valid) return;
@@ -169,47 +176,47 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
     for (;;) {
       int m = l;
       for (; m < k - 1; ++m) {
-        const double dd = fabs(sd[m][lane]) + fabs(sd[m + 1][lane]);
-        const double em = fabs(se[m][lane]);
+        const double dd = fabs(SD(m)) + fabs(SD(m + 1));
+        const double em = fabs(SE(m));
         if (em <= eps * dd || em <= eps * tnorm) break;
       }
       if (m == l) break;
       if (++iter > 30) { status = l + 1; break; }
       if (nhdr >= p.hdr_cap || nrot + (m - l) > p.rot_cap) { status = -1; break; }
-      double g = (sd[l + 1][lane] - sd[l][lane]) / (2.0 * se[l][lane]);
+      double g = (SD(l + 1) - SD(l)) / (2.0 * SE(l));
       double r = hypot(g, 1.0);
-      g = sd[m][lane] - sd[l][lane] + se[l][lane] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+      g = SD(m) - SD(l) + SE(l) / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
       double s = 1.0, c = 1.0, pp = 0.0;
       const int off = nrot;
       int i = m - 1;
       bool underflow = false;
       // d[i], e[i] of the next step are read before this step's stores
-      double ei = se[i][lane], di = sd[i][lane], di1 = sd[i + 1][lane];
+      double ei = SE(i), di = SD(i), di1 = SD(i + 1);
       for (; i >= l; --i) {
         double ein = 0.0, din = 0.0;
         if (i > l) {
-          ein = se[i - 1][lane];
-          din = sd[i - 1][lane];
+          ein = SE(i - 1);
+          din = SD(i - 1);
         }
         const double f = s * ei, bb = c * ei;
         // one reciprocal square root instead of sqrt and a divide on the
         // serial chain: r = sqrt(f^2 + g^2), s = f / r, c = g / r
         const double r2 = fma(f, f, g * g);
         if (r2 == 0.0) {
-          se[i + 1][lane] = 0.0;
-          sd[i + 1][lane] = di1 - pp;
-          se[m][lane] = 0.0;
+          SE(i + 1) = 0.0;
+          SD(i + 1) = di1 - pp;
+          SE(m) = 0.0;
           underflow = true;
           break;
         }
         const double rinv = rsqrt(r2);
-        se[i + 1][lane] = r2 * rinv;
+        SE(i + 1) = r2 * rinv;
         s = f * rinv;
         c = g * rinv;
         g = di1 - pp;
         r = (di - g) * s + 2.0 * c * bb;
         pp = s * r;
-        sd[i + 1][lane] = g + pp;
+        SD(i + 1) = g + pp;
         g = c * r - bb;
         rot[nrot++] = make_double2(c, s);
         ei = ein;
@@ -219,15 +226,17 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       const int lo = underflow ? i + 1 : l;
       if (m - lo > 0) hdr[nhdr++] = make_int4(lo, m, off, 0);
       if (underflow) continue;
-      sd[l][lane] -= pp;
-      se[l][lane] = g;
-      se[m][lane] = 0.0;
+      SD(l) -= pp;
+      SE(l) = g;
+      SE(m) = 0.0;
     }
   }
-  for (int i = 0; i < k; ++i) p.lam[al * kp + i] = sd[i][lane];
+  for (int i = 0; i < k; ++i) p.lam[al * kp + i] = SD(i);
   p.nchase[al] = nhdr;
   p.status[a] = status;
 }
+#undef SD
+#undef SE
 
 // ---------------------------------------------------------------------------
 // k_apply
@@ -1742,7 +1751,8 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<32>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<32>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
+    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<32, true>), dim3(nq), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((k_ql_log<32, false>), dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
@@ -1756,7 +1766,8 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
+    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<64, true>), dim3(nq), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((k_ql_log<64, false>), dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
@@ -1770,7 +1781,8 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
+    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<128, true>), dim3(nq), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((k_ql_log<128, false>), dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);

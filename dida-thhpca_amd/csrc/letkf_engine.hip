@@ -96,7 +96,7 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 
 // Scratch of the split eigensolver for one batch of analyses.
 struct EigScratch {
-  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
+  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list, dq, eq;
   std::vector<int> st_host;
 };
 
@@ -114,6 +114,16 @@ int tridiag_variant() {
 // Analyses per tile of the tile-union products (config products_kernel = 2,
 // horizontal loop, k <= 64).
 constexpr int kTile = 8;
+
+// QL storage: DIDA_QL=global keeps d/e in a lane-interleaved global scratch
+// (occupancy by registers), anything else in LDS.
+int ql_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("DIDA_QL");
+    return (e && std::strcmp(e, "global") == 0) ? 1 : 0;
+  }();
+  return v;
+}
 
 // Eigenvector/update variant for k <= 64: DIDA_APPLY=fused selects the
 // single k_apply kernel, anything else k_vec + k_update.
@@ -177,6 +187,14 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.hdr_cap = hdr_cap;
   ep.tridiag_lds = tridiag_variant();
   ep.apply_fused = apply_variant();
+  ep.ql_global = ql_variant();
+  if (ep.ql_global) {
+    const size_t ng = (size_t)((n + 63) / 64) * 64 * kp;
+    RC_TRY(s.dq.ensure(sizeof(double) * ng));
+    RC_TRY(s.eq.ensure(sizeof(double) * ng));
+    ep.dq = s.dq.as<double>();
+    ep.eq = s.eq.as<double>();
+  }
   ep.nchase = s.nchase.as<int>();
   ep.status = xp.status;
   ep.nzc = xp.nzc;
@@ -390,7 +408,8 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
-                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
+                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list,
+                    &h->eig.dq, &h->eig.eq};
   for (DevBuf* b : bufs) b->release();
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -977,7 +996,7 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   EigScratch es;
   int rc_t = run_transform(st, es, xp, 0);
   DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
-                     &es.list};
+                     &es.list, &es.dq, &es.eq};
   if (rc_t < 0) {
     for (DevBuf* b : ebufs) b->release();
     return rc_t;

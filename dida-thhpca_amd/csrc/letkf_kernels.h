@@ -132,6 +132,9 @@ struct EigParams {
   int rot_cap, hdr_cap;
   int tridiag_lds;          // k <= 64: tridiagonalise in LDS (1) or in VGPRs (0)
   int apply_fused;          // k <= 64: k_apply (1) or k_vec + k_update (0)
+  int ql_global;            // QL with d/e in the lane-interleaved scratch (1) or in LDS (0)
+  double* dq;               // ql_global: ceil(n/64) groups of kp x 64 (d)
+  double* eq;               // ql_global: same layout (e)
   int* nchase;              // batch-local
   int* status;              // global index
   int nzc, vertical, cube_nz, nz, halo, n, n_var;

@@ -139,18 +139,14 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
 // ---------------------------------------------------------------------------
 // k_ql_log: one lane per matrix
 // ---------------------------------------------------------------------------
-// GLOBAL: d and e live in the lane-interleaved scratch p.dq / p.eq (64
-// matrices per group, [i][lane]) instead of LDS, so the kernel's occupancy is
-// bound by registers only and a whole cycle's wavefronts are resident.
-template <int KMAX, bool GLOBAL>
+// d and e of the 64 matrices live in LDS as [i][lane].
+template <int KMAX>
 __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
-  __shared__ double sd_l[GLOBAL ? 1 : KMAX][64];
-  __shared__ double se_l[GLOBAL ? 1 : KMAX][64];
+  __shared__ double sd_l[KMAX][64];
+  __shared__ double se_l[KMAX][64];
   const int lane = threadIdx.x;
-  double* const dq = p.dq + (long long)blockIdx.x * KMAX * 64;
-  double* const eq = p.eq + (long long)blockIdx.x * KMAX * 64;
-#define SD(i) (*(GLOBAL ? &dq[(i) * 64 + lane] : &sd_l[GLOBAL ? 0 : (i)][lane]))
-#define SE(i) (*(GLOBAL ? &eq[(i) * 64 + lane] : &se_l[GLOBAL ? 0 : (i)][lane]))
+#define SD(i) sd_l[i][lane]
+#define SE(i) se_l[i][lane]
   const long long a = p.a0 + (long long)blockIdx.x * 64 + lane;
   const bool valid = a < p.a0 + p.n_an;
   const long long al = a - p.a0;
@@ -857,7 +853,7 @@ void k_vec(EigParams p) {
 
 template <int KMAX>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-void k_update(EigParams p) {
+void k_update_reg(EigParams p) {
   constexpr int TT = KMAX / 16;
   constexpr int LDV = KMAX + 1;
   __shared__ double sV[16 * LDV];
@@ -967,6 +963,160 @@ void k_update(EigParams p) {
             if (i16 == 0) p.axa[o] = mean;
           }
         }
+      }
+    }
+  }
+}
+
+// k_update: xa(:, v) = trans^T xb(:, v) + axb(v), then re-centred, for the
+// variables v of one analysis (letkf_core.F90 ensemble update; the re-centring
+// is the mean/perturbation split of letkf_run).  One block of four wavefronts
+// per analysis: trans (k x k, from the B-operand tiles k_vec leaves in the
+// reflector block) is staged once in LDS and wavefront w takes the
+// (column, 16-variable group) items w, w + 4, ...  The MFMA reduction index
+// is permuted -- at step s, lane quarter q carries member S q + s (S = KMAX/4)
+// -- so that a lane's A operands for a group are one contiguous run of S
+// members of one variable (16-byte loads); B takes the same permutation from
+// LDS.  Registers stay near 100, so four blocks share a CU and the loads of
+// one wavefront overlap the MFMAs of the others.
+template <int KMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_update(EigParams p) {
+  constexpr int TT = KMAX / 16;
+  constexpr int S = KMAX / 4;
+  // S * LDT = 16 (mod 32): the four lane quarters of a B read fall on
+  // disjoint banks in each half-wavefront
+  constexpr int LDT = KMAX == 64 ? KMAX + 1 : KMAX + 2;
+  static_assert(KMAX == 32 || KMAX == 64, "k_update covers k <= 64");
+  __shared__ double sT[KMAX * LDT];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long long a = p.a0 + blockIdx.x;
+  if (a >= p.a0 + p.n_an) return;
+  const long long al = a - p.a0;
+  if (p.status[a] != 0) return;
+  const int k = p.k;
+  const int i16 = lane & 15, q = lane >> 4;
+  const double* __restrict__ tb = p.refl + al * (long long)p.kp * p.kp;
+  for (int idx = tid; idx < TT * TT * 4 * 64; idx += 256) {
+    const int l = idx & 63, c = idx >> 6, r = c & 3, IJ = c >> 2;
+    const int row = 16 * (IJ / TT) + 4 * r + (l >> 4), col = 16 * (IJ % TT) + (l & 15);
+    sT[row * LDT + col] = (row < k && col < k) ? tb[idx] : 0.0;
+  }
+
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
+  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  const long long lat_ext = p.g.nlat + 2 * p.halo;
+  int sk = 1, ek = p.nz;
+  if (p.vertical) {
+    sk = kc * p.cube_nz + 1;
+    ek = min(sk + p.cube_nz - 1, p.nz);
+  }
+  // the analysis's state rows: all n of the column, or (vertical loop) the
+  // levels sk..ek of each 3-D variable and the 2-D variables on the top chunk;
+  // sidx maps the v-th of them to its offset in the column
+  auto run_len = [&](int vv) {
+    return p.var_lev[vv] == 1 ? (ek == p.nz ? 1 : 0) : ek - sk + 1;
+  };
+  int nvec = p.n;
+  if (p.vertical) {
+    nvec = 0;
+    for (int vv = 0; vv < p.n_var; ++vv) nvec += run_len(vv);
+  }
+  auto sidx = [&](int v) {
+    if (v >= nvec) return -1;
+    if (!p.vertical) return v;
+    int rem = v;
+    for (int vv = 0; vv < p.n_var; ++vv) {
+      const int len = run_len(vv);
+      if (rem < len) return p.var_start[vv] - 1 + (p.var_lev[vv] == 1 ? 0 : sk - 1) + rem;
+      rem -= len;
+    }
+    return -1;
+  };
+  const int ngrp = (nvec + 15) / 16;
+  const int ncl = ej - sj + 1;
+  const int n_items = (ei - si + 1) * ncl * ngrp;
+  const bool vec_ld = (k == KMAX) && ((reinterpret_cast<uintptr_t>(p.xb) & 15) == 0);
+  __syncthreads();
+  for (int it = w; it < n_items; it += 4) {
+    const int g = it % ngrp, cl = it / ngrp;
+    const int lo = si + cl / ncl, la = sj + cl % ncl;
+    const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+    const int v0 = 16 * g;
+    // A operands: variable v0 + i16, members S q .. S q + S - 1
+    double af[S];
+    const int s_a = sidx(v0 + i16);
+    if (s_a >= 0 && vec_ld) {
+      const double2* xr = reinterpret_cast<const double2*>(p.xb + (off + s_a) * k + S * q);
+#pragma unroll
+      for (int u = 0; u < S / 2; ++u) {
+        const double2 t2 = xr[u];
+        af[2 * u] = t2.x;
+        af[2 * u + 1] = t2.y;
+      }
+    } else {
+      const double* xr = p.xb + (off + (s_a >= 0 ? s_a : 0)) * k;
+#pragma unroll
+      for (int u = 0; u < S; ++u) {
+        const int m = S * q + u;
+        af[u] = (s_a >= 0 && m < k) ? xr[m] : 0.0;
+      }
+    }
+    double ab4[4];
+    int st4[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      st4[r] = sidx(v0 + q + 4 * r);
+      ab4[r] = (st4[r] >= 0) ? p.axb[off + st4[r]] : 0.0;
+    }
+    d4 out[TT];
+#pragma unroll
+    for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
+    // B of step s + 1 is read while the MFMAs of step s run; the scheduling
+    // barriers keep the compiler from hoisting every B read (and its
+    // registers) to the top of the loop
+    const double* bq = sT + S * q * LDT + i16;
+    double bn[TT];
+#pragma unroll
+    for (int J = 0; J < TT; ++J) bn[J] = bq[16 * J];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      double b[TT];
+#pragma unroll
+      for (int J = 0; J < TT; ++J) b[J] = bn[J];
+      if (s + 1 < S) {
+#pragma unroll
+        for (int J = 0; J < TT; ++J) bn[J] = bq[(s + 1) * LDT + 16 * J];
+      }
+#pragma unroll
+      for (int J = 0; J < TT; ++J)
+        out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], b[J], out[J], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // lane (i16, q) holds xa of variable v0 + q + 4 r, member 16 J + i16
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double sum = 0.0;
+      double xe[TT];
+#pragma unroll
+      for (int J = 0; J < TT; ++J) {
+        const int e = 16 * J + i16;
+        xe[J] = (e < k) ? out[J][r] + ab4[r] : 0.0;
+        sum += xe[J];
+      }
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
+      const double mean = sum / k;
+      if (st4[r] >= 0) {
+        const long long o = off + st4[r];
+#pragma unroll
+        for (int J = 0; J < TT; ++J) {
+          const int e = 16 * J + i16;
+          if (e < k) p.xa[o * k + e] = xe[J] - mean;
+        }
+        if (i16 == 0) p.axa[o] = mean;
       }
     }
   }
@@ -1751,8 +1901,7 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<32>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<32>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<32, true>), dim3(nq), dim3(64), 0, st, p);
-    else hipLaunchKernelGGL((k_ql_log<32, false>), dim3(nq), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
@@ -1760,14 +1909,14 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     } else {
       hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
       if ((e = mark(3)) != hipSuccess) return e;
-      if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(64), 0, st, p);
+      if (p.xb && p.update_reg) hipLaunchKernelGGL(k_update_reg<32>, dim3(nb), dim3(64), 0, st, p);
+      else if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
     }
   } else if (p.k <= 64) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<64, true>), dim3(nq), dim3(64), 0, st, p);
-    else hipLaunchKernelGGL((k_ql_log<64, false>), dim3(nq), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
@@ -1775,14 +1924,14 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     } else {
       hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
       if ((e = mark(3)) != hipSuccess) return e;
-      if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(64), 0, st, p);
+      if (p.xb && p.update_reg) hipLaunchKernelGGL(k_update_reg<64>, dim3(nb), dim3(64), 0, st, p);
+      else if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
     }
   } else if (p.k <= 128) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
-    if (p.ql_global) hipLaunchKernelGGL((k_ql_log<128, true>), dim3(nq), dim3(64), 0, st, p);
-    else hipLaunchKernelGGL((k_ql_log<128, false>), dim3(nq), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
     if (p.apply_fused) {
       hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);

@@ -96,17 +96,20 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 
 // Scratch of the split eigensolver for one batch of analyses.
 struct EigScratch {
-  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list, dq, eq;
+  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
   std::vector<int> st_host;
 };
 
 
-// Tridiagonalisation variant for k <= 64: DIDA_TRIDIAG=lds selects the
-// LDS-resident kernel, anything else the register-resident one.
+// Tridiagonalisation variant: DIDA_TRIDIAG=lds (1) selects the LDS-resident
+// kernel, DIDA_TRIDIAG=reg (2) the register-resident one, unset (0) the
+// faster of the two for the size.
 int tridiag_variant() {
   static const int v = [] {
     const char* e = std::getenv("DIDA_TRIDIAG");
-    return (e && std::strcmp(e, "lds") == 0) ? 1 : 0;
+    if (e && std::strcmp(e, "lds") == 0) return 1;
+    if (e && std::strcmp(e, "reg") == 0) return 2;
+    return 0;
   }();
   return v;
 }
@@ -115,12 +118,22 @@ int tridiag_variant() {
 // horizontal loop, k <= 64).
 constexpr int kTile = 8;
 
-// QL storage: DIDA_QL=global keeps d/e in a lane-interleaved global scratch
-// (occupancy by registers), anything else in LDS.
-int ql_variant() {
+// Visiting order of the products kernels: DIDA_BAND=w > 0 visits bands of w
+// longitudes latitude by latitude, 0 one meridian after another.
+int products_band() {
   static const int v = [] {
-    const char* e = std::getenv("DIDA_QL");
-    return (e && std::strcmp(e, "global") == 0) ? 1 : 0;
+    const char* e = std::getenv("DIDA_BAND");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  return v;
+}
+
+// Update variant for k <= 64: DIDA_UPDATE=reg selects the one-wavefront
+// update with trans in registers, anything else the LDS-staged block update.
+int update_variant() {
+  static const int v = [] {
+    const char* e = std::getenv("DIDA_UPDATE");
+    return (e && std::strcmp(e, "reg") == 0) ? 1 : 0;
   }();
   return v;
 }
@@ -185,16 +198,11 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.hdr = s.hdr.as<int4>();
   ep.rot_cap = rot_cap;
   ep.hdr_cap = hdr_cap;
-  ep.tridiag_lds = tridiag_variant();
+  // k <= 64: rows in registers; k = 65..128: A in LDS (k_tridiag_lds) unless
+  // DIDA_TRIDIAG=reg asks for k_tridiag128
+  ep.tridiag_lds = tridiag_variant() == 1 || (tridiag_variant() == 0 && kp > 64);
   ep.apply_fused = apply_variant();
-  ep.ql_global = ql_variant();
-  if (ep.ql_global) {
-    const size_t ng = (size_t)((n + 63) / 64) * 64 * kp;
-    RC_TRY(s.dq.ensure(sizeof(double) * ng));
-    RC_TRY(s.eq.ensure(sizeof(double) * ng));
-    ep.dq = s.dq.as<double>();
-    ep.eq = s.eq.as<double>();
-  }
+  ep.update_reg = update_variant();
   ep.nchase = s.nchase.as<int>();
   ep.status = xp.status;
   ep.nzc = xp.nzc;
@@ -408,8 +416,7 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
-                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list,
-                    &h->eig.dq, &h->eig.eq};
+                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
   for (DevBuf* b : bufs) b->release();
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -730,6 +737,10 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     pp.vco = h->o_vco.as<double>();
     pp.a0 = h0 * h->nzc;
     pp.n_an = (h1 - h0) * h->nzc;
+    if (products_band() > 0 && h->nzc == 1 && h0 % g.nlat_c == 0 && (h1 - h0) % g.nlat_c == 0) {
+      pp.band_w = products_band();
+      pp.band_l = g.nlat_c;
+    }
     pp.h0 = (int)h0;
     pp.A = h->Abuf.as<double>();
     pp.b = h->bbuf.as<double>();
@@ -996,7 +1007,7 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   EigScratch es;
   int rc_t = run_transform(st, es, xp, 0);
   DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
-                     &es.list, &es.dq, &es.eq};
+                     &es.list};
   if (rc_t < 0) {
     for (DevBuf* b : ebufs) b->release();
     return rc_t;

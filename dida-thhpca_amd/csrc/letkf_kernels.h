@@ -80,6 +80,8 @@ struct ProdParams {
   double* A;                 // batch-local, kp*kp per analysis
   double* b;                 // batch-local, kp per analysis
   const int* order;          // optional processing order (batch-local analysis offsets)
+  int band_w, band_l;        // band_w > 0: visit bands of band_w longitudes latitude by
+                             // latitude (band_l latitudes; batch of whole longitudes)
   int n_blocks;              // set by the launcher
   // tile mode (horizontal loop, k <= 64): row_off indexes tiles of `tile`
   // consecutive analyses; each union row carries its sorted site index
@@ -132,9 +134,7 @@ struct EigParams {
   int rot_cap, hdr_cap;
   int tridiag_lds;          // k <= 64: tridiagonalise in LDS (1) or in VGPRs (0)
   int apply_fused;          // k <= 64: k_apply (1) or k_vec + k_update (0)
-  int ql_global;            // QL with d/e in the lane-interleaved scratch (1) or in LDS (0)
-  double* dq;               // ql_global: ceil(n/64) groups of kp x 64 (d)
-  double* eq;               // ql_global: same layout (e)
+  int update_reg;           // k <= 64: the one-wavefront update with trans in registers
   int* nchase;              // batch-local
   int* status;              // global index
   int nzc, vertical, cube_nz, nz, halo, n, n_var;

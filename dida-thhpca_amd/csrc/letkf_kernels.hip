@@ -325,6 +325,22 @@ __device__ __forceinline__ int xcd_block(int n_blocks) {
   return (blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// Batch-local analysis visited at position li.  With band_w > 0 the
+// analyses (one per column, columns latitude-fastest) are visited in bands of
+// band_w longitudes, latitude by latitude, so the analyses in flight on an
+// XCD form a compact patch and share more of their gathered obs rows in its
+// L2 than a run along one meridian does.  A bijection of [0, n_an): the last
+// band may be narrower.
+__device__ __forceinline__ long long analysis_at(const ProdParams& p, long long li) {
+  if (p.order) return p.order[li];
+  if (p.band_w <= 0) return li;
+  const long long L = p.band_l, B = p.band_w;
+  const long long NL = p.n_an / L;
+  const long long b = li / (B * L), rem = li - b * B * L;
+  const long long Bb = min(B, NL - b * B);
+  return (b * B + rem % Bb) * L + rem / Bb;
+}
+
 template <int T>
 struct PairTable {
   static constexpr int N = T * (T + 1) / 2;
@@ -428,7 +444,7 @@ __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
   if (li >= p.n_an) return;
-  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
+  const long long a = p.a0 + analysis_at(p, li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
@@ -535,7 +551,7 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
   if (li >= p.n_an) return;
-  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
+  const long long a = p.a0 + analysis_at(p, li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
@@ -639,7 +655,7 @@ __global__ __launch_bounds__(256) void k_products_bf16(ProdParams p) {
   const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
   if (li >= p.n_an) return;
-  const long long a = p.a0 + (p.order ? (long long)p.order[li] : li);
+  const long long a = p.a0 + analysis_at(p, li);
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;

@@ -370,8 +370,8 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
         const int i = 8 * B + t;
         if (i > KMAX - 2) continue;
         const double zi = z[i];
-        z[i + 1] = cs[t].y * zi + cs[t].x * carry;
-        carry = cs[t].x * zi - cs[t].y * carry;
+        z[i + 1] = fma(cs[t].x, carry, cs[t].y * zi);
+        carry = fma(-cs[t].y, carry, cs[t].x * zi);
       }
       if (B == blo) z[8 * B] = carry;
     }
@@ -748,8 +748,9 @@ void k_vec(EigParams p) {
           const int i = 8 * B + t;
           if (i > KMAX - 2) continue;
           const double zi = z[i];
-          z[i + 1] = cs[t].y * zi + cs[t].x * carry;
-          carry = cs[t].x * zi - cs[t].y * carry;
+          // carry is the serial chain: one FMA per step
+          z[i + 1] = fma(cs[t].x, carry, cs[t].y * zi);
+          carry = fma(-cs[t].y, carry, cs[t].x * zi);
         }
         if (B == blo) z[8 * B] = carry;
       }
@@ -1252,10 +1253,10 @@ __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int 
     for (int t = 7; t >= 0; --t) {
       const int i = 8 * B + t;
       const double zi = z[i];
-      const double up = cs[t].y * zi + cs[t].x * carry;
+      const double up = fma(cs[t].x, carry, cs[t].y * zi);
       if (i == 63) ztop = up;
       else z[i + 1] = up;
-      carry = cs[t].x * zi - cs[t].y * carry;
+      carry = fma(-cs[t].y, carry, cs[t].x * zi);
     }
     if (B == blo && final_write) z[8 * B] = carry;
   }

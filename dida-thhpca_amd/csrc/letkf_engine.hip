@@ -118,12 +118,12 @@ int tridiag_variant() {
 // horizontal loop, k <= 64).
 constexpr int kTile = 8;
 
-// Visiting order of the products kernels: DIDA_BAND=w > 0 visits bands of w
-// longitudes latitude by latitude, 0 one meridian after another.
+// Visiting order of the products kernels: bands of w longitudes latitude by
+// latitude (w = 32 unless DIDA_BAND sets it; 0: one meridian after another).
 int products_band() {
   static const int v = [] {
     const char* e = std::getenv("DIDA_BAND");
-    return e ? std::max(0, std::atoi(e)) : 0;
+    return e ? std::max(0, std::atoi(e)) : 32;
   }();
   return v;
 }
@@ -155,19 +155,10 @@ size_t eig_bytes_per_analysis(int k) {
          16 * (size_t)kHdrCapPerK * k + sizeof(int);
 }
 
-// Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
-// eigensolver (k_tridiag, k_ql_log, k_apply), then the single-kernel
-// transform for analyses the split path flagged; mode 1 runs only the single
-// kernel.  k > 64 runs the LDS variant of the split path (k_tridiag_lds,
-// k_ql_log, k_apply_lds), which applies flagged analyses itself.
-// kev: null, or 4 events for the split kernels (recorded only if *split is set)
-int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
-                  hipEvent_t* kev = nullptr, bool* split = nullptr) {
-  if (split) *split = false;
-  if (mode == 1 && xp.k <= 64) {
-    HIP_TRY(launch_transform(st, xp));
-    return DIDA_OK;
-  }
+// Eigensolver parameters of the batch xp (its scratch allocated): the split
+// eigensolver (k_tridiag, k_ql_log, k_vec + k_update) over analyses
+// [xp.a0, xp.a0 + xp.n_an).
+int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   const long long n = xp.n_an;
   const int k = xp.k, kp = xp.kp;
   const int rot_cap = kRotCapPerK2 * k * k, hdr_cap = kHdrCapPerK * k;
@@ -179,7 +170,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   RC_TRY(s.rot.ensure((size_t)16 * n * rot_cap));
   RC_TRY(s.hdr.ensure((size_t)16 * n * hdr_cap));
   RC_TRY(s.nchase.ensure(sizeof(int) * n));
-  EigParams ep{};
+  ep = EigParams{};
   ep.g = xp.g;
   ep.k = k;
   ep.kp = kp;
@@ -222,9 +213,15 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   ep.axa = xp.axa;
   ep.trans_out = xp.trans_out;
   ep.w_out = xp.w_out;
-  HIP_TRY(launch_eig(st, ep, kev));
-  if (split) *split = true;
-  if (k > 64) return DIDA_OK;
+  return DIDA_OK;
+}
+
+// After the split eigensolver (k <= 64): the single-kernel transform reruns
+// the analyses the split path flagged (k > 64 applies flagged analyses in
+// the split kernels themselves).  Synchronises the stream.
+int rerun_flagged(hipStream_t st, EigScratch& s, const XformParams& xp) {
+  if (xp.k > 64) return DIDA_OK;
+  const long long n = xp.n_an;
   s.st_host.resize(n);
   HIP_TRY(hipMemcpyAsync(s.st_host.data(), xp.status + xp.a0, sizeof(int) * n,
                          hipMemcpyDeviceToHost, st));
@@ -243,6 +240,24 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
     HIP_TRY(hipStreamSynchronize(st));
   }
   return DIDA_OK;
+}
+
+// Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
+// eigensolver, then the single-kernel transform for analyses the split path
+// flagged; mode 1 runs only the single kernel (k <= 64).
+// kev: null, or 5 events for the split kernels (recorded only if *split is set)
+int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
+                  hipEvent_t* kev = nullptr, bool* split = nullptr) {
+  if (split) *split = false;
+  if (mode == 1 && xp.k <= 64) {
+    HIP_TRY(launch_transform(st, xp));
+    return DIDA_OK;
+  }
+  EigParams ep;
+  RC_TRY(prepare_eig(s, xp, ep));
+  HIP_TRY(launch_eig(st, ep, kev));
+  if (split) *split = true;
+  return rerun_flagged(st, s, xp);
 }
 
 namespace dida {

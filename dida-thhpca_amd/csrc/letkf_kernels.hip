@@ -581,8 +581,24 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   double bacc[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  // the row indices of the next group are loaded one group ahead, so only
+  // the Yb gather of a group waits on memory
+  int idx_n[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long rr = r0 + 8 * q + u;
+    idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+  }
   for (long long base = r0; base < r1; base += 32) {
     bf8 hi[T], lo[T];
+    int idx[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long rr = base + 32 + 8 * q + u;
+      idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const long long rr = base + 8 * q + u;
@@ -590,7 +606,7 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
       double wd = 0.0;
       const float* yr = nullptr;
       if (rr < r1) {
-        const int row = p.row_idx[rr];
+        const int row = idx[u];
         double w;
         if (p.pre_weighted) {
           w = p.row_gc[rr];
@@ -645,7 +661,8 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
 }
 
 template <int KP>
-__global__ __launch_bounds__(256) void k_products_bf16(ProdParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP <= 64 ? 3 : 1)))
+void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
   constexpr int WPA = (NPAIR + 9) / 10;

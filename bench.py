@@ -172,11 +172,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16, help="every Nth column for the CPU baseline")
     ap.add_argument("--parity-cols", type=int, default=48)
-    ap.add_argument("--serve-jobs", type=int, default=0,
+    ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs per arrival rate in the serving leg (0: no serving leg)")
-    ap.add_argument("--serve-rates", default="0.5,0.8",
+    ap.add_argument("--serve-rates", default="0.8",
                     help="arrival rates as fractions of the measured saturation rate")
-    ap.add_argument("--serve-sat-jobs", type=int, default=0,
+    ap.add_argument("--serve-sat-jobs", type=int, default=12,
                     help="jobs of the closed burst that measures saturation (0: = serve-jobs)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy checks")

@@ -95,6 +95,43 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Cross-lane moves of a double on the VALU: a DPP permutation within rows of
+// 16 lanes, and a read of one lane (the lane index must be wavefront-uniform).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Reductions for code where all 64 lanes are active: four DPP steps reduce
+// each row of 16 lanes (xor 1, xor 2, then the half-row and row mirrors, which
+// pair lanes holding equal partial values) on the VALU, then two butterfly
+// shuffles (xor 16, xor 32) combine the rows.  Every lane gets the bitwise
+// same value (partners add the same two operands).
+template <typename Op>
+__device__ __forceinline__ double wave_reduce_full(double v, Op op) {
+  v = op(v, dpp_d<0xB1>(v));    // quad_perm [1,0,3,2]
+  v = op(v, dpp_d<0x4E>(v));    // quad_perm [2,3,0,1]
+  v = op(v, dpp_d<0x141>(v));   // row_half_mirror
+  v = op(v, dpp_d<0x140>(v));   // row_mirror
+  v = op(v, __shfl_xor(v, 16, 64));
+  return op(v, __shfl_xor(v, 32, 64));
+}
+__device__ __forceinline__ double wave_sum_full(double v) {
+  return wave_reduce_full(v, [](double a, double b) { return a + b; });
+}
+__device__ __forceinline__ double wave_min_full(double v) {
+  return wave_reduce_full(v, [](double a, double b) { return fmin(a, b); });
+}
+
 // exclusive prefix sum of an int over the 64 lanes
 __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
   int x = v;

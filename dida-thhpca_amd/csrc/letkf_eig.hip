@@ -32,7 +32,8 @@ namespace dida {
 // k_tridiag
 // ---------------------------------------------------------------------------
 template <int KMAX>
-__global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void k_tridiag(EigParams p) {
   constexpr int NB = KMAX / 8;
   __shared__ double sv[KMAX], sw[KMAX];
   __shared__ double se_s[KMAX + 1], st_s[KMAX + 1];
@@ -55,8 +56,9 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
   for (int j = 0; j + 2 < k; ++j) {
     const bool act = in_r && lane > j;
     const double x = act ? xcol : 0.0;
-    const double x0 = __shfl(xcol, j + 1, 64);
-    const double rest = wave_sum((act && lane > j + 1) ? x * x : 0.0);
+    const double x0 = readlane_d(xcol, j + 1);
+    const double rest = wave_sum_full((act && lane > j + 1) ? x * x : 0.0);
+    __builtin_amdgcn_sched_barrier(0);
     if (rest == 0.0) {
       // column already in tridiagonal form; column j+1 goes through LDS,
       // written under a uniform branch (no per-slot selects)
@@ -94,7 +96,8 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     }
     double pr = (pr0 + pr1) + (pr2 + pr3);
     pr = act ? pr * tau : 0.0;
-    const double K = 0.5 * tau * wave_sum(v * pr);
+    const double K = 0.5 * tau * wave_sum_full(v * pr);
+    __builtin_amdgcn_sched_barrier(0);
     const double wr = act ? pr - K * v : 0.0;
     if (lane < KMAX) sw[lane] = wr;
     __syncthreads();
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(64) void k_tridiag(EigParams p) {
     dr = (c == lane) ? arow[c] : dr;
     ek2 = (c == k - 2) ? arow[c] : ek2;
   }
-  ek2 = __shfl(ek2, k - 1, 64);
+  ek2 = readlane_d(ek2, k - 1);
   __syncthreads();
   double* __restrict__ dg = p.d + al * kp;
   double* __restrict__ eg = p.e + al * kp;
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
   // ---- eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
   double lam = in_r ? p.lam[al * kp + lane] : 0.0;
   if (p.guard) {
-    const double mn = uniform(wave_min(in_r ? lam : 1e300));
+    const double mn = wave_min_full(in_r ? lam : 1e300);
     if (mn < 0.0) lam = lam - mn;
   }
   lam = lam * 1.0 + (double)(k - 1) / p.inflation;
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(64) void k_apply(EigParams p) {
 // NW partials so that back-to-back calls need one barrier each.
 template <int NW>
 __device__ __forceinline__ double block_sum(double v, double* red, int& slot) {
-  const double w = wave_sum(v);
+  const double w = wave_sum_full(v);
   if constexpr (NW == 1) {
     return uniform(w);
   } else {
@@ -590,7 +593,7 @@ __device__ __forceinline__ double block_sum(double v, double* red, int& slot) {
 }
 
 __device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
-  const double w = wave_min(v);
+  const double w = wave_min_full(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[2 * slot + wv] = w;
   __syncthreads();
@@ -761,7 +764,7 @@ void k_vec(EigParams p) {
   // ---- C. eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
   double lam = in_r ? p.lam[al * kp + lane] : 0.0;
   if (p.guard) {
-    const double mn = uniform(wave_min(in_r ? lam : 1e300));
+    const double mn = wave_min_full(in_r ? lam : 1e300);
     if (mn < 0.0) lam = lam - mn;
   }
   lam = lam * 1.0 + (double)(k - 1) / p.inflation;

@@ -1136,17 +1136,30 @@ void k_update(EigParams p) {
 // ---------------------------------------------------------------------------
 
 // Householder tridiagonalisation for k = 65..128 with half rows of A in
-// registers (thread t: row t % 128, columns 64h .. 64h+63, h = t / 128);
-// same outputs as k_tridiag.  The halves' partial products are summed
-// through LDS in a fixed order; column j+1 after step j's update goes
-// through LDS under a uniform branch.
+// registers (thread t: row t % 128, columns 64h .. 64h+63, h = t / 128, one
+// half per wavefront); same outputs as k_tridiag.  Three barriers per step:
+//  - the pivot column j is in LDS (double-buffered by the parity of j) with
+//    rows <= j zeroed; each wavefront writes v_{j+1} = x0 - alpha over its
+//    entry j+1 before reading it (every wavefront writes the same value), so
+//    v is read from the column directly; x0 and the sum of squares of the
+//    column below it come from the previous step;
+//  - the partial products of the two halves and the per-wavefront partials of
+//    v.p share one barrier;
+//  - w, then the update, the extraction of column j+1 (stores under
+//    wavefront-uniform branches, read back for its squares) share the step's
+//    last barrier.
+// Partial sums are combined in a fixed order, so every thread holds the same
+// values.
 __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
   constexpr int KMAX = 128;
-  __shared__ double sv[KMAX], sw[KMAX], sxn[KMAX], spr[2][KMAX];
+  __shared__ double sx[2][KMAX];          // pivot column, rows <= j zeroed
+  __shared__ double sx0[2];               // A(j+1, j) of the pivot column
+  __shared__ double sw[KMAX], spr[2][KMAX];
   __shared__ double se_s[KMAX + 1], st_s[KMAX + 1], sd[KMAX];
-  __shared__ double red[8];
+  __shared__ double rsq[2][4], rvp[4];    // per-wavefront partials
   const int tid = threadIdx.x;
   const int r = tid & 127, h = uniform(tid >> 7);   // the half is one per wavefront
+  const int wv = tid >> 6, lane = tid & 63;
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
   const long long al = a - p.a0;
@@ -1155,24 +1168,55 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
   double* __restrict__ refl = p.refl + al * (long long)kp * kp;
   const bool in_r = r < k;
   double arow[64];
+  // eight columns in flight at a time: the loads need one address each
 #pragma unroll
-  for (int c = 0; c < 64; ++c) {
-    const int col = 64 * h + c;
-    arow[c] = (in_r && col < k) ? Ag[(long long)col * kp + r] : 0.0;
+  for (int B = 0; B < 8; ++B) {
+#pragma unroll
+    for (int c = 8 * B; c < 8 * B + 8; ++c) {
+      // unconditional in-bounds load (r < kp), then a select: no branch
+      // per column
+      const int col = 64 * h + c;
+      const double av = Ag[(long long)col * kp + r];
+      arow[c] = (in_r && col < k) ? av : 0.0;
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
-  if (h == 0) sxn[r] = arow[0];
-  int slot = 0;
+  // column 0 (held by half 0) and the squares of its rows > 1
+  if (h == 0) {
+    sx[0][r] = (r > 0) ? arow[0] : 0.0;
+    if (r == 1) sx0[0] = arow[0];
+  }
+  {
+    const double sq = wave_sum_full((h == 0 && r > 1) ? arow[0] * arow[0] : 0.0);
+    if (lane == 0) rsq[0][wv] = sq;
+  }
   __syncthreads();
-  for (int j = 0; j + 2 < k; ++j) {
-    const bool act = in_r && r > j;
-    const double x = act ? sxn[r] : 0.0;
-    const double x0 = uniform(sxn[j + 1]);
-    const double rest = block_sum<4>((h == 0 && act && r > j + 1) ? x * x : 0.0, red, slot);
-    if (rest == 0.0) {
-      // column already in tridiagonal form; pass column j+1 on
+  // column j+1 -> sx[nxt] (rows <= j+1 zeroed) and sx0[nxt]; partial squares
+  // of its rows > j+2 -> rsq[nxt]
+  auto pass_column = [&](int j, int nxt) {
+    const int hj1 = (j + 1) >> 6;
+    if (h == hj1) {
 #pragma unroll
-      for (int c = 0; c < 64; ++c)
-        if (64 * h + c == j + 1) sxn[r] = arow[c];
+      for (int c = 0; c < 64; ++c) {
+        if (64 * h + c == j + 1) {
+          sx[nxt][r] = (r > j + 1) ? arow[c] : 0.0;
+          if (r == j + 2) sx0[nxt] = arow[c];
+        }
+      }
+    }
+    const double xn = (h == hj1 && r > j + 2) ? sx[nxt][r] : 0.0;
+    const double sq = wave_sum_full(xn * xn);
+    if (lane == 0) rsq[nxt][wv] = sq;
+  };
+  for (int j = 0; j + 2 < k; ++j) {
+    const int cur = j & 1, nxt = cur ^ 1;
+    double* col = sx[cur];
+    const bool act = in_r && r > j;
+    const double x0 = uniform(sx0[cur]);
+    const double rest = (rsq[cur][0] + rsq[cur][1]) + (rsq[cur][2] + rsq[cur][3]);
+    if (rest == 0.0) {
+      // column already in tridiagonal form: pass column j+1 on unchanged
+      pass_column(j, nxt);
       if (h == 0) refl[(long long)j * kp + r] = 0.0;
       if (tid == 0) { se_s[j] = x0; st_s[j] = 0.0; }
       __syncthreads();
@@ -1182,36 +1226,45 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
     const double alpha = (x0 >= 0.0) ? -nrm : nrm;
     const double v0 = x0 - alpha;
     const double tau = 2.0 / (rest + v0 * v0);
-    const double v = act ? ((r == j + 1) ? v0 : x) : 0.0;
-    if (h == 0) sv[r] = v;
-    __syncthreads();
+    if (lane == 0) col[j + 1] = v0;
+    const double vr = act ? col[r] : 0.0;
+    // partial p over this half (v_c = 0 for c <= j)
     double p0 = 0.0, p1 = 0.0;
 #pragma unroll
     for (int B = 0; B < 8; ++B) {
 #pragma unroll
       for (int c = 8 * B; c < 8 * B + 8; c += 2) {
-        p0 = fma(arow[c], sv[64 * h + c], p0);
-        p1 = fma(arow[c + 1], sv[64 * h + c + 1], p1);
+        p0 = fma(arow[c], col[64 * h + c], p0);
+        p1 = fma(arow[c + 1], col[64 * h + c + 1], p1);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-    spr[h][r] = p0 + p1;
+    const double ph = p0 + p1;
+    if (in_r) spr[h][r] = ph;
+    {
+      const double vp = wave_sum_full(vr * ph);
+      if (lane == 0) rvp[wv] = vp;
+    }
     __syncthreads();
     const double pr = act ? (spr[0][r] + spr[1][r]) * tau : 0.0;
-    const double K = 0.5 * tau * block_sum<4>(h == 0 ? v * pr : 0.0, red, slot);
-    const double wr = act ? pr - K * v : 0.0;
+    const double K = 0.5 * tau * tau * ((rvp[0] + rvp[1]) + (rvp[2] + rvp[3]));
+    const double wr = act ? pr - K * vr : 0.0;
     if (h == 0) sw[r] = wr;
     __syncthreads();
+    // A -= v w^T + w v^T over this half
+    if (act) {
 #pragma unroll
-    for (int B = 0; B < 8; ++B) {
+      for (int B = 0; B < 8; ++B) {
 #pragma unroll
-      for (int c = 8 * B; c < 8 * B + 8; ++c) {
-        if (act) arow[c] -= v * sw[64 * h + c] + wr * sv[64 * h + c];
-        if (64 * h + c == j + 1) sxn[r] = arow[c];
+        for (int c = 8 * B; c < 8 * B + 8; ++c) {
+          const int cg = 64 * h + c;
+          arow[c] -= fma(vr, sw[cg], wr * col[cg]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
     }
-    if (h == 0) refl[(long long)j * kp + r] = v;
+    pass_column(j, nxt);
+    if (h == 0) refl[(long long)j * kp + r] = vr;
     if (tid == 0) { se_s[j] = alpha; st_s[j] = tau; }
     __syncthreads();
   }

@@ -102,14 +102,11 @@ struct EigScratch {
 
 
 // Tridiagonalisation variant: DIDA_TRIDIAG=lds (1) selects the LDS-resident
-// kernel, DIDA_TRIDIAG=reg (2) the register-resident one, unset (0) the
-// faster of the two for the size.
+// kernel, anything else the register-resident one (faster at every k).
 int tridiag_variant() {
   static const int v = [] {
     const char* e = std::getenv("DIDA_TRIDIAG");
-    if (e && std::strcmp(e, "lds") == 0) return 1;
-    if (e && std::strcmp(e, "reg") == 0) return 2;
-    return 0;
+    return (e && std::strcmp(e, "lds") == 0) ? 1 : 0;
   }();
   return v;
 }
@@ -189,9 +186,9 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   ep.hdr = s.hdr.as<int4>();
   ep.rot_cap = rot_cap;
   ep.hdr_cap = hdr_cap;
-  // k <= 64: rows in registers; k = 65..128: A in LDS (k_tridiag_lds) unless
-  // DIDA_TRIDIAG=reg asks for k_tridiag128
-  ep.tridiag_lds = tridiag_variant() == 1 || (tridiag_variant() == 0 && kp > 64);
+  // rows (k <= 64) or half rows (k_tridiag128) of A in registers unless
+  // DIDA_TRIDIAG=lds asks for the LDS-resident kernel
+  ep.tridiag_lds = tridiag_variant() == 1;
   ep.apply_fused = apply_variant();
   ep.update_reg = update_variant();
   ep.nchase = s.nchase.as<int>();

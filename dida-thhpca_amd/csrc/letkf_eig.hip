@@ -11,18 +11,22 @@
 //   k_ql_log    one LANE per matrix (64 matrices per wavefront): the scalar
 //               QL recurrence, which is a serial chain per matrix; writes the
 //               eigenvalues and a log of the Givens rotations of every chase
-//   k_apply     one wavefront per matrix, lane r keeps row r of Q in VGPRs:
-//               Q = H_0 ... H_{k-3} by forward accumulation, the logged
-//               rotations applied in order (V = Q Z), then
-//               overline_w = V diag(1/lambda) V^T b,
-//               T = V diag(sqrt((k-1)/lambda)) V^T, trans = T + overline_w 1^T
-//               (letkf_core.F90:129-225) and the ensemble update
+//   k_vec       one wavefront per matrix (k <= 64), lane r keeps row r of Z
+//               in VGPRs: Q = H_0 ... H_{k-3} by forward accumulation, the
+//               logged rotations applied in order (V = Q Z), then
+//               overline_w = V diag(1/lambda) V^T b and
+//               trans = V diag(sqrt((k-1)/lambda)) V^T + overline_w 1^T
+//               (letkf_core.F90:129-225) on f64 MFMA
+//   k_update    the ensemble update xa = trans^T xb + axb, re-centred
 //               (xb_2_xa, letkf_compute.F90:259-329, or the per-level update
-//               of the vertical loop, letkf_compute.F90:501-552).
+//               of the vertical loop, letkf_compute.F90:501-552)
+// k = 65..128 runs k_tridiag128, k_ql_log<128>, k_vec128 and k_update128
+// with half rows per thread (256 threads per matrix).
 //
 // Matrices whose QL did not converge or whose rotation log overflowed are
-// flagged (status != 0) and skipped by k_apply; the host reruns them with the
-// single-kernel transform.
+// flagged (status != 0).  For k <= 64 k_vec and k_update skip them and the
+// host reruns them with the single-kernel transform; for k > 64 they are
+// applied with the inexact basis and the flag is returned to the caller.
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
@@ -181,12 +185,16 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       }
       if (m == l) break;
       if (++iter > 30) { status = l + 1; break; }
-      if (nhdr >= p.hdr_cap || nrot + (m - l) > p.rot_cap) { status = -1; break; }
+      if (nhdr >= p.hdr_cap || nrot + (m - l) + 14 > p.rot_cap) { status = -1; break; }
       double g = (SD(l + 1) - SD(l)) / (2.0 * SE(l));
       double r = hypot(g, 1.0);
       g = SD(m) - SD(l) + SE(l) / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
       double s = 1.0, c = 1.0, pp = 0.0;
+      // the chase's log is block-aligned: positions m .. 8 ceil(m/8) - 1
+      // (above it) and 8 floor(lo/8) .. lo - 1 (below) hold the identity, so
+      // a reader takes whole blocks of 8 positions without masking
       const int off = nrot;
+      for (int q = ((m + 7) & ~7) - 1; q >= m; --q) rot[nrot++] = make_double2(1.0, 0.0);
       int i = m - 1;
       bool underflow = false;
       // d[i], e[i] of the next step are read before this step's stores
@@ -223,7 +231,9 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
         di = din;
       }
       const int lo = underflow ? i + 1 : l;
-      if (m - lo > 0) hdr[nhdr++] = make_int4(lo, m, off, 0);
+      for (int q = lo - 1; q >= (lo & ~7); --q) rot[nrot++] = make_double2(1.0, 0.0);
+      if (m - lo > 0) hdr[nhdr++] = make_int4(lo, m, off, nrot - off);
+      else nrot = off;
       if (underflow) continue;
       SD(l) -= pp;
       SE(l) = g;
@@ -236,341 +246,6 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
 }
 #undef SD
 #undef SE
-
-// ---------------------------------------------------------------------------
-// k_apply
-// ---------------------------------------------------------------------------
-template <int KMAX>
-__global__ __launch_bounds__(64) void k_apply(EigParams p) {
-  constexpr int NB = KMAX / 8;
-  constexpr int TT = KMAX / 16;          // 16x16 tiles per dimension
-  constexpr int LDV = KMAX + 1;
-  constexpr int HMAX = kHdrCapPerK * KMAX;            // chase headers per analysis
-  constexpr int RING = 16 * KMAX;                      // rotation ring (entries)
-  constexpr int REFILL = RING / 2;                     // entries per ring refill
-  // phase B layout (doubles): headers 2*HMAX, ring 2*RING, dense chase table 2*KMAX
-  constexpr int SB_B = 2 * HMAX + 2 * RING + 2 * KMAX;
-  constexpr int SBUF = (KMAX * LDV > SB_B) ? KMAX * LDV : SB_B;
-  static_assert(REFILL >= KMAX && REFILL % 64 == 0, "a refill covers any chase");
-  // phase A: reflectors (row j = v_j); phase B: chase headers + rotation ring;
-  // phases C, D: V rows, then staged background vectors
-  __shared__ __align__(16) double sbuf[SBUF];
-  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX];
-  double* sV = sbuf;
-  const int lane = threadIdx.x;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  if (p.status[a] != 0) return;          // rerun by the single-kernel transform
-  const int k = p.k, kp = p.kp;
-  const bool in_r = lane < k;
-
-  // ---- A. Q rows by forward accumulation: Z <- Z H_j ----
-  // All reflectors and taus are staged in LDS first (loads issued in groups
-  // of 16, one memory latency per group instead of one per reflector).
-  const double* __restrict__ refl = p.refl + al * (long long)kp * kp;
-  const double* __restrict__ tg = p.tau + al * kp;
-  const int nref = k > 2 ? k - 2 : 0;
-#pragma unroll
-  for (int j0 = 0; j0 < KMAX; j0 += 16) {
-    double v[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t)
-      v[t] = (j0 + t < nref && lane < KMAX) ? refl[(long long)(j0 + t) * kp + lane] : 0.0;
-#pragma unroll
-    for (int t = 0; t < 16; ++t)
-      if (j0 + t < nref && lane < KMAX) sV[(j0 + t) * LDV + lane] = v[t];
-  }
-  if (lane < nref) st[lane] = tg[lane];
-  __syncthreads();
-  double z[KMAX];
-#pragma unroll
-  for (int c = 0; c < KMAX; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
-  for (int j = 0; j < nref; ++j) {
-    const double tau = uniform(st[j]);
-    if (tau == 0.0) continue;
-    const double* q = sV + j * LDV;
-    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-#pragma unroll
-    for (int B = 0; B < NB; ++B) {
-      if (8 * B + 7 <= j) continue;
-      d0 = fma(z[8 * B + 0], q[8 * B + 0], d0);
-      d1 = fma(z[8 * B + 1], q[8 * B + 1], d1);
-      d2 = fma(z[8 * B + 2], q[8 * B + 2], d2);
-      d3 = fma(z[8 * B + 3], q[8 * B + 3], d3);
-      d0 = fma(z[8 * B + 4], q[8 * B + 4], d0);
-      d1 = fma(z[8 * B + 5], q[8 * B + 5], d1);
-      d2 = fma(z[8 * B + 6], q[8 * B + 6], d2);
-      d3 = fma(z[8 * B + 7], q[8 * B + 7], d3);
-    }
-    const double dot = ((d0 + d1) + (d2 + d3)) * tau;
-#pragma unroll
-    for (int B = 0; B < NB; ++B) {
-      if (8 * B + 7 <= j) continue;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) z[8 * B + t] = fma(-dot, q[8 * B + t], z[8 * B + t]);
-    }
-  }
-  __syncthreads();
-
-  // ---- B. the logged QL rotations, chase by chase ----
-  // Headers are staged in LDS once; rotations stream through an LDS ring,
-  // REFILL entries (REFILL / 64 coalesced loads per lane) at a time.
-  int4* shdr = reinterpret_cast<int4*>(sbuf);
-  double2* sring = reinterpret_cast<double2*>(sbuf + 2 * HMAX);
-  double2* stab = reinterpret_cast<double2*>(sbuf + 2 * HMAX + 2 * RING);
-  const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
-  const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
-  const int nch = min(p.nchase[al], HMAX);
-  for (int i = lane; i < nch; i += 64) shdr[i] = hdr[i];
-  __syncthreads();
-  int nrot = 0;
-  if (nch > 0) {
-    const int4 hl = shdr[nch - 1];
-    nrot = uniform(hl.z + (hl.y - hl.x));
-  }
-  int committed = 0;                     // rotation entries present in the ring
-  for (int ch = 0; ch < nch; ++ch) {
-    int4 h = shdr[ch];
-    h.x = uniform(h.x);
-    h.y = uniform(h.y);
-    h.z = uniform(h.z);
-    const int lo = h.x, hi = h.y;
-    if (h.z + (hi - lo) > committed) {
-      double2 v[REFILL / 64];
-#pragma unroll
-      for (int t = 0; t < REFILL / 64; ++t) {
-        const int e = committed + 64 * t + lane;
-        v[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
-      }
-#pragma unroll
-      for (int t = 0; t < REFILL / 64; ++t)
-        sring[(committed + 64 * t + lane) & (RING - 1)] = v[t];
-      committed += REFILL;
-      __syncthreads();
-    }
-    const int bhi = (hi - 1) >> 3, blo = lo >> 3;
-    const int base = h.z + hi - 1;       // entry of the rotation at position i is base - i
-    // dense table of this chase: position i holds its rotation, positions
-    // outside [lo, hi) the identity, so the loop below is multiplies only
-    if (lane < KMAX) {
-      const bool in = lane >= lo && lane < hi;
-      const double2 g = sring[(base - lane) & (RING - 1)];
-      stab[lane] = in ? g : make_double2(1.0, 0.0);
-    }
-    __syncthreads();
-    double carry = 0.0;
-#pragma unroll
-    for (int B = NB - 1; B >= 0; --B) {
-      if (B > bhi || B < blo) continue;
-      const int top = (8 * B + 7 < KMAX - 1) ? 8 * B + 7 : KMAX - 2;
-      if (B == bhi) carry = z[top + 1];
-      double2 cs[8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) cs[t] = stab[8 * B + t];
-#pragma unroll
-      for (int t = 7; t >= 0; --t) {
-        const int i = 8 * B + t;
-        if (i > KMAX - 2) continue;
-        const double zi = z[i];
-        z[i + 1] = fma(cs[t].x, carry, cs[t].y * zi);
-        carry = fma(-cs[t].y, carry, cs[t].x * zi);
-      }
-      if (B == blo) z[8 * B] = carry;
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-
-  // ---- eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
-  double lam = in_r ? p.lam[al * kp + lane] : 0.0;
-  if (p.guard) {
-    const double mn = wave_min_full(in_r ? lam : 1e300);
-    if (mn < 0.0) lam = lam - mn;
-  }
-  lam = lam * 1.0 + (double)(k - 1) / p.inflation;
-  if (lane < KMAX) {
-    sf[lane] = in_r ? sqrt((double)(k - 1) / lam) : 0.0;
-    sg[lane] = in_r ? 1.0 / lam : 0.0;
-    sb[lane] = in_r ? p.b[al * kp + lane] : 0.0;
-  }
-  // V rows to LDS (padding rows/cols beyond k are the identity, harmless:
-  // f, g and b are zero there)
-  if (lane < KMAX) {
-#pragma unroll
-    for (int c = 0; c < KMAX; ++c) sV[lane * LDV + c] = z[c];
-  }
-  __syncthreads();
-  // overline_w = V diag(1/lambda) V^T b  (letkf_core.F90:129-195 regrouped)
-  if (lane < KMAX) {
-    double tb = 0.0;
-    for (int r = 0; r < KMAX; ++r) tb = fma(sV[r * LDV + lane], sb[r], tb);
-    st[lane] = tb * sg[lane];
-  }
-  __syncthreads();
-  double wbar = 0.0;
-#pragma unroll
-  for (int c = 0; c < KMAX; ++c) wbar = fma(z[c], st[c], wbar);
-  if (lane < KMAX) sw[lane] = in_r ? wbar : 0.0;
-  __syncthreads();
-
-  // ---- T = V diag(f) V^T on f64 MFMA (16x16x4), all TT x TT tiles ----
-  // A(I,s) = V[16I + l%16][4s + l/16], B(s,J) = f_{4s+l/16} V[16J + l%16][4s + l/16];
-  // tile (I,J) reg r of lane l holds T[16I + l/16 + 4r][16J + l%16], which is
-  // also the B operand of the update GEMM for k-step s = 4I + r.
-  const int i16 = lane & 15, q = lane >> 4;
-  d4 acc[TT][TT];
-#pragma unroll
-  for (int I = 0; I < TT; ++I)
-#pragma unroll
-    for (int J = 0; J < TT; ++J) acc[I][J] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
-  for (int sK = 0; sK < KMAX / 4; ++sK) {
-    const int c = 4 * sK + q;
-    const double fc = sf[c];
-    double fr[TT];
-#pragma unroll
-    for (int I = 0; I < TT; ++I) fr[I] = sV[(16 * I + i16) * LDV + c];
-#pragma unroll
-    for (int I = 0; I < TT; ++I)
-#pragma unroll
-      for (int J = 0; J < TT; ++J)
-        acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[I], fc * fr[J], acc[I][J], 0, 0, 0);
-  }
-  // trans(f, e) = T(f, e) + overline_w(f)  (letkf_core.F90:221-225)
-#pragma unroll
-  for (int I = 0; I < TT; ++I)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double wf = sw[16 * I + q + 4 * r];
-#pragma unroll
-      for (int J = 0; J < TT; ++J) acc[I][J][r] += wf;
-    }
-  if (p.trans_out) {
-    double* tr = p.trans_out + a * (long long)k * k;
-#pragma unroll
-    for (int I = 0; I < TT; ++I)
-#pragma unroll
-      for (int J = 0; J < TT; ++J)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int f = 16 * I + q + 4 * r, e = 16 * J + i16;
-          if (f < k && e < k) tr[(long long)e * k + f] = acc[I][J][r];
-        }
-  }
-  if (p.w_out && in_r) p.w_out[a * (long long)k + lane] = wbar;
-  if (!p.xb) return;
-
-  // ---- ensemble update on f64 MFMA: Xa(16 vectors x k) = Xb * trans ----
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
-  const long long lat_ext = p.g.nlat + 2 * p.halo;
-  int sk = 1, ek = p.nz;
-  if (p.vertical) {
-    sk = kc * p.cube_nz + 1;
-    ek = min(sk + p.cube_nz - 1, p.nz);
-  }
-  // runs of contiguous state vectors of one column: (state start, length)
-  int rs[kMaxVars], rl[kMaxVars];
-  int nrun = 0, nvec = 0;
-  if (!p.vertical) {
-    rs[0] = 0; rl[0] = p.n; nrun = 1;
-  } else {
-    for (int vv = 0; vv < p.n_var; ++vv) {
-      const int start = p.var_start[vv] - 1;
-      if (p.var_lev[vv] == 1) {
-        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
-      } else {
-        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
-      }
-    }
-  }
-  for (int t = 0; t < nrun; ++t) nvec += rl[t];
-  // state index of vector v of the column (-1 past the end)
-  auto sidx = [&](int v) {
-    int rem = v;
-    for (int t = 0; t < nrun; ++t) {
-      if (rem < rl[t]) return rs[t] + rem;
-      rem -= rl[t];
-    }
-    return -1;
-  };
-  // one group = 16 state vectors: lane e loads member e of each, plus the
-  // means of the 4 vectors whose outputs it writes; the next group is loaded
-  // while the current one is multiplied
-  double xn[16], abn[4];
-  auto load_group = [&](long long off, int v0) {
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-      xn[u] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + lane] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int v = v0 + (lane >> 4) + 4 * r;
-      const int s_i = (v < nvec) ? sidx(v) : -1;
-      abn[r] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
-    }
-  };
-  for (int lo = si; lo <= ei; ++lo) {
-    for (int la = sj; la <= ej; ++la) {
-      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
-      load_group(off, 0);
-      for (int v0 = 0; v0 < nvec; v0 += 16) {
-        __syncthreads();
-        if (lane < KMAX) {
-#pragma unroll
-          for (int u = 0; u < 16; ++u) sV[u * LDV + lane] = xn[u];
-        }
-        double ab4[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) ab4[r] = abn[r];
-        __syncthreads();
-        if (v0 + 16 < nvec) load_group(off, v0 + 16);
-        d4 out[TT];
-#pragma unroll
-        for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int sK = 0; sK < KMAX / 4; ++sK) {
-          const double xa_frag = sV[i16 * LDV + 4 * sK + q];
-#pragma unroll
-          for (int J = 0; J < TT; ++J)
-            out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa_frag, acc[sK >> 2][J][sK & 3],
-                                                          out[J], 0, 0, 0);
-        }
-        // lane holds Xa[vector q + 4r][member 16J + i16]
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int v = v0 + q + 4 * r;
-          const int st_i = (v < nvec) ? sidx(v) : -1;
-          const double ab = ab4[r];
-          double sum = 0.0;
-          double xe[TT];
-#pragma unroll
-          for (int J = 0; J < TT; ++J) {
-            const int e = 16 * J + i16;
-            xe[J] = (e < k) ? out[J][r] + ab : 0.0;
-            sum += xe[J];
-          }
-#pragma unroll
-          for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
-          const double mean = sum / k;
-          if (st_i >= 0) {
-            const long long o = off + st_i;
-#pragma unroll
-            for (int J = 0; J < TT; ++J) {
-              const int e = 16 * J + i16;
-              if (e < k) p.xa[o * k + e] = xe[J] - mean;
-            }
-            if (i16 == 0) p.axa[o] = mean;
-          }
-        }
-      }
-    }
-  }
-}
 
 // Sum over the NW wavefronts of a workgroup; every thread gets the same value
 // (wave sums, then the partials in a fixed order).  `red` holds two slots of
@@ -610,13 +285,26 @@ __device__ __forceinline__ double block2_min(double v, double* red, int& slot) {
 // analysis' reflector block (consumed by then).  k_update streams the state
 // vectors of the analysis through that operand.
 // ---------------------------------------------------------------------------
+// The rotations of positions 8B .. 8B+7 of a chase: the log holds whole
+// blocks of 8 positions (identity outside the chase), position i at ring
+// entry base - i, so a block is eight consecutive entries read from one
+// address (the ring keeps a copy of its first 8 entries past its end, so a
+// run never wraps).  base is wavefront-uniform.
+template <int RING>
+__device__ __forceinline__ void ring_block(const double2* sring, int base, int B,
+                                           double2 (&cs)[8]) {
+  const double2* e = sring + ((base - 8 * B - 7) & (RING - 1));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) cs[t] = e[7 - t];
+}
+
 template <int KMAX>
 struct VecLds {
   static constexpr int LDV = KMAX + 1;
   static constexpr int RING = 8 * KMAX;              // rotation ring (entries)
   static constexpr int REFILL = RING / 2;
   static constexpr int A = 16 * LDV;                 // 16 reflectors
-  static constexpr int B = 2 * RING + 2 * KMAX;      // ring + dense chase table
+  static constexpr int B = 2 * (RING + 8);          // rotation ring + its mirrored head
   static constexpr int C = 16 * LDV;                 // 16 columns of V
   static constexpr int N = (A > B ? (A > C ? A : C) : (B > C ? B : C));
 };
@@ -635,7 +323,7 @@ void k_vec(EigParams p) {
   constexpr int IPP = TT < 2 ? TT : 2;               // tile rows per pass
   using L = VecLds<KMAX>;
   constexpr int LDV = L::LDV, RING = L::RING, REFILL = L::REFILL;
-  static_assert(REFILL >= KMAX && REFILL % 64 == 0, "a refill covers any chase");
+  static_assert(REFILL >= KMAX + 16 && REFILL % 64 == 0, "a refill covers any chase");
   __shared__ __align__(16) double sbuf[L::N];
   __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
   const int lane = threadIdx.x;
@@ -695,17 +383,16 @@ void k_vec(EigParams p) {
   }
   __syncthreads();
 
-  // ---- B. logged QL rotations: a ring of REFILL-entry refills, dense chase tables ----
+  // ---- B. logged QL rotations streamed through an LDS ring of REFILL-entry refills ----
   {
     double2* sring = reinterpret_cast<double2*>(sbuf);
-    double2* stab = reinterpret_cast<double2*>(sbuf + 2 * RING);
     const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
     const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
     const int nch = uniform(min(p.nchase[al], p.hdr_cap));
     int nrot = 0;
     if (nch > 0) {
       const int4 hl = hdr[nch - 1];
-      nrot = uniform(hl.z + (hl.y - hl.x));
+      nrot = uniform(hl.z + hl.w);
     }
     int committed = 0;
     int4 hn = nch > 0 ? hdr[0] : make_int4(0, 0, 0, 0);
@@ -714,29 +401,28 @@ void k_vec(EigParams p) {
       h.x = uniform(h.x);
       h.y = uniform(h.y);
       h.z = uniform(h.z);
+      h.w = uniform(h.w);
       if (ch + 1 < nch) hn = hdr[ch + 1];
       const int lo = h.x, hi = h.y;
-      if (h.z + (hi - lo) > committed) {
+      if (h.z + h.w > committed) {
         double2 v[REFILL / 64];
 #pragma unroll
         for (int t = 0; t < REFILL / 64; ++t) {
           const int e = committed + 64 * t + lane;
           v[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
         }
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < REFILL / 64; ++t)
-          sring[(committed + 64 * t + lane) & (RING - 1)] = v[t];
+        for (int t = 0; t < REFILL / 64; ++t) {
+          const int slot = (committed + 64 * t + lane) & (RING - 1);
+          sring[slot] = v[t];
+          if (slot < 8) sring[RING + slot] = v[t];
+        }
         committed += REFILL;
         __syncthreads();
       }
       const int bhi = (hi - 1) >> 3, blo = lo >> 3;
-      const int base = h.z + hi - 1;
-      if (lane < KMAX) {
-        const bool in = lane >= lo && lane < hi;
-        const double2 g = sring[(base - lane) & (RING - 1)];
-        stab[lane] = in ? g : make_double2(1.0, 0.0);
-      }
-      __syncthreads();
+      const int base = h.z + ((hi + 7) & ~7) - 1;
       double carry = 0.0;
 #pragma unroll
       for (int B = NB - 1; B >= 0; --B) {
@@ -744,8 +430,7 @@ void k_vec(EigParams p) {
         const int top = (8 * B + 7 < KMAX - 1) ? 8 * B + 7 : KMAX - 2;
         if (B == bhi) carry = z[top + 1];
         double2 cs[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) cs[t] = stab[8 * B + t];
+        ring_block<RING>(sring, base, B, cs);
 #pragma unroll
         for (int t = 7; t >= 0; --t) {
           const int i = 8 * B + t;
@@ -757,7 +442,6 @@ void k_vec(EigParams p) {
         }
         if (B == blo) z[8 * B] = carry;
       }
-      __syncthreads();
     }
   }
 
@@ -853,123 +537,6 @@ void k_vec(EigParams p) {
     }
   }
   if (p.w_out && in_r) p.w_out[a * (long long)k + lane] = wbar;
-}
-
-template <int KMAX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-void k_update_reg(EigParams p) {
-  constexpr int TT = KMAX / 16;
-  constexpr int LDV = KMAX + 1;
-  __shared__ double sV[16 * LDV];
-  const int lane = threadIdx.x;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  if (p.status[a] != 0) return;
-  const int k = p.k, kp = p.kp;
-  const bool in_r = lane < k;
-  const int i16 = lane & 15, q = lane >> 4;
-  const double* __restrict__ tb = p.refl + al * (long long)kp * kp;
-  d4 acc[TT][TT];
-#pragma unroll
-  for (int I = 0; I < TT; ++I)
-#pragma unroll
-    for (int J = 0; J < TT; ++J)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[I][J][r] = tb[tile_off<KMAX>(I, J, r, lane)];
-
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
-  const long long lat_ext = p.g.nlat + 2 * p.halo;
-  int sk = 1, ek = p.nz;
-  if (p.vertical) {
-    sk = kc * p.cube_nz + 1;
-    ek = min(sk + p.cube_nz - 1, p.nz);
-  }
-  int rs[kMaxVars], rl[kMaxVars];
-  int nrun = 0, nvec = 0;
-  if (!p.vertical) {
-    rs[0] = 0; rl[0] = p.n; nrun = 1;
-  } else {
-    for (int vv = 0; vv < p.n_var; ++vv) {
-      const int start = p.var_start[vv] - 1;
-      if (p.var_lev[vv] == 1) {
-        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
-      } else {
-        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
-      }
-    }
-  }
-  for (int t = 0; t < nrun; ++t) nvec += rl[t];
-  auto sidx = [&](int v) {
-    int rem = v;
-    for (int t = 0; t < nrun; ++t) {
-      if (rem < rl[t]) return rs[t] + rem;
-      rem -= rl[t];
-    }
-    return -1;
-  };
-  // two wavefronts per SIMD hide the loads of one group behind the other
-  // wavefront's MFMAs, so there is no register prefetch here
-  for (int lo = si; lo <= ei; ++lo) {
-    for (int la = sj; la <= ej; ++la) {
-      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
-      for (int v0 = 0; v0 < nvec; v0 += 16) {
-        __syncthreads();
-        for (int u = 0; u < 16; ++u) {
-          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          if (lane < KMAX) sV[u * LDV + lane] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + lane] : 0.0;
-        }
-        double ab4[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int v = v0 + q + 4 * r;
-          const int s_i = (v < nvec) ? sidx(v) : -1;
-          ab4[r] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
-        }
-        __syncthreads();
-        d4 out[TT];
-#pragma unroll
-        for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int sK = 0; sK < KMAX / 4; ++sK) {
-          const double xa_frag = sV[i16 * LDV + 4 * sK + q];
-#pragma unroll
-          for (int J = 0; J < TT; ++J)
-            out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa_frag, acc[sK >> 2][J][sK & 3],
-                                                          out[J], 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int v = v0 + q + 4 * r;
-          const int st_i = (v < nvec) ? sidx(v) : -1;
-          const double ab = ab4[r];
-          double sum = 0.0;
-          double xe[TT];
-#pragma unroll
-          for (int J = 0; J < TT; ++J) {
-            const int e = 16 * J + i16;
-            xe[J] = (e < k) ? out[J][r] + ab : 0.0;
-            sum += xe[J];
-          }
-#pragma unroll
-          for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
-          const double mean = sum / k;
-          if (st_i >= 0) {
-            const long long o = off + st_i;
-#pragma unroll
-            for (int J = 0; J < TT; ++J) {
-              const int e = 16 * J + i16;
-              if (e < k) p.xa[o * k + e] = xe[J] - mean;
-            }
-            if (i16 == 0) p.axa[o] = mean;
-          }
-        }
-      }
-    }
-  }
 }
 
 // k_update: xa(:, v) = trans^T xb(:, v) + axb(v), then re-centred, for the
@@ -1290,12 +857,14 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
   tg[r] = tv;
 }
 
-// Applies the positions il in [lo_l, hi_l) of a chase (dense table `tab`,
-// identity outside) to a half row z[0..63]; z[64] is `ztop` (in/out).  The
-// carry enters at the top block as z[top+1] and leaves as the return value;
-// with `final_write`, z[lo_l] takes it (the chase ends in this half).
+// Applies the positions il in [lo_l, hi_l) of a chase to a half row
+// z[0..63] (rotation of local position il at ring entry base_l - il, identity
+// outside); z[64] is `ztop` (in/out).  The carry enters at the top block as
+// z[top+1] and leaves as the return value; with `final_write`, z[lo_l] takes
+// it (the chase ends in this half).
+template <int RING>
 __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int lo_l, int hi_l,
-                                             const double2* __restrict__ tab, bool final_write) {
+                                             const double2* sring, int base_l, bool final_write) {
   const int bhi = (hi_l - 1) >> 3, blo = lo_l >> 3;
   double carry = 0.0;
 #pragma unroll
@@ -1303,8 +872,7 @@ __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int 
     if (B > bhi || B < blo) continue;
     if (B == bhi) carry = (B == 7) ? ztop : z[8 * B + 8];
     double2 cs[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) cs[t] = tab[8 * B + t];
+    ring_block<RING>(sring, base_l, B, cs);
 #pragma unroll
     for (int t = 7; t >= 0; --t) {
       const int i = 8 * B + t;
@@ -1322,7 +890,7 @@ __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int 
 __global__ __launch_bounds__(256) void k_vec128(EigParams p) {
   constexpr int KMAX = 128, LDV = KMAX + 1, NT = 256;
   constexpr int RING = 1024, REFILL = 512;
-  __shared__ __align__(16) double sbuf[16 * LDV > 2 * RING + 2 * KMAX ? 16 * LDV : 2 * RING + 2 * KMAX];
+  __shared__ __align__(16) double sbuf[16 * LDV > 2 * (RING + 8) ? 16 * LDV : 2 * (RING + 8)];
   __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
   __shared__ double sx[2][2][KMAX];      // partial dots [parity][half][row]
   __shared__ double sc[KMAX], sz64[KMAX];
@@ -1333,7 +901,7 @@ __global__ __launch_bounds__(256) void k_vec128(EigParams p) {
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
   const long long al = a - p.a0;
-  if (p.status[a] != 0) return;
+  // flagged analyses are applied too (inexact basis; the flag is returned)
   const int k = p.k, kp = p.kp;
   const bool in_r = r < k;
   double* __restrict__ refl = p.refl + al * (long long)kp * kp;
@@ -1375,54 +943,54 @@ __global__ __launch_bounds__(256) void k_vec128(EigParams p) {
   // ---- B. logged QL rotations ----
   {
     double2* sring = reinterpret_cast<double2*>(sbuf);
-    double2* stab = reinterpret_cast<double2*>(sbuf + 2 * RING);
     const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
     const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
     const int nch = uniform(min(p.nchase[al], p.hdr_cap));
     int nrot = 0;
     if (nch > 0) {
       const int4 hl = hdr[nch - 1];
-      nrot = uniform(hl.z + (hl.y - hl.x));
+      nrot = uniform(hl.z + hl.w);
     }
     int committed = 0;
     double ztop = 0.0;
     for (int ch = 0; ch < nch; ++ch) {
       int4 hh = hdr[ch];
-      const int lo = uniform(hh.x), hi = uniform(hh.y), off = uniform(hh.z);
-      if (off + (hi - lo) > committed) {
+      const int lo = uniform(hh.x), hi = uniform(hh.y), off = uniform(hh.z), len = uniform(hh.w);
+      // refills are the only block-wide barriers outside crossing chases: the
+      // slots a refill overwrites may still be read by a chase that started
+      // below the previous refill's entries, hence the leading barrier
+      if (off + len > committed) {
         __syncthreads();
         for (int e = tid; e < REFILL; e += NT) {
           const int g = committed + e;
-          sring[g & (RING - 1)] = (g < nrot) ? rot[g] : make_double2(1.0, 0.0);
+          const double2 v = (g < nrot) ? rot[g] : make_double2(1.0, 0.0);
+          const int slot = g & (RING - 1);
+          sring[slot] = v;
+          if (slot < 8) sring[RING + slot] = v;
         }
         committed += REFILL;
+        __syncthreads();
       }
-      __syncthreads();
-      const int base = off + hi - 1;
-      if (tid < KMAX) {
-        const bool in = tid >= lo && tid < hi;
-        const double2 g = sring[(base - tid) & (RING - 1)];
-        stab[tid] = in ? g : make_double2(1.0, 0.0);
-      }
-      __syncthreads();
+      const int base = off + ((hi + 7) & ~7) - 1;
       const bool cross = lo < 64 && hi >= 64;
       if (!cross) {
-        // the whole chase is in one half
+        // the whole chase is in one half: the other half's wavefronts go on
         if ((hi <= 63 && h == 0) || (lo >= 64 && h == 1)) {
           ztop = 0.0;
-          (void)chase_half(z, ztop, lo - 64 * h, hi - 64 * h, stab + 64 * h, true);
+          (void)chase_half<RING>(z, ztop, lo - 64 * h, hi - 64 * h, sring, base - 64 * h, true);
         }
       } else {
         if (h == 1) {
           // hi == 64: no position in this half, the carry is column 64 itself
           ztop = 0.0;
-          const double cy = (hi > 64) ? chase_half(z, ztop, 0, hi - 64, stab + 64, false) : z[0];
+          const double cy =
+              (hi > 64) ? chase_half<RING>(z, ztop, 0, hi - 64, sring, base - 64, false) : z[0];
           sc[r] = cy;
         }
         __syncthreads();
         if (h == 0) {
           ztop = sc[r];
-          (void)chase_half(z, ztop, lo, 64, stab, true);
+          (void)chase_half<RING>(z, ztop, lo, 64, sring, base, true);
           sz64[r] = ztop;
         }
         __syncthreads();
@@ -1550,7 +1118,6 @@ __global__ __launch_bounds__(256) void k_update128(EigParams p) {
   const long long a = p.a0 + blockIdx.x;
   if (a >= p.a0 + p.n_an) return;
   const long long al = a - p.a0;
-  if (p.status[a] != 0) return;
   const int k = p.k, kp = p.kp;
   const int i16 = lane & 15, q = lane >> 4;
   const double* __restrict__ tb = p.refl + al * (long long)kp * kp;
@@ -1737,213 +1304,6 @@ __global__ __launch_bounds__(KMAX < 64 ? 64 : KMAX) void k_tridiag_lds(EigParams
   tg[r] = tv;
 }
 
-// Q = H_0 ... H_{k-3} and the logged rotations applied to it in LDS, then
-// overline_w, trans and the ensemble update (as k_apply, with the products
-// regrouped: T x = V diag(f) (V^T x) and overline_w . x = st . (V^T x)).
-// Every analysis is applied, flagged ones included: a rotation log that
-// overflowed or a recurrence that did not converge leaves an inexact basis,
-// and the flag is returned to the caller.
-template <int KMAX>
-__global__ __launch_bounds__(KMAX) void k_apply_lds(EigParams p) {
-  static_assert(KMAX == 128, "two wavefronts per matrix");
-  constexpr int LD = KMAX + 1;
-  constexpr int NV = 8;                              // state vectors per update step
-  __shared__ double sZ[KMAX * LD];                   // Z[r][c] at sZ[c * LD + r]
-  __shared__ __align__(16) double sbuf[2 * NV * KMAX];   // phases A/B: sq, scs; D: sx, sy
-  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX];
-  __shared__ double red[4 * NV];
-  const int r = threadIdx.x;
-  const int lane = r & 63, wv = r >> 6;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  const int k = p.k, kp = p.kp;
-  const bool in_r = r < k;
-  double* sq = sbuf;                                  // [2][KMAX]
-  double2* scs = reinterpret_cast<double2*>(sbuf + 2 * KMAX);   // [2][KMAX]
-
-  // ---- A. Z <- Z H_j, row r per thread ----
-  for (int c = 0; c < KMAX; ++c) sZ[c * LD + r] = (c == r) ? 1.0 : 0.0;
-  const double* __restrict__ refl = p.refl + al * (long long)kp * kp;
-  const double* __restrict__ tg = p.tau + al * kp;
-  for (int j = 0; j + 2 < k; ++j) {
-    const int buf = j & 1;
-    sq[buf * KMAX + r] = refl[(long long)j * kp + r];
-    const double tau = uniform(tg[j]);
-    __syncthreads();
-    if (tau == 0.0) continue;
-    const double* q = sq + buf * KMAX;
-    double d0 = 0.0, d1 = 0.0;
-    int c = j + 1;
-    for (; c + 1 < k; c += 2) {
-      d0 = fma(sZ[c * LD + r], q[c], d0);
-      d1 = fma(sZ[(c + 1) * LD + r], q[c + 1], d1);
-    }
-    if (c < k) d0 = fma(sZ[c * LD + r], q[c], d0);
-    const double dot = (d0 + d1) * tau;
-    for (c = j + 1; c < k; ++c) sZ[c * LD + r] = fma(-dot, q[c], sZ[c * LD + r]);
-  }
-
-  // ---- B. the logged QL rotations, chase by chase ----
-  const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
-  const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
-  const int nch = p.nchase[al];
-  for (int ch = 0; ch < nch; ++ch) {
-    const int buf = ch & 1;
-    int4 h = hdr[ch];
-    h.x = uniform(h.x);
-    h.y = uniform(h.y);
-    h.z = uniform(h.z);
-    const int lo = h.x, hi = h.y;
-    if (r < hi - lo) scs[buf * KMAX + r] = rot[h.z + r];
-    __syncthreads();
-    const double2* cs = scs + buf * KMAX;
-    double carry = sZ[hi * LD + r];
-    double zi = sZ[(hi - 1) * LD + r];
-    for (int i = hi - 1; i >= lo; --i) {
-      const double zn = (i > lo) ? sZ[(i - 1) * LD + r] : 0.0;
-      const double2 g = cs[hi - 1 - i];
-      sZ[(i + 1) * LD + r] = g.y * zi + g.x * carry;
-      carry = g.x * zi - g.y * carry;
-      zi = zn;
-    }
-    sZ[lo * LD + r] = carry;
-  }
-  __syncthreads();
-
-  // ---- C. eigenvalue functions, overline_w, trans ----
-  int slot = 0;
-  double lam = in_r ? p.lam[al * kp + r] : 0.0;
-  if (p.guard) {
-    const double mn = block2_min(in_r ? lam : 1e300, red, slot);
-    if (mn < 0.0) lam = lam - mn;
-  }
-  lam = lam * 1.0 + (double)(k - 1) / p.inflation;
-  sf[r] = in_r ? sqrt((double)(k - 1) / lam) : 0.0;
-  sg[r] = in_r ? 1.0 / lam : 0.0;
-  sb[r] = in_r ? p.b[al * kp + r] : 0.0;
-  __syncthreads();
-  {
-    // st_c = (V^T b)_c / lambda_c, thread r computes column c = r
-    double tb = 0.0;
-    for (int i = 0; i < k; ++i) tb = fma(sZ[r * LD + i], sb[i], tb);
-    st[r] = tb * sg[r];
-  }
-  __syncthreads();
-  double wbar = 0.0;
-  for (int c = 0; c < k; ++c) wbar = fma(sZ[c * LD + r], st[c], wbar);
-  sw[r] = in_r ? wbar : 0.0;
-  if (p.w_out && in_r) p.w_out[a * (long long)k + r] = wbar;
-  if (p.trans_out) {
-    // trans(f, e) = sum_c V[f][c] f_c V[e][c] + overline_w(f); thread f = r
-    double* tr = p.trans_out + a * (long long)k * k;
-    for (int e = 0; e < k; ++e) {
-      double t = 0.0;
-      for (int c = 0; c < k; ++c) t = fma(sZ[c * LD + r] * sf[c], sZ[c * LD + e], t);
-      if (in_r) tr[(long long)e * k + r] = t + wbar;
-    }
-  }
-  if (!p.xb) return;
-
-  // ---- D. ensemble update ----
-  double* sx = sbuf;                 // [NV][KMAX] background members
-  double* sy = sbuf + NV * KMAX;     // [NV][KMAX] (V^T x)_c
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
-  const long long lat_ext = p.g.nlat + 2 * p.halo;
-  int sk = 1, ek = p.nz;
-  if (p.vertical) {
-    sk = kc * p.cube_nz + 1;
-    ek = min(sk + p.cube_nz - 1, p.nz);
-  }
-  int rs[kMaxVars], rl[kMaxVars];
-  int nrun = 0, nvec = 0;
-  if (!p.vertical) {
-    rs[0] = 0; rl[0] = p.n; nrun = 1;
-  } else {
-    for (int vv = 0; vv < p.n_var; ++vv) {
-      const int start = p.var_start[vv] - 1;
-      if (p.var_lev[vv] == 1) {
-        if (ek == p.nz) { rs[nrun] = start; rl[nrun] = 1; ++nrun; }
-      } else {
-        rs[nrun] = start + sk - 1; rl[nrun] = ek - sk + 1; ++nrun;
-      }
-    }
-  }
-  for (int t = 0; t < nrun; ++t) nvec += rl[t];
-  auto sidx = [&](int v) {
-    int rem = v;
-    for (int t = 0; t < nrun; ++t) {
-      if (rem < rl[t]) return rs[t] + rem;
-      rem -= rl[t];
-    }
-    return -1;
-  };
-  for (int lo = si; lo <= ei; ++lo) {
-    for (int la = sj; la <= ej; ++la) {
-      const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
-      for (int v0 = 0; v0 < nvec; v0 += NV) {
-        __syncthreads();
-        for (int u = 0; u < NV; ++u) {
-          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          sx[u * KMAX + r] = (s_i >= 0 && in_r) ? p.xb[(off + s_i) * k + r] : 0.0;
-        }
-        __syncthreads();
-        // y_u,c = (V^T x_u)_c, thread r computes column c = r
-        double y[NV];
-#pragma unroll
-        for (int u = 0; u < NV; ++u) y[u] = 0.0;
-        for (int i = 0; i < k; ++i) {
-          const double z = sZ[r * LD + i];
-#pragma unroll
-          for (int u = 0; u < NV; ++u) y[u] = fma(z, sx[u * KMAX + i], y[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < NV; ++u) sy[u * KMAX + r] = y[u];
-        __syncthreads();
-        // xa_u,e = sum_c V[e][c] f_c y_u,c + sum_c st_c y_u,c + axb_u
-        double o[NV], wd[NV];
-#pragma unroll
-        for (int u = 0; u < NV; ++u) { o[u] = 0.0; wd[u] = 0.0; }
-        for (int c = 0; c < k; ++c) {
-          const double zf = sZ[c * LD + r] * sf[c];
-          const double tc = st[c];
-#pragma unroll
-          for (int u = 0; u < NV; ++u) {
-            const double yc = sy[u * KMAX + c];
-            o[u] = fma(zf, yc, o[u]);
-            wd[u] = fma(tc, yc, wd[u]);
-          }
-        }
-        double xe[NV], part[NV];
-#pragma unroll
-        for (int u = 0; u < NV; ++u) {
-          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          const double ab = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
-          xe[u] = in_r ? (o[u] + wd[u]) + ab : 0.0;
-          part[u] = wave_sum(xe[u]);
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int u = 0; u < NV; ++u) red[u * 2 + wv] = part[u];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NV; ++u) {
-          const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          if (s_i < 0) continue;
-          const double mean = (red[2 * u] + red[2 * u + 1]) / k;
-          const long long o2 = off + s_i;
-          if (in_r) p.xa[o2 * k + r] = xe[u] - mean;
-          if (r == 0) p.axa[o2] = mean;
-        }
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1960,44 +1320,27 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    if (p.apply_fused) {
-      hipLaunchKernelGGL(k_apply<32>, dim3(nb), dim3(64), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-    } else {
-      hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-      if (p.xb && p.update_reg) hipLaunchKernelGGL(k_update_reg<32>, dim3(nb), dim3(64), 0, st, p);
-      else if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
-    }
+    hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
+    if ((e = mark(3)) != hipSuccess) return e;
+    if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
   } else if (p.k <= 64) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    if (p.apply_fused) {
-      hipLaunchKernelGGL(k_apply<64>, dim3(nb), dim3(64), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-    } else {
-      hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-      if (p.xb && p.update_reg) hipLaunchKernelGGL(k_update_reg<64>, dim3(nb), dim3(64), 0, st, p);
-      else if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
-    }
+    hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
+    if ((e = mark(3)) != hipSuccess) return e;
+    if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
   } else if (p.k <= 128) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
     if ((e = mark(2)) != hipSuccess) return e;
-    if (p.apply_fused) {
-      hipLaunchKernelGGL(k_apply_lds<128>, dim3(nb), dim3(128), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-    } else {
-      hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
-      if ((e = mark(3)) != hipSuccess) return e;
-      if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
-    }
+    hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+    if ((e = mark(3)) != hipSuccess) return e;
+    if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
   } else {
     return hipErrorInvalidValue;
   }

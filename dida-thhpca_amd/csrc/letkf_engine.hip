@@ -125,30 +125,11 @@ int products_band() {
   return v;
 }
 
-// Update variant for k <= 64: DIDA_UPDATE=reg selects the one-wavefront
-// update with trans in registers, anything else the LDS-staged block update.
-int update_variant() {
-  static const int v = [] {
-    const char* e = std::getenv("DIDA_UPDATE");
-    return (e && std::strcmp(e, "reg") == 0) ? 1 : 0;
-  }();
-  return v;
-}
-
-// Eigenvector/update variant for k <= 64: DIDA_APPLY=fused selects the
-// single k_apply kernel, anything else k_vec + k_update.
-int apply_variant() {
-  static const int v = [] {
-    const char* e = std::getenv("DIDA_APPLY");
-    return (e && std::strcmp(e, "fused") == 0) ? 1 : 0;
-  }();
-  return v;
-}
 
 // bytes of split-eigensolver scratch per analysis
 size_t eig_bytes_per_analysis(int k) {
   const int kp = kp_of(k);
-  return sizeof(double) * ((size_t)kp * kp + 4 * kp) + 16 * (size_t)kRotCapPerK2 * k * k +
+  return sizeof(double) * ((size_t)kp * kp + 4 * kp) + 16 * (size_t)rot_cap_of(k) +
          16 * (size_t)kHdrCapPerK * k + sizeof(int);
 }
 
@@ -158,7 +139,7 @@ size_t eig_bytes_per_analysis(int k) {
 int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   const long long n = xp.n_an;
   const int k = xp.k, kp = xp.kp;
-  const int rot_cap = kRotCapPerK2 * k * k, hdr_cap = kHdrCapPerK * k;
+  const int rot_cap = rot_cap_of(k), hdr_cap = kHdrCapPerK * k;
   RC_TRY(s.refl.ensure(sizeof(double) * n * kp * kp));
   RC_TRY(s.tau.ensure(sizeof(double) * n * kp));
   RC_TRY(s.d.ensure(sizeof(double) * n * kp));
@@ -189,8 +170,6 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   // rows (k <= 64) or half rows (k_tridiag128) of A in registers unless
   // DIDA_TRIDIAG=lds asks for the LDS-resident kernel
   ep.tridiag_lds = tridiag_variant() == 1;
-  ep.apply_fused = apply_variant();
-  ep.update_reg = update_variant();
   ep.nchase = s.nchase.as<int>();
   ep.status = xp.status;
   ep.nzc = xp.nzc;

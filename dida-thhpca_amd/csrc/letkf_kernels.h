@@ -12,7 +12,9 @@ struct dida_letkf_config;
 namespace dida {
 // analysis columns per batch whose scratch fits in `bytes`
 long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes);
-constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis
+constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis, plus
+// the identity entries that align each chase to blocks of 8 positions
+inline int rot_cap_of(int k) { return kRotCapPerK2 * k * k + 16 * k + 64; }
 constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 
 struct Geom {
@@ -133,8 +135,6 @@ struct EigParams {
   int4* hdr;                // batch-local, hdr_cap chases (lo, hi, off) per analysis
   int rot_cap, hdr_cap;
   int tridiag_lds;          // k <= 64: tridiagonalise in LDS (1) or in VGPRs (0)
-  int apply_fused;          // k <= 64: k_apply (1) or k_vec + k_update (0)
-  int update_reg;           // k <= 64: the one-wavefront update with trans in registers
   int* nchase;              // batch-local
   int* status;              // global index
   int nzc, vertical, cube_nz, nz, halo, n, n_var;

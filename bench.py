@@ -72,13 +72,33 @@ def kernel_work(name, stats, cfg, n_an, mixed):
     return fl, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
 
 
-def roofline(km, stats, cfg, n_an, mixed):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "letkf_c2_k64_traffic.json")
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a device-to-device copy of known size), when that
+    summary was taken on this workload; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes")
+
+
+def roofline(km, stats, cfg, n_an, mixed, workload=None):
     cands = [x for x in ("products", "tridiag", "ql", "apply", "update") if km.get(x, 0.0) > 0]
     dom = max(cands, key=lambda x: km[x])
     amount, peak, bound, unit = kernel_work(dom, stats, cfg, n_an, mixed)
     achieved = amount / (km[dom] * 1e-3) / 1e12
     out = dict(bound=bound, kernel=KERNEL_NAMES[dom], achieved=round(achieved, 4), peak=peak,
-               unit=unit, frac=round(achieved / peak, 5), traffic=None,
+               unit=unit, frac=round(achieved / peak, 5),
+               traffic=measured_traffic(KERNEL_NAMES[dom], workload),
                ms_per_launch=round(km[dom], 4),
                kernel_ms={KERNEL_NAMES[x]: round(km[x], 4) for x in cands})
     out["bytes_per_launch" if bound == "hbm" else "flops_per_launch"] = amount
@@ -125,7 +145,9 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
     from dida_letkf.config import PREC_FP64
     head = res[args.mixed]
     st = head["stats"]
-    roof = roofline(head["kernel_ms"], st, cfg, n_a, args.mixed)
+    workload = (f"{args.config}: {cfg.nlon}x{cfg.nlat}x{cfg.nz} grid, n={cfg.n}, k={cfg.k}, "
+                f"{n_obs_rows} clustered obs rows, c={cfg.local_dist:g} km, N_A={n_a} analyses/cycle")
+    roof = roofline(head["kernel_ms"], st, cfg, n_a, args.mixed, workload)
     line = {
         "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
         "value": round(head["value"], 2),
@@ -139,9 +161,7 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
         "vs_baseline": None,
         "dtype": "bf16x3+f64" if args.mixed == 2 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {cfg.nlon}x{cfg.nlat}x{cfg.nz} grid, n={cfg.n}, "
-                               f"k={cfg.k}, {n_obs_rows} clustered obs rows, c={cfg.local_dist:g} km, "
-                               f"N_A={n_a} analyses/cycle",
+        "config": {"workload": workload,
                    "n_analyses": n_a, "k": cfg.k, "obs_rows": n_obs_rows,
                    "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
                    "parallelism": "1 GPU"},
@@ -285,7 +305,9 @@ def main():
     rng = np.random.default_rng(1)
     if args.no_accuracy:
         args.parity_cols = 0
-    cols = np.sort(rng.choice(N_A, min(args.parity_cols, N_A), replace=False)).astype(np.int32)
+    n_cols = cfg.nlat * cfg.nlon        # horizontal columns (cube_n = 1); N_A counts
+    cols = np.sort(rng.choice(n_cols, min(args.parity_cols, n_cols),   # vertical chunks too
+                              replace=False)).astype(np.int32)
     if cols.size:
         ref = orc.letkf_compute(inst, columns=cols, nthreads=min(16, os.cpu_count() or 1))
         ref_full = ref["xa"] + ref["axa"][..., None]
@@ -313,12 +335,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         ncore = min(16, os.cpu_count() or 1)
-        samp = np.arange(0, N_A, args.cpu_sample, dtype=np.int32)
+        samp = np.arange(0, cfg.nlat * cfg.nlon, args.cpu_sample, dtype=np.int32)
         tc = time.perf_counter()
         orc.letkf_compute(inst, columns=samp, nthreads=ncore)
         tcpu = time.perf_counter() - tc
-        cpu = dict(value=float(samp.size / tcpu), unit="local analyses/s", cores=ncore, kind="port",
-                   sample=f"every {args.cpu_sample}th column of the same cycle ({samp.size} analyses, "
+        n_samp = samp.size * (N_A // (cfg.nlat * cfg.nlon))   # analyses per column: vertical chunks
+        cpu = dict(value=float(n_samp / tcpu), unit="local analyses/s", cores=ncore, kind="port",
+                   sample=f"every {args.cpu_sample}th column of the same cycle ({n_samp} analyses, "
                           f"{tcpu:.1f} s wall); brute-force gen_yb_dis over all obs as on one rank, "
                           f"own Householder+QL eigensolver, FP64")
         log(f"cpu baseline: {cpu}")

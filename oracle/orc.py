@@ -223,6 +223,10 @@ def letkf_compute(inst, nthreads: Optional[int] = None, columns: Optional[np.nda
     nobsl = np.full(na, -1, np.int32)
     status = np.zeros(na, np.int32)
     cols = None if columns is None else np.ascontiguousarray(columns, np.int32)
+    if cols is not None and cols.size:
+        nh = (-(-cfg.nlat // cfg.cube_n)) * (-(-cfg.nlon // cfg.cube_n))
+        if int(cols.min()) < 0 or int(cols.max()) >= nh:
+            raise ValueError(f"columns must be horizontal indices in [0, {nh})")
     if nthreads is None:
         nthreads = os.cpu_count() or 1
     rc = L.orc_letkf_compute(C.byref(pb), _p(xb, _dp), _p(axb, _dp), _p(xa, _dp), _p(axa, _dp),

@@ -35,3 +35,18 @@ def test_line_has_contract_keys():
     assert abs(r2["achieved"] - n_a * (16 * k * m + 16 * m) / 3.0e-3 / 1e12) < 1e-3
     assert line["mixed_over_fp64"] == round(1.18e6 / 1.0e6, 3)
     json.dumps(line)
+
+
+def test_traffic_from_committed_profile():
+    """roofline.traffic comes from the committed rocprofv3 --pmc summary, and
+    only for the workload it was measured on."""
+    sys.path.insert(0, ROOT)
+    import bench
+    with open(bench.TRAFFIC_FILE) as f:
+        t = json.load(f)
+    k = t["kernels"]["k_vec"]
+    est = (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"]) * 1024
+    assert abs(k["bytes"] - est) < 1e-3 * est
+    assert bench.measured_traffic("k_vec", t["workload"]) == k["bytes"]
+    assert bench.measured_traffic("k_vec", "another workload") is None
+    assert bench.measured_traffic("no_such_kernel", t["workload"]) is None

@@ -35,8 +35,11 @@ namespace dida {
 // ---------------------------------------------------------------------------
 // k_tridiag
 // ---------------------------------------------------------------------------
+// k = 64 is capped at 168 registers (three wavefronts per SIMD: the kernel is
+// bound by the latency of its per-step reductions and LDS round trips, and a
+// few spilled scalars cost less than the third wavefront gains)
 template <int KMAX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX == 64 ? 3 : 2)))
 void k_tridiag(EigParams p) {
   constexpr int NB = KMAX / 8;
   __shared__ double sv[KMAX], sw[KMAX];

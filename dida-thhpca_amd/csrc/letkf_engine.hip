@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "dida_letkf.h"
+#include "letkf_device.h"
 #include "letkf_kernels.h"
 
 using namespace dida;
@@ -276,7 +277,7 @@ struct dida_letkf_handle {
   int nb_lat = 0, nb_lon = 0;
   double bdeg_lat = 0, bdeg_lon = 0, theta_deg = 0;
   DevBuf key_in, key_out, val_in, val_out, hist, bin_start, sort_tmp;
-  DevBuf s_cos, s_sin, s_lonr, s_row0, s_nrow;
+  DevBuf s_cos, s_sin, s_lonr, s_cosl, s_sinl, s_row0, s_nrow;
   DevBuf yb32, rinv32;
   DevBuf d_hyai, d_hybi;
   DevBuf row_site, nobsl_dev;         // tile-mode selection and row counts
@@ -402,7 +403,8 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
   DevBuf* bufs[] = {&h->glat, &h->glon, &h->o_lat, &h->o_lon, &h->o_rowptr, &h->o_yb64,
                     &h->o_dep, &h->o_rinv64, &h->o_vco, &h->key_in, &h->key_out, &h->val_in,
                     &h->val_out, &h->hist, &h->bin_start, &h->sort_tmp, &h->s_cos, &h->s_sin,
-                    &h->s_lonr, &h->s_row0, &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai,
+                    &h->s_lonr, &h->s_cosl, &h->s_sinl, &h->s_row0, &h->s_nrow, &h->yb32,
+                    &h->rinv32, &h->d_hyai,
                     &h->d_hybi, &h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc,
                     &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
@@ -480,11 +482,14 @@ static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
   RC_TRY(h->s_cos.ensure(sizeof(double) * ns));
   RC_TRY(h->s_sin.ensure(sizeof(double) * ns));
   RC_TRY(h->s_lonr.ensure(sizeof(double) * ns));
+  RC_TRY(h->s_cosl.ensure(sizeof(double) * ns));
+  RC_TRY(h->s_sinl.ensure(sizeof(double) * ns));
   RC_TRY(h->s_row0.ensure(sizeof(int) * ns));
   RC_TRY(h->s_nrow.ensure(sizeof(int) * ns));
   HIP_TRY(launch_site_gather(st, n_sites, h->val_out.as<int>(), lat, lon, row_ptr,
                              h->s_cos.as<double>(), h->s_sin.as<double>(), h->s_lonr.as<double>(),
-                             h->s_row0.as<int>(), h->s_nrow.as<int>()));
+                             h->s_cosl.as<double>(), h->s_sinl.as<double>(), h->s_row0.as<int>(),
+                             h->s_nrow.as<int>()));
   // rows in the storage precision of the mode
   const long long nr = std::max(n_rows, 1);
   RC_TRY(h->rinv32.ensure(sizeof(float) * nr));
@@ -627,11 +632,22 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
   for (long long h0 = 0; h0 < h->n_h; h0 += mb) {
     const long long h1 = std::min(h->n_h, h0 + mb);
     const int ncol = (int)(h1 - h0);
-    SelectParams sp;
+    SelectParams sp{};
     sp.g = g;
     sp.s_cos = h->s_cos.as<double>();
     sp.s_sin = h->s_sin.as<double>();
     sp.s_lonr = h->s_lonr.as<double>();
+    sp.s_cosl = h->s_cosl.as<double>();
+    sp.s_sinl = h->s_sinl.as<double>();
+    {
+      // the support test ri <= 2 in cos(angle): v >= cos(2c / a) (every
+      // point when 2c / a >= pi), with a guard band far wider than the
+      // rounding of either evaluation
+      const double ang = 2.0 * c.local_dist / kEarthF32;
+      const double thr = ang >= M_PI ? -1.0 : std::cos(ang);
+      sp.v_lo = thr - 1e-12;
+      sp.v_hi = thr + 1e-12;
+    }
     sp.s_row0 = h->s_row0.as<int>();
     sp.s_nrow = h->s_nrow.as<int>();
     sp.bin_start = h->bin_start.as<int>();

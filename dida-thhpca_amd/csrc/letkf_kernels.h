@@ -31,9 +31,12 @@ struct SelectParams {
   const double* s_cos;
   const double* s_sin;
   const double* s_lonr;
+  const double* s_cosl;   // cos / sin of s_lonr
+  const double* s_sinl;
   const int* s_row0;
   const int* s_nrow;
   const int* bin_start;   // nb_lat*nb_lon + 1
+  double v_lo, v_hi;      // guard band of the support test in cos(angle)
   int nb_lat, nb_lon;
   double bdeg_lat, bdeg_lon;
   double theta_deg;       // support half-angle incl. margin, degrees
@@ -189,7 +192,8 @@ hipError_t launch_site_keys(hipStream_t st, int n_sites, const double* lat, cons
                             double bdeg_lon, int* key, int* val, int* hist);
 hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, const double* lat,
                               const double* lon, const int* row_ptr, double* s_cos, double* s_sin,
-                              double* s_lonr, int* s_row0, int* s_nrow);
+                              double* s_lonr, double* s_cosl, double* s_sinl, int* s_row0,
+                              int* s_nrow);
 hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);

@@ -52,6 +52,7 @@ __global__ void k_site_gather(int n_sites, const int* __restrict__ order,
                               const double* __restrict__ lat, const double* __restrict__ lon,
                               const int* __restrict__ row_ptr, double* __restrict__ s_cos,
                               double* __restrict__ s_sin, double* __restrict__ s_lonr,
+                              double* __restrict__ s_cosl, double* __restrict__ s_sinl,
                               int* __restrict__ s_row0, int* __restrict__ s_nrow) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_sites) return;
@@ -59,7 +60,10 @@ __global__ void k_site_gather(int n_sites, const int* __restrict__ order,
   double xr = deg2rad_ref(lat[s]);
   s_cos[i] = cos(xr);
   s_sin[i] = sin(xr);
-  s_lonr[i] = deg2rad_ref(lon[s]);
+  const double lr = deg2rad_ref(lon[s]);
+  s_lonr[i] = lr;
+  s_cosl[i] = cos(lr);
+  s_sinl[i] = sin(lr);
   int r0 = row_ptr ? row_ptr[s] : s;
   int r1 = row_ptr ? row_ptr[s + 1] : s + 1;
   s_row0[i] = r0;
@@ -129,11 +133,27 @@ __device__ __forceinline__ void __syncwarp_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A candidate site is inside the support iff its exact latlon_dist_r8 gives
+// ri <= 2 (schr and schr2 are positive up to ri = 2 and zero beyond).  In
+// v = cos(angle) that is v >= cos(2c/a) up to the rounding of the exact
+// evaluation, so a cheap v (cos of the longitude difference expanded with
+// the sites' precomputed cos/sin of longitude, a few ulps from the exact v)
+// decides every candidate outside a guard band [v_lo, v_hi] of 1e-12 around
+// the threshold: below it rejected, above it accepted; inside it, and for
+// every candidate whose GC weight is needed, the exact reference arithmetic
+// runs.  A NaN v falls in neither range and takes the exact path.
+__device__ __forceinline__ double fast_v(const SelectParams& p, int s, double c2, double s2,
+                                         double cl2, double sl2) {
+  return p.s_cos[s] * c2 * (p.s_cosl[s] * cl2 + p.s_sinl[s] * sl2) + p.s_sin[s] * s2;
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   __shared__ int s_pre[4][64], s_r0[4][64];
   __shared__ double s_gc[4][64];
+  __shared__ int s_q[4][128];            // fill pass: queued candidates (FIFO, site order)
   const int lane = threadIdx.x & 63;
+  const int wv = (threadIdx.x >> 6) & 3;
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int ah = p.h0 + wid;
   if (ah >= p.h1) return;
@@ -143,67 +163,106 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   const double clon = (double)p.g.glon[(si + ei) / 2 - 1];
   const double c2 = cos(deg2rad_ref(clat)), s2 = sin(deg2rad_ref(clat));
   const double l2 = deg2rad_ref(clon);
+  const double cl2 = cos(l2), sl2 = sin(l2);
   const BinRange R = bin_range(p, clat, clon, p.theta_deg);
   const int b0 = R.b0, b1 = R.b1, nr = R.nr;
   const int* ja = R.ja;
   const int* jb = R.jb;
   long long count = 0;
   long long base = FILL ? p.off[wid] : 0;
+  auto exact_gc = [&](int s) {
+    const double d = latlon_dist_pre(p.s_cos[s], p.s_sin[s], p.s_lonr[s], c2, s2, l2);
+    return gc_weight(d * 1.0 / p.local_dist, p.gc_filter);
+  };
+  // fill pass: the first n queued candidates (n <= 64, one per lane): exact
+  // weights, then their rows written by all 64 lanes -- entry e belongs to the
+  // lane whose [pre, pre + rc) holds it (binary search over the wavefront's
+  // prefix counts in LDS), so a station column's rows do not serialise on
+  // one lane
+  auto emit = [&](int n) {
+    double gc = 0.0;
+    int rc = 0, r0 = 0;
+    if (lane < n) {
+      const int s = s_q[wv][lane];
+      gc = exact_gc(s);
+      if (gc > 0.0) {
+        rc = p.s_nrow[s];
+        r0 = p.s_row0[s];
+      }
+    }
+    const int pre = wave_excl_scan(rc, lane);
+    const int tot = __shfl(pre + rc, 63, 64);
+    s_pre[wv][lane] = pre;
+    s_r0[wv][lane] = r0;
+    s_gc[wv][lane] = gc;
+    __syncwarp_lds();
+    for (int e = lane; e < tot; e += 64) {
+      int lo = 0, hi = 63;             // last lane with pre <= e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[wv][mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      const long long pos = base + e;
+      const int row = s_r0[wv][lo] + (e - s_pre[wv][lo]);
+      const double g = s_gc[wv][lo];
+      if (pos < p.cap) {
+        p.row_idx[pos] = row;
+        if (p.weights) {
+#pragma clang fp contract(off)
+          const double w = (double)p.rinv32[row] * g;
+          p.row_gc[pos] = w;
+          p.row_wd[pos] = w * p.dep[row];
+        } else {
+          p.row_gc[pos] = g;
+        }
+      }
+    }
+    __syncwarp_lds();
+    base += tot;
+  };
+  int qn = 0;                            // queued candidates (wavefront-uniform)
   for (int b = b0; b <= b1; ++b) {
     for (int r = 0; r < nr; ++r) {
       const int s0 = p.bin_start[b * p.nb_lon + ja[r]];
       const int s1 = p.bin_start[b * p.nb_lon + jb[r] + 1];
       for (int cs = s0; cs < s1; cs += 64) {
         const int s = cs + lane;
-        double gc = 0.0;
-        int rc = 0;
-        if (s < s1) {
-          double d = latlon_dist_pre(p.s_cos[s], p.s_sin[s], p.s_lonr[s], c2, s2, l2);
-          gc = gc_weight(d * 1.0 / p.local_dist, p.gc_filter);
-          if (gc > 0.0) rc = p.s_nrow[s];
-        }
+        const double v = (s < s1) ? fast_v(p, s, c2, s2, cl2, sl2) : -2.0;
+        const bool out = v < p.v_lo;       // sure outside (a NaN is not)
         if (!FILL) {
+          const bool in = v > p.v_hi;
+          int rc = in ? p.s_nrow[s] : 0;
+          const bool band = s < s1 && !out && !in;
+          if (__builtin_amdgcn_ballot_w64(band)) {
+            if (band && exact_gc(s) > 0.0) rc = p.s_nrow[s];
+          }
           count += rc;
         } else {
-          // the chunk's rows are written by all 64 lanes: entry e belongs to
-          // the lane whose [pre, pre + rc) holds it (binary search over the
-          // wavefront's prefix counts in LDS), so a station column's rows do
-          // not serialise on one lane
-          const int pre = wave_excl_scan(rc, lane);
-          const int tot = __shfl(pre + rc, 63, 64);
-          const int wv = (threadIdx.x >> 6) & 3;
-          s_pre[wv][lane] = pre;
-          s_r0[wv][lane] = rc ? p.s_row0[s] : 0;
-          s_gc[wv][lane] = gc;
-          __syncwarp_lds();
-          for (int e = lane; e < tot; e += 64) {
-            int lo = 0, hi = 63;             // last lane with pre <= e
-            while (lo < hi) {
-              const int mid = (lo + hi + 1) >> 1;
-              if (s_pre[wv][mid] <= e) lo = mid;
-              else hi = mid - 1;
-            }
-            const long long pos = base + e;
-            const int row = s_r0[wv][lo] + (e - s_pre[wv][lo]);
-            const double g = s_gc[wv][lo];
-            if (pos < p.cap) {
-              p.row_idx[pos] = row;
-              if (p.weights) {
-#pragma clang fp contract(off)
-                const double w = (double)p.rinv32[row] * g;
-                p.row_gc[pos] = w;
-                p.row_wd[pos] = w * p.dep[row];
-              } else {
-                p.row_gc[pos] = g;
-              }
-            }
+          const bool keep = s < s1 && !out;
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
+          if (keep) {
+            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+            s_q[wv][qn + below] = s;
           }
+          qn += __builtin_popcountll(m);
           __syncwarp_lds();
-          base += tot;
+          if (qn >= 64) {
+            emit(64);
+            // move the remaining qn - 64 (< 64) entries to the queue's head
+            const int rest = qn - 64;
+            const int t = (lane < rest) ? s_q[wv][64 + lane] : 0;
+            __syncwarp_lds();
+            if (lane < rest) s_q[wv][lane] = t;
+            __syncwarp_lds();
+            qn = rest;
+          }
         }
       }
     }
   }
+  if (FILL && qn > 0) emit(qn);
   if (!FILL) {
     count = wave_sum_ll(count);
     if (lane == 0) p.cnt[wid] = count;
@@ -1002,10 +1061,12 @@ hipError_t launch_site_keys(hipStream_t st, int n_sites, const double* lat, cons
 
 hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, const double* lat,
                               const double* lon, const int* row_ptr, double* s_cos, double* s_sin,
-                              double* s_lonr, int* s_row0, int* s_nrow) {
+                              double* s_lonr, double* s_cosl, double* s_sinl, int* s_row0,
+                              int* s_nrow) {
   if (n_sites <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_site_gather, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites,
-                     order, lat, lon, row_ptr, s_cos, s_sin, s_lonr, s_row0, s_nrow);
+                     order, lat, lon, row_ptr, s_cos, s_sin, s_lonr, s_cosl, s_sinl, s_row0,
+                     s_nrow);
   return hipGetLastError();
 }
 

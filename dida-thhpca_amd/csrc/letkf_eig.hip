@@ -180,11 +180,20 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
   for (int l = 0; l < k && status == 0; ++l) {
     int iter = 0;
     for (;;) {
+      // first m >= l with a negligible e[m] (a loop with one exit: no
+      // per-iteration break bookkeeping)
       int m = l;
-      for (; m < k - 1; ++m) {
-        const double dd = fabs(SD(m)) + fabs(SD(m + 1));
+      double dm = SD(l);
+      bool go = m < k - 1;
+      while (go) {
+        const double dm1 = SD(m + 1);
         const double em = fabs(SE(m));
-        if (em <= eps * dd || em <= eps * tnorm) break;
+        go = !(em <= eps * (fabs(dm) + fabs(dm1)) || em <= eps * tnorm);
+        if (go) {
+          ++m;
+          dm = dm1;
+          go = m < k - 1;
+        }
       }
       if (m == l) break;
       if (++iter > 30) { status = l + 1; break; }
@@ -200,40 +209,42 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       for (int q = ((m + 7) & ~7) - 1; q >= m; --q) rot[nrot++] = make_double2(1.0, 0.0);
       int i = m - 1;
       bool underflow = false;
-      // d[i], e[i] of the next step are read before this step's stores
+      int iu = l - 1;
+      // d[i], e[i] of the next step are read before this step's stores; the
+      // body is straight-line (the underflow exit is folded into selects and
+      // the loop bound), so the lanes' divergence costs one exit test per step
       double ei = SE(i), di = SD(i), di1 = SD(i + 1);
       for (; i >= l; --i) {
-        double ein = 0.0, din = 0.0;
-        if (i > l) {
-          ein = SE(i - 1);
-          din = SD(i - 1);
-        }
+        const int im = i > 0 ? i - 1 : 0;
+        const double ein = SE(im), din = SD(im);
         const double f = s * ei, bb = c * ei;
         // one reciprocal square root instead of sqrt and a divide on the
         // serial chain: r = sqrt(f^2 + g^2), s = f / r, c = g / r
         const double r2 = fma(f, f, g * g);
-        if (r2 == 0.0) {
-          SE(i + 1) = 0.0;
-          SD(i + 1) = di1 - pp;
-          SE(m) = 0.0;
-          underflow = true;
-          break;
-        }
-        const double rinv = rsqrt(r2);
-        SE(i + 1) = r2 * rinv;
+        const bool uz = r2 == 0.0;            // underflow: the chase stops here
+        const double rinv = rsqrt(uz ? 1.0 : r2);
+        const double sd_uz = di1 - pp;
         s = f * rinv;
         c = g * rinv;
         g = di1 - pp;
         r = (di - g) * s + 2.0 * c * bb;
         pp = s * r;
-        SD(i + 1) = g + pp;
+        SE(i + 1) = uz ? 0.0 : r2 * rinv;
+        SD(i + 1) = uz ? sd_uz : g + pp;
         g = c * r - bb;
-        rot[nrot++] = make_double2(c, s);
+        rot[nrot] = make_double2(c, s);
+        nrot += uz ? 0 : 1;
+        if (uz) {
+          underflow = true;
+          iu = i;
+          i = l;                               // ends the loop after this step
+        }
         ei = ein;
         di1 = di;
         di = din;
       }
-      const int lo = underflow ? i + 1 : l;
+      if (underflow) SE(m) = 0.0;
+      const int lo = underflow ? iu + 1 : l;
       for (int q = lo - 1; q >= (lo & ~7); --q) rot[nrot++] = make_double2(1.0, 0.0);
       if (m - lo > 0) hdr[nhdr++] = make_int4(lo, m, off, nrot - off);
       else nrot = off;

@@ -149,14 +149,18 @@ void k_tridiag(EigParams p) {
 // ---------------------------------------------------------------------------
 // k_ql_log: one lane per matrix
 // ---------------------------------------------------------------------------
-// d and e of the 64 matrices live in LDS as [i][lane].
+// d and e of the 64 matrices live in LDS as [i][lane], behind one padding
+// row (index -1) so that the sweep's prefetch of i - 1 stays in bounds at
+// i = 0.
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
-  __shared__ double sd_l[KMAX][64];
-  __shared__ double se_l[KMAX][64];
+  __shared__ double sd_l[KMAX + 1][64];
+  __shared__ double se_l[KMAX + 1][64];
   const int lane = threadIdx.x;
-#define SD(i) sd_l[i][lane]
-#define SE(i) se_l[i][lane]
+#define SD(i) sd_l[(i) + 1][lane]
+#define SE(i) se_l[(i) + 1][lane]
+  SD(-1) = 0.0;
+  SE(-1) = 0.0;
   const long long a = p.a0 + (long long)blockIdx.x * 64 + lane;
   const bool valid = a < p.a0 + p.n_an;
   const long long al = a - p.a0;
@@ -210,39 +214,49 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       int i = m - 1;
       bool underflow = false;
       int iu = l - 1;
-      // d[i], e[i] of the next step are read before this step's stores; the
-      // body is straight-line (the underflow exit is folded into selects and
-      // the loop bound), so the lanes' divergence costs one exit test per step
+      // d[i], e[i] of the next step are read before this step's stores.  The
+      // body is straight-line: pointers walk down the lane's LDS column and
+      // its log, and an underflow (r2 == 0, rare) is repaired after the
+      // step's stores under a wavefront-uniform test, then ends the lane's
+      // loop; so the lanes' divergence costs one exit test per step
       double ei = SE(i), di = SD(i), di1 = SD(i + 1);
-      for (; i >= l; --i) {
-        const int im = i > 0 ? i - 1 : 0;
-        const double ein = SE(im), din = SD(im);
+      double* pd = &SD(i);
+      double* pe = &SE(i);
+      double2* pr = rot + nrot;
+      for (; i >= l; --i, pd -= 64, pe -= 64, ++pr) {
+        const double ein = pe[-64], din = pd[-64];
         const double f = s * ei, bb = c * ei;
         // one reciprocal square root instead of sqrt and a divide on the
         // serial chain: r = sqrt(f^2 + g^2), s = f / r, c = g / r
         const double r2 = fma(f, f, g * g);
-        const bool uz = r2 == 0.0;            // underflow: the chase stops here
-        const double rinv = rsqrt(uz ? 1.0 : r2);
+        const double rinv = rsqrt(r2);
         const double sd_uz = di1 - pp;
         s = f * rinv;
         c = g * rinv;
         g = di1 - pp;
         r = (di - g) * s + 2.0 * c * bb;
         pp = s * r;
-        SE(i + 1) = uz ? 0.0 : r2 * rinv;
-        SD(i + 1) = uz ? sd_uz : g + pp;
+        pe[64] = r2 * rinv;
+        pd[64] = g + pp;
         g = c * r - bb;
-        rot[nrot] = make_double2(c, s);
-        nrot += uz ? 0 : 1;
-        if (uz) {
-          underflow = true;
-          iu = i;
-          i = l;                               // ends the loop after this step
+        *pr = make_double2(c, s);
+        if (__builtin_amdgcn_ballot_w64(r2 == 0.0)) {
+          if (r2 == 0.0) {
+            // the chase stops at i: e[i+1] = 0, d[i+1] = d[i+1] - p, no
+            // rotation logged for this step
+            pe[64] = 0.0;
+            pd[64] = sd_uz;
+            underflow = true;
+            iu = i;
+            i = l;
+            --pr;
+          }
         }
         ei = ein;
         di1 = di;
         di = din;
       }
+      nrot = (int)(pr - rot);
       if (underflow) SE(m) = 0.0;
       const int lo = underflow ? iu + 1 : l;
       for (int q = lo - 1; q >= (lo & ~7); --q) rot[nrot++] = make_double2(1.0, 0.0);

@@ -638,15 +638,15 @@ void k_update(EigParams p) {
   const int ncl = ej - sj + 1;
   const int n_items = (ei - si + 1) * ncl * ngrp;
   const bool vec_ld = (k == KMAX) && ((reinterpret_cast<uintptr_t>(p.xb) & 15) == 0);
-  __syncthreads();
-  for (int it = w; it < n_items; it += 4) {
-    const int g = it % ngrp, cl = it / ngrp;
+  auto item_off = [&](int it) {
+    const int cl = it / ngrp;
     const int lo = si + cl / ncl, la = sj + cl % ncl;
-    const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
-    const int v0 = 16 * g;
-    // A operands: variable v0 + i16, members S q .. S q + S - 1
-    double af[S];
-    const int s_a = sidx(v0 + i16);
+    return ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+  };
+  // A operands of item it: variable 16 g + i16, members S q .. S q + S - 1
+  auto load_a = [&](int it, double (&af)[S]) {
+    const long long off = item_off(it);
+    const int s_a = sidx(16 * (it % ngrp) + i16);
     if (s_a >= 0 && vec_ld) {
       const double2* xr = reinterpret_cast<const double2*>(p.xb + (off + s_a) * k + S * q);
 #pragma unroll
@@ -663,6 +663,15 @@ void k_update(EigParams p) {
         af[u] = (s_a >= 0 && m < k) ? xr[m] : 0.0;
       }
     }
+  };
+  // the first item's A operands are read before the staging barrier, each
+  // later item's as soon as the previous item's MFMAs have consumed theirs
+  double af[S];
+  if (w < n_items) load_a(w, af);
+  __syncthreads();
+  for (int it = w; it < n_items; it += 4) {
+    const long long off = item_off(it);
+    const int v0 = 16 * (it % ngrp);
     double ab4[4];
     int st4[4];
 #pragma unroll
@@ -694,6 +703,7 @@ void k_update(EigParams p) {
         out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], b[J], out[J], 0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
     }
+    if (it + 4 < n_items) load_a(it + 4, af);
     // lane (i16, q) holds xa of variable v0 + q + 4 r, member 16 J + i16
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

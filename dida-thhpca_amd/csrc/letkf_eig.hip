@@ -741,8 +741,13 @@ void k_update(EigParams p) {
 // ---------------------------------------------------------------------------
 
 // Householder tridiagonalisation for k = 65..128 with half rows of A in
-// registers (thread t: row t % 128, columns 64h .. 64h+63, h = t / 128, one
-// half per wavefront); same outputs as k_tridiag.  Three barriers per step:
+// registers (thread t: row t % 128, the columns of parity h = t / 128, one
+// half per wavefront): the halves interleave, so both keep the same share of
+// the shrinking trailing matrix, and blocks of columns <= j (where v and w
+// are zero) are skipped.  A column already in tridiagonal form runs the same
+// step with v = 0 and tau = 0 (an exact identity update), so the loop has one
+// path and A stays in the same registers.  Three wavefronts per SIMD.  Same
+// outputs as k_tridiag.  Three barriers per step:
 //  - the pivot column j is in LDS (double-buffered by the parity of j) with
 //    rows <= j zeroed; each wavefront writes v_{j+1} = x0 - alpha over its
 //    entry j+1 before reading it (every wavefront writes the same value), so
@@ -755,7 +760,7 @@ void k_update(EigParams p) {
 //    last barrier.
 // Partial sums are combined in a fixed order, so every thread holds the same
 // values.
-__global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tridiag128(EigParams p) {
   constexpr int KMAX = 128;
   __shared__ double sx[2][KMAX];          // pivot column, rows <= j zeroed
   __shared__ double sx0[2];               // A(j+1, j) of the pivot column
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
     for (int c = 8 * B; c < 8 * B + 8; ++c) {
       // unconditional in-bounds load (r < kp), then a select: no branch
       // per column
-      const int col = 64 * h + c;
+      const int col = 2 * c + h;
       const double av = Ag[(long long)col * kp + r];
       arow[c] = (in_r && col < k) ? av : 0.0;
     }
@@ -799,11 +804,11 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
   // column j+1 -> sx[nxt] (rows <= j+1 zeroed) and sx0[nxt]; partial squares
   // of its rows > j+2 -> rsq[nxt]
   auto pass_column = [&](int j, int nxt) {
-    const int hj1 = (j + 1) >> 6;
+    const int hj1 = (j + 1) & 1;
     if (h == hj1) {
 #pragma unroll
       for (int c = 0; c < 64; ++c) {
-        if (64 * h + c == j + 1) {
+        if (2 * c + h == j + 1) {
           sx[nxt][r] = (r > j + 1) ? arow[c] : 0.0;
           if (r == j + 2) sx0[nxt] = arow[c];
         }
@@ -819,28 +824,24 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
     const bool act = in_r && r > j;
     const double x0 = uniform(sx0[cur]);
     const double rest = (rsq[cur][0] + rsq[cur][1]) + (rsq[cur][2] + rsq[cur][3]);
-    if (rest == 0.0) {
-      // column already in tridiagonal form: pass column j+1 on unchanged
-      pass_column(j, nxt);
-      if (h == 0) refl[(long long)j * kp + r] = 0.0;
-      if (tid == 0) { se_s[j] = x0; st_s[j] = 0.0; }
-      __syncthreads();
-      continue;
-    }
+    // a column already in tridiagonal form (rest = 0) takes the same path
+    // with v = 0 and tau = 0: the update is then exactly the identity
+    const bool none = rest == 0.0;
     const double nrm = sqrt(rest + x0 * x0);
     const double alpha = (x0 >= 0.0) ? -nrm : nrm;
-    const double v0 = x0 - alpha;
-    const double tau = 2.0 / (rest + v0 * v0);
+    const double v0 = none ? 0.0 : x0 - alpha;
+    const double tau = none ? 0.0 : 2.0 / (rest + v0 * v0);
     if (lane == 0) col[j + 1] = v0;
     const double vr = act ? col[r] : 0.0;
     // partial p over this half (v_c = 0 for c <= j)
     double p0 = 0.0, p1 = 0.0;
 #pragma unroll
     for (int B = 0; B < 8; ++B) {
+      if (16 * B + 14 + h <= j) continue;      // v is zero over these columns
 #pragma unroll
       for (int c = 8 * B; c < 8 * B + 8; c += 2) {
-        p0 = fma(arow[c], col[64 * h + c], p0);
-        p1 = fma(arow[c + 1], col[64 * h + c + 1], p1);
+        p0 = fma(arow[c], col[2 * c + h], p0);
+        p1 = fma(arow[c + 1], col[2 * c + 2 + h], p1);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -856,31 +857,30 @@ __global__ __launch_bounds__(256) void k_tridiag128(EigParams p) {
     const double wr = act ? pr - K * vr : 0.0;
     if (h == 0) sw[r] = wr;
     __syncthreads();
-    // A -= v w^T + w v^T over this half
-    if (act) {
+    // A -= v w^T + w v^T over this half (vr = wr = 0 in rows <= j)
 #pragma unroll
-      for (int B = 0; B < 8; ++B) {
+    for (int B = 0; B < 8; ++B) {
+      if (16 * B + 14 + h <= j) continue;      // v and w are zero over these columns
 #pragma unroll
-        for (int c = 8 * B; c < 8 * B + 8; ++c) {
-          const int cg = 64 * h + c;
-          arow[c] -= fma(vr, sw[cg], wr * col[cg]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+      for (int c = 8 * B; c < 8 * B + 8; ++c) {
+        const int cg = 2 * c + h;
+        arow[c] -= fma(vr, sw[cg], wr * col[cg]);
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
     pass_column(j, nxt);
     if (h == 0) refl[(long long)j * kp + r] = vr;
-    if (tid == 0) { se_s[j] = alpha; st_s[j] = tau; }
+    if (tid == 0) { se_s[j] = none ? x0 : alpha; st_s[j] = tau; }
     __syncthreads();
   }
   // diagonal (thread of the half holding column r) and e[k-2] = A[k-1][k-2]
 #pragma unroll
   for (int c = 0; c < 64; ++c)
-    if (64 * h + c == r) sd[r] = arow[c];
+    if (2 * c + h == r) sd[r] = arow[c];
   if (r == k - 1) {
 #pragma unroll
     for (int c = 0; c < 64; ++c)
-      if (64 * h + c == k - 2) se_s[KMAX] = arow[c];
+      if (2 * c + h == k - 2) se_s[KMAX] = arow[c];
   }
   __syncthreads();
   if (h != 0) return;
@@ -925,7 +925,7 @@ __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int 
   return carry;
 }
 
-__global__ __launch_bounds__(256) void k_vec128(EigParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_vec128(EigParams p) {
   constexpr int KMAX = 128, LDV = KMAX + 1, NT = 256;
   constexpr int RING = 1024, REFILL = 512;
   __shared__ __align__(16) double sbuf[16 * LDV > 2 * (RING + 8) ? 16 * LDV : 2 * (RING + 8)];
@@ -1011,29 +1011,34 @@ __global__ __launch_bounds__(256) void k_vec128(EigParams p) {
       }
       const int base = off + ((hi + 7) & ~7) - 1;
       const bool cross = lo < 64 && hi >= 64;
-      if (!cross) {
-        // the whole chase is in one half: the other half's wavefronts go on
-        if ((hi <= 63 && h == 0) || (lo >= 64 && h == 1)) {
-          ztop = 0.0;
-          (void)chase_half<RING>(z, ztop, lo - 64 * h, hi - 64 * h, sring, base - 64 * h, true);
+      // stage 0: a one-half chase, or the upper half of a crossing chase;
+      // stage 1: the lower half of a crossing chase
+      for (int stage = 0; stage < (cross ? 2 : 1); ++stage) {
+        bool act;
+        int lo_l, hi_l;
+        if (!cross) {
+          act = (hi <= 63 && h == 0) || (lo >= 64 && h == 1);
+          lo_l = lo - 64 * h;
+          hi_l = hi - 64 * h;
+        } else if (stage == 0) {
+          act = h == 1;
+          lo_l = 0;
+          hi_l = hi - 64;
+        } else {
+          act = h == 0;
+          lo_l = lo;
+          hi_l = 64;
         }
-      } else {
-        if (h == 1) {
-          // hi == 64: no position in this half, the carry is column 64 itself
-          ztop = 0.0;
-          const double cy =
-              (hi > 64) ? chase_half<RING>(z, ztop, 0, hi - 64, sring, base - 64, false) : z[0];
-          sc[r] = cy;
+        const bool fw = !cross || stage == 1;
+        if (act) {
+          ztop = (cross && stage == 1) ? sc[r] : 0.0;
+          const double cy = chase_half<RING>(z, ztop, lo_l, hi_l, sring, base - 64 * h, fw);
+          if (cross && stage == 0) sc[r] = (hi > 64) ? cy : z[0];
+          if (cross && stage == 1) sz64[r] = ztop;
         }
-        __syncthreads();
-        if (h == 0) {
-          ztop = sc[r];
-          (void)chase_half<RING>(z, ztop, lo, 64, sring, base, true);
-          sz64[r] = ztop;
-        }
-        __syncthreads();
-        if (h == 1) z[0] = sz64[r];
+        if (cross) __syncthreads();
       }
+      if (cross && h == 1) z[0] = sz64[r];
     }
   }
   __syncthreads();

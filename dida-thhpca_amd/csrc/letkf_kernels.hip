@@ -719,8 +719,122 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   }
 }
 
+// k = 65..128 (four wavefronts per analysis): each 32-row group is converted
+// once.  Wavefront WS splits the rows of its T/4 column tiles into BF16 hi/lo
+// (and accumulates their part of b), the block shares the operands through a
+// double-buffered LDS stage with one barrier per group, and each wavefront
+// then runs the MFMAs of its tile pairs from LDS.  Same values and
+// accumulation order as products_bf16_wave.
+template <int KP, int WS, int PPW>
+__device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long long a, int ah,
+                                                     double vc, long long r0, long long r1,
+                                                     int lane, bf8* __restrict__ sh) {
+  constexpr int T = KP / 16;
+  constexpr int TW = T / 4;                  // tiles converted by this wavefront
+  using PT = PairTable<T>;
+  constexpr int P0 = WS * PPW;
+  constexpr int P1 = (P0 + PPW < PT::N) ? P0 + PPW : PT::N;
+  constexpr int NP = P1 - P0;
+  const int i16 = lane & 15, q = lane >> 4;
+  f4 acc[NP];
+#pragma unroll
+  for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
+  double bacc[TW];
+#pragma unroll
+  for (int t = 0; t < TW; ++t) bacc[t] = 0.0;
+  int idx_n[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const long long rr = r0 + 8 * q + u;
+    idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+  }
+  int buf = 0;
+  for (long long base = r0; base < r1; base += 32, buf ^= 1) {
+    bf8 hi[TW], lo[TW];
+    int idx[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long rr = base + 32 + 8 * q + u;
+      idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long long rr = base + 8 * q + u;
+      float s = 0.f;
+      double wd = 0.0;
+      const float* yr = nullptr;
+      if (rr < r1) {
+        const int row = idx[u];
+        double w;
+        if (p.pre_weighted) {
+          w = p.row_gc[rr];
+          wd = p.row_wd[rr];
+        } else {
+          w = row_weight(p, row, p.row_gc[rr], vc);
+          wd = w * p.dep[row];
+        }
+        s = sqrtf((float)w);
+        yr = p.yb32 + (long long)row * p.k;
+      }
+#pragma unroll
+      for (int tt = 0; tt < TW; ++tt) {
+        const int m = 16 * (WS * TW + tt) + i16;
+        const float yv = (yr && m < p.k) ? yr[m] : 0.f;
+        const float yh = s * yv;
+        const __bf16 h = (__bf16)yh;
+        hi[tt][u] = h;
+        lo[tt][u] = (__bf16)(yh - (float)h);
+        bacc[tt] = fma(wd, (double)yv, bacc[tt]);
+      }
+    }
+    bf8* sb = sh + buf * (2 * T * 64);
+#pragma unroll
+    for (int tt = 0; tt < TW; ++tt) {
+      sb[(WS * TW + tt) * 64 + lane] = hi[tt];
+      sb[(T + WS * TW + tt) * 64 + lane] = lo[tt];
+    }
+    __syncthreads();
+    // the buffer written next is the other one; it was last read before this
+    // barrier, so one barrier per group suffices
+    bf8 H[T], L[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      H[t] = sb[t * 64 + lane];
+      L[t] = sb[(T + t) * 64 + lane];
+    }
+#pragma unroll
+    for (int x = 0; x < NP; ++x) {
+      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], H[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], L[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(L[I], H[J], acc[x], 0, 0, 0);
+    }
+  }
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+#pragma unroll
+  for (int x = 0; x < NP; ++x) {
+    const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
+      A[(long long)col * KP + row] = (double)acc[x][r];
+      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
+    }
+  }
+  double* b = p.b + (a - p.a0) * KP;
+#pragma unroll
+  for (int tt = 0; tt < TW; ++tt) {
+    double v = bacc[tt];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (q == 0) b[16 * (WS * TW + tt) + i16] = v;
+  }
+}
+
 template <int KP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP <= 64 ? 3 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -735,11 +849,15 @@ void k_products_bf16(ProdParams p) {
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
   const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-  if (wsub == 0) products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
-  if constexpr (WPA > 1) {
-    if (wsub == 1) products_bf16_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 2) products_bf16_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 3) products_bf16_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+  if constexpr (WPA == 1) {
+    products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+  } else {
+    static_assert(WPA == 4 && KP % 64 == 0, "four wavefronts share an analysis");
+    __shared__ bf8 sh[2 * 2 * (KP / 16) * 64];
+    if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+    if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+    if (wsub == 2) products_bf16_shared<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+    if (wsub == 3) products_bf16_shared<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane, sh);
   }
 }
 

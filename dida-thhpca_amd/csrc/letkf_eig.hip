@@ -737,7 +737,9 @@ void k_update(EigParams p) {
 // 64h .. 64h+63) of row r = t % 128.  A chase that crosses the middle (lo < 64
 // <= hi) runs on the upper half first; its carry goes to the lower half
 // through LDS, and the lower half's step at position 63 returns the new
-// column-64 entry the same way.
+// column-64 entry the same way.  The lower part runs one chase late, beside
+// the next chase's upper part, so both halves work through runs of crossing
+// chases.
 // ---------------------------------------------------------------------------
 
 // Householder tridiagonalisation for k = 65..128 with half rows of A in
@@ -902,7 +904,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // it (the chase ends in this half).
 template <int RING>
 __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int lo_l, int hi_l,
-                                             const double2* sring, int base_l, bool final_write) {
+                                             const double2* sring, int base_l, bool final_write,
+                                             bool top_done) {
   const int bhi = (hi_l - 1) >> 3, blo = lo_l >> 3;
   double carry = 0.0;
 #pragma unroll
@@ -911,6 +914,7 @@ __device__ __forceinline__ double chase_half(double (&z)[64], double& ztop, int 
     if (B == bhi) carry = (B == 7) ? ztop : z[8 * B + 8];
     double2 cs[8];
     ring_block<RING>(sring, base_l, B, cs);
+    if (B == 7 && top_done) cs[7] = make_double2(1.0, 0.0);
 #pragma unroll
     for (int t = 7; t >= 0; --t) {
       const int i = 8 * B + t;
@@ -962,18 +966,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int j = j0; j < j1; ++j) {
       const double tau = uniform(stau[j]);
       const double* q = sbuf + (j - j0) * LDV + 64 * h;
+      // the reflector is zero over columns <= j: blocks of 8 there are skipped
       double d0 = 0.0, d1 = 0.0;
 #pragma unroll
-      for (int c = 0; c < 64; c += 2) {
-        d0 = fma(z[c], q[c], d0);
-        d1 = fma(z[c + 1], q[c + 1], d1);
+      for (int B = 0; B < 8; ++B) {
+        if (64 * h + 8 * B + 7 <= j) continue;
+#pragma unroll
+        for (int c = 8 * B; c < 8 * B + 8; c += 2) {
+          d0 = fma(z[c], q[c], d0);
+          d1 = fma(z[c + 1], q[c + 1], d1);
+        }
       }
       sx[j & 1][h][r] = d0 + d1;
       __syncthreads();
-      if (tau == 0.0) continue;
       const double dot = (sx[j & 1][0][r] + sx[j & 1][1][r]) * tau;
 #pragma unroll
-      for (int c = 0; c < 64; ++c) z[c] = fma(-dot, q[c], z[c]);
+      for (int B = 0; B < 8; ++B) {
+        if (64 * h + 8 * B + 7 <= j) continue;
+#pragma unroll
+        for (int c = 8 * B; c < 8 * B + 8; ++c) z[c] = fma(-dot, q[c], z[c]);
+      }
     }
   }
   __syncthreads();
@@ -991,54 +1003,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     int committed = 0;
     double ztop = 0.0;
-    for (int ch = 0; ch < nch; ++ch) {
-      int4 hh = hdr[ch];
-      const int lo = uniform(hh.x), hi = uniform(hh.y), off = uniform(hh.z), len = uniform(hh.w);
-      // refills are the only block-wide barriers outside crossing chases: the
-      // slots a refill overwrites may still be read by a chase that started
-      // below the previous refill's entries, hence the leading barrier
-      if (off + len > committed) {
-        __syncthreads();
-        for (int e = tid; e < REFILL; e += NT) {
-          const int g = committed + e;
-          const double2 v = (g < nrot) ? rot[g] : make_double2(1.0, 0.0);
-          const int slot = g & (RING - 1);
-          sring[slot] = v;
-          if (slot < 8) sring[RING + slot] = v;
+    // A crossing chase's lower part runs one chase late, beside the next
+    // chase's upper part: its step at position 63 (which needs the upper
+    // part's carry and gives column 64 back) runs alone between two barriers,
+    // the rest of it together with the next chase's piece in the other half.
+    bool pend = false;
+    int plo = 0, pbase = 0;
+    for (int ch = 0; ch <= nch; ++ch) {
+      const bool have = ch < nch;       // ch == nch: the last pending lower part
+      int lo = 0, hi = 0, base = 0;
+      bool cross = false;
+      if (have) {
+        const int4 hh = hdr[ch];
+        lo = uniform(hh.x);
+        hi = uniform(hh.y);
+        const int off = uniform(hh.z), len = uniform(hh.w);
+        // refills are the only block-wide barriers besides the crossing
+        // chases': the slots a refill overwrites may still be read by a chase
+        // that started below the previous refill's entries, hence the leading
+        // barrier (the pending chase is the previous one: a refill, needed
+        // only when this chase ends past the ring's entries, never reaches it)
+        if (off + len > committed) {
+          __syncthreads();
+          for (int e = tid; e < REFILL; e += NT) {
+            const int g = committed + e;
+            const double2 v = (g < nrot) ? rot[g] : make_double2(1.0, 0.0);
+            const int slot = g & (RING - 1);
+            sring[slot] = v;
+            if (slot < 8) sring[RING + slot] = v;
+          }
+          committed += REFILL;
+          __syncthreads();
         }
-        committed += REFILL;
-        __syncthreads();
+        base = off + ((hi + 7) & ~7) - 1;
+        cross = lo < 64 && hi >= 64;
       }
-      const int base = off + ((hi + 7) & ~7) - 1;
-      const bool cross = lo < 64 && hi >= 64;
-      // stage 0: a one-half chase, or the upper half of a crossing chase;
-      // stage 1: the lower half of a crossing chase
-      for (int stage = 0; stage < (cross ? 2 : 1); ++stage) {
-        bool act;
-        int lo_l, hi_l;
-        if (!cross) {
+      if (pend) {
+        // position 63 of the pending chase: carry from the upper part (sc),
+        // new column 64 to the upper half (sz64); z[63] keeps the carry on
+        if (h == 0) {
+          const double cin = sc[r];
+          const double2 cs = sring[(pbase - 63) & (RING - 1)];
+          const double z63 = z[63];
+          sz64[r] = fma(cs.x, cin, cs.y * z63);
+          z[63] = fma(-cs.y, cin, cs.x * z63);
+        }
+        __syncthreads();
+        if (h == 1) z[0] = sz64[r];
+      }
+      // piece 0: the rest of the pending lower part (its position 63 is done);
+      // piece 1: this chase's piece in this half (a crossing chase: the upper
+      // part, whose carry goes to sc)
+      for (int t = 0; t < 2; ++t) {
+        bool act = false, fw = true;
+        int lo_l = 0, hi_l = 0, bl = 0;
+        if (t == 0) {
+          act = pend && h == 0;
+          lo_l = plo;
+          hi_l = 64;
+          bl = pbase;
+        } else if (cross) {
+          act = h == 1 && hi > 64;
+          hi_l = hi - 64;
+          bl = base - 64;
+          fw = false;
+        } else if (have) {
           act = (hi <= 63 && h == 0) || (lo >= 64 && h == 1);
           lo_l = lo - 64 * h;
           hi_l = hi - 64 * h;
-        } else if (stage == 0) {
-          act = h == 1;
-          lo_l = 0;
-          hi_l = hi - 64;
-        } else {
-          act = h == 0;
-          lo_l = lo;
-          hi_l = 64;
+          bl = base - 64 * h;
         }
-        const bool fw = !cross || stage == 1;
         if (act) {
-          ztop = (cross && stage == 1) ? sc[r] : 0.0;
-          const double cy = chase_half<RING>(z, ztop, lo_l, hi_l, sring, base - 64 * h, fw);
-          if (cross && stage == 0) sc[r] = (hi > 64) ? cy : z[0];
-          if (cross && stage == 1) sz64[r] = ztop;
+          ztop = 0.0;
+          const double cy = chase_half<RING>(z, ztop, lo_l, hi_l, sring, bl, fw, t == 0);
+          if (t == 1 && cross) sc[r] = cy;
         }
-        if (cross) __syncthreads();
       }
-      if (cross && h == 1) z[0] = sz64[r];
+      // hi == 64: no position in the upper half, the carry is column 64
+      if (cross && h == 1 && hi == 64) sc[r] = z[0];
+      pend = cross;
+      if (pend) {
+        plo = lo;
+        pbase = base;
+        __syncthreads();
+      }
     }
   }
   __syncthreads();

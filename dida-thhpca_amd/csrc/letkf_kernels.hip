@@ -623,7 +623,29 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
 }
 
 // BF16 hi/lo split: yhat = sqrt(w) y (FP32) = hi + lo with hi = bf16(yhat),
-// lo = bf16(yhat - hi); A ~ hi hi^T + hi lo^T + lo hi^T, FP32 accumulation.
+// lo = bf16(yhat - hi); A ~ hi hi^T + hi lo^T + lo hi^T, FP32 accumulation
+// (b = sum w d y too).
+
+// One 32-row group position of an analysis' row list (rl, with n rows):
+// sqrt(w) and w d in FP32, zero past the end of the list (whose last row is
+// read in its place, so every load stays in bounds without a branch).
+__device__ __forceinline__ void group_row(const ProdParams& p, long long r0, int n, int rl,
+                                          int row, double vc, float& s, float& wdf) {
+  const int rc = rl < n ? rl : n - 1;
+  double w, wd;
+  if (p.pre_weighted) {
+    w = p.row_gc[r0 + rc];
+    wd = p.row_wd[r0 + rc];
+  } else {
+    w = row_weight(p, row, p.row_gc[r0 + rc], vc);
+    wd = w * p.dep[row];
+  }
+  const bool in = rl < n;
+  // w > 0 is a normal number: the hardware square root (1 ulp) suffices
+  s = in ? __builtin_amdgcn_sqrtf((float)w) : 0.f;
+  wdf = in ? (float)wd : 0.f;
+}
+
 template <int KP, int WS, int PPW>
 __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long long a, int ah,
                                                    double vc, long long r0, long long r1,
@@ -637,55 +659,40 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   f4 acc[NP];
 #pragma unroll
   for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
-  double bacc[T];
+  float bacc[T];
 #pragma unroll
-  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
+  for (int t = 0; t < T; ++t) bacc[t] = 0.f;
+  const int n = (int)(r1 - r0);
+  const int* __restrict__ ridx = p.row_idx + r0;
   // the row indices of the next group are loaded one group ahead, so only
-  // the Yb gather of a group waits on memory
+  // the Yb gather of a group waits on memory; positions past the end read
+  // the list's last entry
   int idx_n[8];
+  if (n > 0) {
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const long long rr = r0 + 8 * q + u;
-    idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(8 * q + u, n - 1)];
   }
-  for (long long base = r0; base < r1; base += 32) {
+  for (int g0 = 0; g0 < n; g0 += 32) {
     bf8 hi[T], lo[T];
     int idx[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const long long rr = base + 32 + 8 * q + u;
-      idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
-    }
+    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(g0 + 32 + 8 * q + u, n - 1)];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const long long rr = base + 8 * q + u;
-      float s = 0.f;
-      double wd = 0.0;
-      const float* yr = nullptr;
-      if (rr < r1) {
-        const int row = idx[u];
-        double w;
-        if (p.pre_weighted) {
-          w = p.row_gc[rr];
-          wd = p.row_wd[rr];
-        } else {
-          w = row_weight(p, row, p.row_gc[rr], vc);
-          wd = w * p.dep[row];
-        }
-        s = sqrtf((float)w);
-        yr = p.yb32 + (long long)row * p.k;
-      }
+      float s, wdf;
+      group_row(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
+      const float* yr = p.yb32 + (long long)idx[u] * p.k;
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         const int m = 16 * t + i16;
-        const float yv = (yr && m < p.k) ? yr[m] : 0.f;
+        const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
         const float yh = s * yv;
         const __bf16 h = (__bf16)yh;
         hi[t][u] = h;
         lo[t][u] = (__bf16)(yh - (float)h);
-        if (WS == 0) bacc[t] = fma(wd, (double)yv, bacc[t]);
+        if (WS == 0) bacc[t] = fmaf(wdf, yv, bacc[t]);
       }
     }
 #pragma unroll
@@ -711,10 +718,10 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
     double* b = p.b + (a - p.a0) * KP;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      double v = bacc[t];
+      float v = bacc[t];
       v += __shfl_xor(v, 16, 64);
       v += __shfl_xor(v, 32, 64);
-      if (q == 0) b[16 * t + i16] = v;
+      if (q == 0) b[16 * t + i16] = (double)v;
     }
   }
 }
@@ -739,54 +746,38 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
   f4 acc[NP];
 #pragma unroll
   for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
-  double bacc[TW];
+  float bacc[TW];
 #pragma unroll
-  for (int t = 0; t < TW; ++t) bacc[t] = 0.0;
+  for (int t = 0; t < TW; ++t) bacc[t] = 0.f;
+  const int n = (int)(r1 - r0);
+  const int* __restrict__ ridx = p.row_idx + r0;
   int idx_n[8];
+  if (n > 0) {
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const long long rr = r0 + 8 * q + u;
-    idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
+    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(8 * q + u, n - 1)];
   }
   int buf = 0;
-  for (long long base = r0; base < r1; base += 32, buf ^= 1) {
+  for (int g0 = 0; g0 < n; g0 += 32, buf ^= 1) {
     bf8 hi[TW], lo[TW];
     int idx[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const long long rr = base + 32 + 8 * q + u;
-      idx_n[u] = (rr < r1) ? p.row_idx[rr] : 0;
-    }
+    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(g0 + 32 + 8 * q + u, n - 1)];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const long long rr = base + 8 * q + u;
-      float s = 0.f;
-      double wd = 0.0;
-      const float* yr = nullptr;
-      if (rr < r1) {
-        const int row = idx[u];
-        double w;
-        if (p.pre_weighted) {
-          w = p.row_gc[rr];
-          wd = p.row_wd[rr];
-        } else {
-          w = row_weight(p, row, p.row_gc[rr], vc);
-          wd = w * p.dep[row];
-        }
-        s = sqrtf((float)w);
-        yr = p.yb32 + (long long)row * p.k;
-      }
+      float s, wdf;
+      group_row(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
+      const float* yr = p.yb32 + (long long)idx[u] * p.k;
 #pragma unroll
       for (int tt = 0; tt < TW; ++tt) {
         const int m = 16 * (WS * TW + tt) + i16;
-        const float yv = (yr && m < p.k) ? yr[m] : 0.f;
+        const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
         const float yh = s * yv;
         const __bf16 h = (__bf16)yh;
         hi[tt][u] = h;
         lo[tt][u] = (__bf16)(yh - (float)h);
-        bacc[tt] = fma(wd, (double)yv, bacc[tt]);
+        bacc[tt] = fmaf(wdf, yv, bacc[tt]);
       }
     }
     bf8* sb = sh + buf * (2 * T * 64);
@@ -826,10 +817,10 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
   double* b = p.b + (a - p.a0) * KP;
 #pragma unroll
   for (int tt = 0; tt < TW; ++tt) {
-    double v = bacc[tt];
+    float v = bacc[tt];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
-    if (q == 0) b[16 * (WS * TW + tt) + i16] = v;
+    if (q == 0) b[16 * (WS * TW + tt) + i16] = (double)v;
   }
 }
 

@@ -528,44 +528,41 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
   f4 acc[NP];
 #pragma unroll
   for (int x = 0; x < NP; ++x) acc[x] = f4{0.f, 0.f, 0.f, 0.f};
-  double bacc[T];
+  float bacc[T];
 #pragma unroll
-  for (int t = 0; t < T; ++t) bacc[t] = 0.0;
-  auto load = [&](long long rr, float* y, float& wf, double& wd) {
-    wf = 0.f;
-    wd = 0.0;
-    if (rr < r1) {
-      const int row = p.row_idx[rr];
-      double w;
-      if (p.pre_weighted) {
-        w = p.row_gc[rr];
-        wd = p.row_wd[rr];
-      } else {
-        w = row_weight(p, row, p.row_gc[rr], vc);
-        wd = w * p.dep[row];
-      }
-      wf = (float)w;
-      const float* yr = p.yb32 + (long long)row * p.k;
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const int m = 16 * t + i16;
-        y[t] = (m < p.k) ? yr[m] : 0.f;
-      }
+  for (int t = 0; t < T; ++t) bacc[t] = 0.f;
+  // row rl of the analysis' list; past the end the list's last row is read
+  // (every load in bounds without a branch) with zero weight
+  const int n = (int)(r1 - r0);
+  auto load = [&](int rl, float* y, float& wf, float& wdf) {
+    const int rc = rl < n ? rl : n - 1;
+    const int row = p.row_idx[r0 + rc];
+    double w, wd;
+    if (p.pre_weighted) {
+      w = p.row_gc[r0 + rc];
+      wd = p.row_wd[r0 + rc];
     } else {
+      w = row_weight(p, row, p.row_gc[r0 + rc], vc);
+      wd = w * p.dep[row];
+    }
+    const bool in = rl < n;
+    wf = in ? (float)w : 0.f;
+    wdf = in ? (float)wd : 0.f;
+    const float* yr = p.yb32 + (long long)row * p.k;
 #pragma unroll
-      for (int t = 0; t < T; ++t) y[t] = 0.f;
+    for (int t = 0; t < T; ++t) {
+      const int m = 16 * t + i16;
+      y[t] = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
     }
   };
-  float yn[T], wfn;
-  double wdn;
-  load(r0 + q, yn, wfn, wdn);
-  for (long long base = r0; base < r1; base += 4) {
+  float yn[T], wfn = 0.f, wdn = 0.f;
+  if (n > 0) load(q, yn, wfn, wdn);
+  for (int g0 = 0; g0 < n; g0 += 4) {
     float y[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) y[t] = yn[t];
-    const float wf = wfn;
-    const double wd = wdn;
-    load(base + 4 + q, yn, wfn, wdn);
+    const float wf = wfn, wd = wdn;
+    load(g0 + 4 + q, yn, wfn, wdn);
 #pragma unroll
     for (int x = 0; x < NP; ++x) {
       const int I = PT::I(P0 + x), J = PT::J(P0 + x);
@@ -573,7 +570,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
     }
     if (WS == 0) {
 #pragma unroll
-      for (int t = 0; t < T; ++t) bacc[t] = fma(wd, (double)y[t], bacc[t]);
+      for (int t = 0; t < T; ++t) bacc[t] = fmaf(wd, y[t], bacc[t]);
     }
   }
   double* A = p.A + (a - p.a0) * (long long)KP * KP;

@@ -72,6 +72,33 @@ def test_letkf_core_dropin(k, p):
     assert np.abs(w_g - w_o).max() < TOL64 * max(1.0, np.abs(w_o).max())
 
 
+@pytest.mark.parametrize("k,cuts", [(100, (30, 70)), (128, (40, 90)), (128, (60, 64, 66)),
+                                    (128, (8, 16, 24, 32, 40, 48, 56))])
+def test_letkf_core_block_structured(k, cuts):
+    """Members in groups that share no obs: A = (k-1)/rho I + Y R^-1 Y^T is
+    block diagonal with the given member cuts, so the tridiagonal form has
+    zero subdiagonals there and the QL log mixes chases that stay below
+    column 64, cross it and stay above it (the k = 65..128 kernels' three
+    chase kinds, in every order the cuts produce)."""
+    import orc
+    from dida_letkf import engine
+    rng = np.random.default_rng(k + sum(cuts))
+    edges = [0, *cuts, k]
+    per = 12
+    p = per * (len(edges) - 1)
+    Y = np.zeros((k, p))
+    for g in range(len(edges) - 1):
+        Y[edges[g]:edges[g + 1], per * g:per * (g + 1)] = rng.standard_normal(
+            (edges[g + 1] - edges[g], per))
+    rinv = (1.0 / rng.uniform(0.3, 3.0, p) ** 2).astype(np.float32)
+    dep = rng.standard_normal(p)
+    rloc = rng.uniform(0.0, 1.0, p)
+    t_g, w_g = engine.letkf_core(Y, rinv, dep, rloc, inflation_factor=1.15)
+    t_o, w_o, _ = orc.letkf_core(Y, rinv, dep, rloc, inflation_factor=1.15)
+    assert rel(t_g, t_o) < TOL64, rel(t_g, t_o)
+    assert np.abs(w_g - w_o).max() < TOL64 * max(1.0, np.abs(w_o).max())
+
+
 def test_letkf_core_batched_mixed_sizes():
     import orc
     from dida_letkf import engine

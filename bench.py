@@ -11,7 +11,7 @@ obs-space products, FP64 eigensolve + transform, ensemble update).  The
 mixed-precision form (BF16 hi/lo split products, FP32 accumulation, FP64
 eigensolve) is the headline; the plain FP64 form is timed beside it.
 
-Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: this build runs on
+Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: the engine runs on
 one GPU.
 """
 from __future__ import annotations
@@ -209,8 +209,9 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the path shards by latitude bands, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(local analyses are independent: the path would shard by latitude bands, "
+              f"one band and its obs halo per GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
     from dida_letkf import synth, engine

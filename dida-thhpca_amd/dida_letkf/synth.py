@@ -206,16 +206,16 @@ def _h_weights(lat, lon, lev, var, cfg: LetkfConfig, grid_lat, grid_lon):
 # --------------------------------------------------------------------------
 
 PRESETS = {
-    # plain FP64, CPU-runnable
+    # c0: 72x36, one level and variable, k=20, 2,000 uniform obs (CPU-runnable)
     "c0": dict(nlat=36, nlon=72, nz=1, n2d=1, n3d=0, k=20, n_obs=2000, mix="uniform",
                sigma_obs="c0", radius_km=1000.0, smooth_km=500.0, vert=None),
-    # GMCORE-shaped, vertical localisation in level units
+    # c1: GMCORE-shaped 360x180x32, k=32, vertical localisation in level units
     "c1": dict(nlat=180, nlon=360, nz=32, n2d=1, n3d=3, k=32, n_obs=100_000, mix="uniform3d",
                sigma_obs="tuv", radius_km=800.0, smooth_km=500.0, vert=2.0),
-    # clustered obs, k=64 (the headline workload)
+    # c2: 360x180x32, clustered obs, k=64 (the benchmarked workload)
     "c2": dict(nlat=180, nlon=360, nz=32, n2d=1, n3d=3, k=64, n_obs=500_000, mix="clustered",
                sigma_obs="tuv", radius_km=600.0, smooth_km=500.0, vert=None),
-    # 720x360x32, k=64/128, 2M obs, log-uniform sigma
+    # c3: 720x360x32, k=64/128, 2M clustered obs, log-uniform sigma
     "c3": dict(nlat=360, nlon=720, nz=32, n2d=1, n3d=3, k=64, n_obs=2_000_000, mix="clustered",
                sigma_obs="loguniform", radius_km=600.0, smooth_km=500.0, vert=None),
 }

@@ -119,7 +119,7 @@ def test_letkf_core_batched_mixed_sizes():
 # ---------------------------------------------------------------------------
 
 def test_config0_full_fp64():
-    """72x36, k=20, 2000 obs, GC 1000 km -- the full instance."""
+    """The c0 workload (72x36, k=20, 2000 obs, GC 1000 km) in full."""
     from dida_letkf import synth
     inst = synth.generate("c0", seed=1)
     gpu, ref = _run_both(inst)
@@ -127,7 +127,7 @@ def test_config0_full_fp64():
 
 
 def test_clustered_k64_reduced():
-    """configs[2] shape (clustered obs, 4 vars, k=64) on a reduced grid."""
+    """The c2 shape (clustered obs, 4 vars, k=64) on a reduced grid."""
     from dida_letkf import synth
     inst = synth.generate("c2", seed=3, nlat=24, nlon=48, nz=4, n_obs=20000, local_dist=900.0)
     gpu, ref = _run_both(inst)
@@ -136,7 +136,7 @@ def test_clustered_k64_reduced():
 
 
 def test_vertical_level_units_k32():
-    """configs[1] shape: vertical localisation in level units, per-level analyses."""
+    """The c1 shape: vertical localisation in level units, per-level analyses."""
     from dida_letkf import synth
     inst = synth.generate("c1", seed=5, nlat=12, nlon=24, nz=6, n_obs=3000)
     assert inst.cfg.vertical
@@ -301,7 +301,7 @@ def test_transform_paths_match_oracle(tk):
 # ---------------------------------------------------------------------------
 
 def test_config3_shape_k128_reduced():
-    """configs[3] shape at k=128 (clustered obs, log-uniform sigma) on a reduced grid."""
+    """The c3 shape at k=128 (clustered obs, log-uniform sigma) on a reduced grid."""
     from dida_letkf import synth
     inst = synth.generate("c3", seed=23, k=128, nlat=12, nlon=24, nz=3, n_obs=15000,
                           local_dist=900.0)

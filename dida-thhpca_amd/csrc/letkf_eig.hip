@@ -1439,4 +1439,68 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
   return mark(4);
 }
 
+// analyses [off, off + n) of the batch p: batch-local arrays offset, global
+// ones (status, states, trans_out, w_out) unchanged
+static EigParams eig_slice(const EigParams& p, long long off, long long n) {
+  EigParams q = p;
+  const long long kp = p.kp;
+  q.a0 = p.a0 + off;
+  q.n_an = n;
+  q.A = p.A + off * kp * kp;
+  q.b = p.b + off * kp;
+  q.refl = p.refl + off * kp * kp;
+  q.tau = p.tau + off * kp;
+  q.d = p.d + off * kp;
+  q.e = p.e + off * kp;
+  q.lam = p.lam + off * kp;
+  q.rot = p.rot + off * (long long)p.rot_cap;
+  q.hdr = p.hdr + off * (long long)p.hdr_cap;
+  q.nchase = p.nchase + off;
+  return q;
+}
+
+// one of the four k = 65..128 kernels over the batch p
+static void launch_eig128_stage(hipStream_t st, const EigParams& p, int stage) {
+  const unsigned nb = (unsigned)p.n_an;
+  const unsigned nq = (unsigned)((p.n_an + 63) / 64);
+  if (stage == 0) {
+    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
+    else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
+  } else if (stage == 1) {
+    hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
+  } else if (stage == 2) {
+    hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+  } else if (p.xb) {
+    hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+  }
+}
+
+hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_first,
+                                hipEvent_t e_join, const EigParams& p, hipEvent_t* ev) {
+  if (p.k <= 64 || p.n_an < 2 * kEigPipeMinHalf) return launch_eig(st, p, ev);
+  // the QL recurrence holds a CU's LDS with one wavefront and leaves its other
+  // SIMDs idle; the second half of the batch starts on st2 once the first
+  // half's tridiagonalisation is done, so its kernels run beside the first
+  // half's (tridiagonalisation beside QL, QL beside the eigenvectors)
+  const long long n1 = (p.n_an + 1) / 2;
+  const EigParams pa = eig_slice(p, 0, n1), pb = eig_slice(p, n1, p.n_an - n1);
+  auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+  hipError_t e = mark(0);
+  if (e != hipSuccess) return e;
+  launch_eig128_stage(st, pa, 0);
+  if ((e = hipEventRecord(e_first, st)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(st2, e_first, 0)) != hipSuccess) return e;
+  for (int stage = 0; stage < 4; ++stage) launch_eig128_stage(st2, pb, stage);
+  if ((e = hipEventRecord(e_join, st2)) != hipSuccess) return e;
+  if ((e = mark(1)) != hipSuccess) return e;
+  launch_eig128_stage(st, pa, 1);
+  if ((e = mark(2)) != hipSuccess) return e;
+  launch_eig128_stage(st, pa, 2);
+  if ((e = mark(3)) != hipSuccess) return e;
+  launch_eig128_stage(st, pa, 3);
+  if ((e = hipStreamWaitEvent(st, e_join, 0)) != hipSuccess) return e;
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return mark(4);
+}
+
 }  // namespace dida

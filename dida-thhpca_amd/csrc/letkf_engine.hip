@@ -99,6 +99,20 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 struct EigScratch {
   DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
   std::vector<int> st_host;
+  // second stream and events of the k = 65..128 half-batch pipeline, created
+  // on first use (default priority)
+  hipStream_t st2 = nullptr;
+  hipEvent_t e_first = nullptr, e_join = nullptr;
+  void release_streams() {
+    if (st2) {
+      (void)hipStreamSynchronize(st2);
+      (void)hipStreamDestroy(st2);
+      st2 = nullptr;
+    }
+    if (e_first) (void)hipEventDestroy(e_first);
+    if (e_join) (void)hipEventDestroy(e_join);
+    e_first = e_join = nullptr;
+  }
 };
 
 
@@ -232,7 +246,16 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   }
   EigParams ep;
   RC_TRY(prepare_eig(s, xp, ep));
-  HIP_TRY(launch_eig(st, ep, kev));
+  if (xp.k > 64 && xp.n_an >= 2 * kEigPipeMinHalf) {
+    if (!s.st2) {
+      HIP_TRY(hipStreamCreateWithFlags(&s.st2, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&s.e_first, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&s.e_join, hipEventDisableTiming));
+    }
+    HIP_TRY(launch_eig_pipelined(st, s.st2, s.e_first, s.e_join, ep, kev));
+  } else {
+    HIP_TRY(launch_eig(st, ep, kev));
+  }
   if (split) *split = true;
   return rerun_flagged(st, s, xp);
 }
@@ -400,6 +423,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
 int dida_letkf_destroy(dida_letkf_handle* h) {
   if (!h) return DIDA_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->eig.release_streams();
   DevBuf* bufs[] = {&h->glat, &h->glon, &h->o_lat, &h->o_lon, &h->o_rowptr, &h->o_yb64,
                     &h->o_dep, &h->o_rinv64, &h->o_vco, &h->key_in, &h->key_out, &h->val_in,
                     &h->val_out, &h->hist, &h->bin_start, &h->sort_tmp, &h->s_cos, &h->s_sin,
@@ -1016,10 +1040,13 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
                      &es.list};
   if (rc_t < 0) {
+    (void)hipStreamSynchronize(st);
+    es.release_streams();
     for (DevBuf* b : ebufs) b->release();
     return rc_t;
   }
   HIP_TRY(hipStreamSynchronize(st));
+  es.release_streams();
   for (DevBuf* b : ebufs) b->release();
   HIP_TRY(hipMemcpyAsync(trans, d_t->p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(overline_w, d_w->p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost, st));

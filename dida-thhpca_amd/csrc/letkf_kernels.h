@@ -206,5 +206,13 @@ hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p);
 // halo of a [lon + 2h][lat + 2h][n] field with `inner` values per element
 hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);
 hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev = nullptr);
+// k = 65..128 batches of at least 2 kEigPipeMinHalf analyses: the batch in two
+// halves, the second on st2 (ordered after the first half's
+// tridiagonalisation, joined back into st through e_join); ev as above, with
+// the first half's kernel boundaries (the second half's kernels run beside
+// them).  Smaller batches and k <= 64 take launch_eig on st.
+constexpr long long kEigPipeMinHalf = 1024;
+hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_first,
+                                hipEvent_t e_join, const EigParams& p, hipEvent_t* ev = nullptr);
 
 }  // namespace dida

@@ -166,6 +166,10 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
                    "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
                    "parallelism": "1 GPU"},
         "kernel_ms": head["kernel_ms"],
+        "kernel_ms_note": ("HIP-event time of each kernel over the whole batch, from two extra "
+                           "steps with the eigensolver on one stream"
+                           if "kernel_ms_overlapped" in head else
+                           "HIP-event time of each kernel over the whole batch"),
         "roofline": roof,
         "cpu_baseline": cpu,
         "accuracy": acc,
@@ -200,6 +204,9 @@ def parse():
                     help="jobs of the closed burst that measures saturation (0: = serve-jobs)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy checks")
+    ap.add_argument("--eig-one-stream", action="store_true",
+                    help="time the steps with the eigensolver on one stream (transform_kernel "
+                         "2; the default overlaps two half-batches on two streams)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (the calibration "
                          "kernel of the HBM traffic counters) and exit")
@@ -246,11 +253,11 @@ def main():
              vco=T(o.vcoord))
     stream = torch.cuda.current_stream()
 
-    def make_engine(prec):
+    def make_engine(prec, transform_kernel=0):
         import copy
         c = copy.copy(cfg)
         c.precision = prec
-        eng = engine.LetkfEngine(c)
+        eng = engine.LetkfEngine(c, transform_kernel=transform_kernel)
         eng.set_stream(stream.cuda_stream)
         eng.set_grid(inst.grid_lat, inst.grid_lon)
         return eng
@@ -262,18 +269,20 @@ def main():
         return eng.analyze_device(d["xb"].data_ptr(), d["axb"].data_ptr(), xa.data_ptr(),
                                   axa.data_ptr())
 
-    def run_mode(prec):
-        eng = make_engine(prec)
+    def run_mode(prec, transform_kernel=0, steps=None, warmup=None):
+        steps = args.steps if steps is None else steps
+        warmup = args.warmup if warmup is None else warmup
+        eng = make_engine(prec, transform_kernel)
         xa = torch.zeros_like(d["xb"])
         axa = torch.zeros_like(d["axb"])
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step(eng, xa, axa)
         torch.cuda.synchronize()
         k_ms = dict(select=0.0, products=0.0, transform=0.0, tridiag=0.0, ql=0.0, apply=0.0,
                     update=0.0)
         ts = time.perf_counter()
         flagged = 0
-        for _ in range(args.steps):
+        for _ in range(steps):
             flagged += step(eng, xa, axa)
             st = eng.stats
             k_ms["select"] += st.ms_select
@@ -288,17 +297,25 @@ def main():
         stats = eng.stats.as_dict()
         eng.close()
         for kk in k_ms:
-            k_ms[kk] /= max(args.steps, 1)
-        return dict(elapsed=el, ms_step=1e3 * el / args.steps, value=N_A * args.steps / el,
+            k_ms[kk] /= max(steps, 1)
+        return dict(elapsed=el, ms_step=1e3 * el / steps, value=N_A * steps / el,
                     kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa)
 
+    # timed steps: the eigensolver overlaps two half-batches on two streams
+    # (transform_kernel 0), so its kernels are timed one at a time in a short
+    # extra leg with the eigensolver on one stream (transform_kernel 2)
+    tk = 2 if args.eig_one_stream else 0
     res = {}
     modes = [args.mixed] + ([] if args.no_fp64 else [PREC_FP64])
     for prec in modes:
-        r = run_mode(prec)
+        r = run_mode(prec, tk)
+        if tk == 0:
+            r["kernel_ms_overlapped"] = r["kernel_ms"]
+            r["kernel_ms"] = run_mode(prec, 2, steps=2, warmup=1)["kernel_ms"]
         res[prec] = r
         log(f"precision {prec}: {r['value']:.4g} analyses/s, {r['ms_step']:.2f} ms/step, "
-            f"kernels {r['kernel_ms']}, rows {r['stats']['total_rows']} max {r['stats']['max_rows']}")
+            f"kernels (one stream) {r['kernel_ms']}, rows {r['stats']['total_rows']} "
+            f"max {r['stats']['max_rows']}")
 
     # ---- accuracy: FP64 vs oracle on sampled columns; mixed vs FP64 increments
     import orc

@@ -1459,25 +1459,36 @@ static EigParams eig_slice(const EigParams& p, long long off, long long n) {
   return q;
 }
 
-// one of the four k = 65..128 kernels over the batch p
-static void launch_eig128_stage(hipStream_t st, const EigParams& p, int stage) {
+// one of the four split-eigensolver kernels (tridiagonalisation, QL,
+// eigenvectors + trans, update) over the batch p
+template <int KMAX>
+static void launch_eig_stage_k(hipStream_t st, const EigParams& p, int stage) {
   const unsigned nb = (unsigned)p.n_an;
   const unsigned nq = (unsigned)((p.n_an + 63) / 64);
   if (stage == 0) {
-    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
-    else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
+    if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<KMAX>, dim3(nb), dim3(KMAX < 64 ? 64 : KMAX), 0, st, p);
+    else if constexpr (KMAX == 128) hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL(k_tridiag<KMAX>, dim3(nb), dim3(64), 0, st, p);
   } else if (stage == 1) {
-    hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_ql_log<KMAX>, dim3(nq), dim3(64), 0, st, p);
   } else if (stage == 2) {
-    hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+    if constexpr (KMAX == 128) hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL(k_vec<KMAX>, dim3(nb), dim3(64), 0, st, p);
   } else if (p.xb) {
-    hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+    if constexpr (KMAX == 128) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+    else hipLaunchKernelGGL(k_update<KMAX>, dim3(nb), dim3(256), 0, st, p);
   }
+}
+
+static void launch_eig_stage(hipStream_t st, const EigParams& p, int stage) {
+  if (p.k <= 32) launch_eig_stage_k<32>(st, p, stage);
+  else if (p.k <= 64) launch_eig_stage_k<64>(st, p, stage);
+  else launch_eig_stage_k<128>(st, p, stage);
 }
 
 hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_first,
                                 hipEvent_t e_join, const EigParams& p, hipEvent_t* ev) {
-  if (p.k <= 64 || p.n_an < 2 * kEigPipeMinHalf) return launch_eig(st, p, ev);
+  if (p.k > 128 || p.n_an < 2 * kEigPipeMinHalf) return launch_eig(st, p, ev);
   // the QL recurrence holds a CU's LDS with one wavefront and leaves its other
   // SIMDs idle; the second half of the batch starts on st2 once the first
   // half's tridiagonalisation is done, so its kernels run beside the first
@@ -1487,17 +1498,17 @@ hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_fi
   auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
-  launch_eig128_stage(st, pa, 0);
+  launch_eig_stage(st, pa, 0);
   if ((e = hipEventRecord(e_first, st)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(st2, e_first, 0)) != hipSuccess) return e;
-  for (int stage = 0; stage < 4; ++stage) launch_eig128_stage(st2, pb, stage);
+  for (int stage = 0; stage < 4; ++stage) launch_eig_stage(st2, pb, stage);
   if ((e = hipEventRecord(e_join, st2)) != hipSuccess) return e;
   if ((e = mark(1)) != hipSuccess) return e;
-  launch_eig128_stage(st, pa, 1);
+  launch_eig_stage(st, pa, 1);
   if ((e = mark(2)) != hipSuccess) return e;
-  launch_eig128_stage(st, pa, 2);
+  launch_eig_stage(st, pa, 2);
   if ((e = mark(3)) != hipSuccess) return e;
-  launch_eig128_stage(st, pa, 3);
+  launch_eig_stage(st, pa, 3);
   if ((e = hipStreamWaitEvent(st, e_join, 0)) != hipSuccess) return e;
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return mark(4);

@@ -235,7 +235,9 @@ int rerun_flagged(hipStream_t st, EigScratch& s, const XformParams& xp) {
 
 // Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
 // eigensolver, then the single-kernel transform for analyses the split path
-// flagged; mode 1 runs only the single kernel (k <= 64).
+// flagged; mode 1 runs only the single kernel (k <= 64).  Mode 0 runs batches
+// of 2 kEigPipeMinHalf analyses or more as two half-batches on two streams;
+// mode 2 keeps the split eigensolver on st (each kernel timed by itself).
 // kev: null, or 5 events for the split kernels (recorded only if *split is set)
 int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
                   hipEvent_t* kev = nullptr, bool* split = nullptr) {
@@ -246,7 +248,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   }
   EigParams ep;
   RC_TRY(prepare_eig(s, xp, ep));
-  if (xp.k > 64 && xp.n_an >= 2 * kEigPipeMinHalf) {
+  if (mode == 0 && xp.n_an >= 2 * kEigPipeMinHalf) {
     if (!s.st2) {
       HIP_TRY(hipStreamCreateWithFlags(&s.st2, hipStreamNonBlocking));
       HIP_TRY(hipEventCreateWithFlags(&s.e_first, hipEventDisableTiming));

@@ -90,8 +90,12 @@ typedef struct dida_letkf_config {
     int keep_transforms;    /* keep trans/overline_w of every analysis */
     int max_batch_columns;  /* 0 = automatic */
     int transform_kernel;   /* 0: split eigensolver (tridiagonalisation,
-                               lane-per-matrix QL, rotation apply); 1: the
-                               single-kernel transform */
+                               lane-per-matrix QL, rotation apply), a batch
+                               of 2048 analyses or more as two half-batches
+                               on two streams; 1: the single-kernel
+                               transform (k <= 64); 2: the split eigensolver
+                               on the caller's stream only (its kernels
+                               timed one at a time) */
     int products_kernel;    /* 0: automatic (= 1); 1: one selection list per
                                analysis; 2: tiles of 8 analyses sharing their
                                LDS-staged union of obs rows (horizontal loop,

@@ -140,6 +140,25 @@ def serving_leg(torch, dev, args, precision):
     return out
 
 
+def obs_buckets(nob):
+    """Histogram of the local obs count p per analysis, with each bucket's share
+    of the rows (the products' work).  Analyses are not bucketed on the
+    device: each is one workgroup of a batch-wide launch, dispatched as CUs
+    free up, in a banded visiting order."""
+    nob = np.asarray(nob, np.int64)
+    edges = [0, 1, 256, 512, 1024, 2048, 4096, 1 << 40]
+    names = ["0", "1-255", "256-511", "512-1023", "1024-2047", "2048-4095", ">=4096"]
+    tot = max(int(nob.sum()), 1)
+    out = []
+    for lo, hi, nm in zip(edges[:-1], edges[1:], names):
+        m = (nob >= lo) & (nob < hi)
+        if m.any():
+            out.append({"p": nm, "analyses": int(m.sum()),
+                        "rows_share": round(float(nob[m].sum()) / tot, 4)})
+    return {"p_mean": round(float(nob.mean()), 1), "p_max": int(nob.max()),
+            "buckets": out}
+
+
 def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
     """The one JSON line: the headline (mixed) mode, the FP64 form beside it."""
     from dida_letkf.config import PREC_FP64
@@ -171,6 +190,7 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
                            if "kernel_ms_overlapped" in head else
                            "HIP-event time of each kernel over the whole batch"),
         "roofline": roof,
+        "local_obs": obs_buckets(head["nobsl"]) if head.get("nobsl") is not None else None,
         "cpu_baseline": cpu,
         "accuracy": acc,
     }
@@ -203,7 +223,8 @@ def parse():
     ap.add_argument("--serve-sat-jobs", type=int, default=12,
                     help="jobs of the closed burst that measures saturation (0: = serve-jobs)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
-    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy checks")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the accuracy checks and the local-obs histogram")
     ap.add_argument("--eig-one-stream", action="store_true",
                     help="time the steps with the eigensolver on one stream (transform_kernel "
                          "2; the default overlaps two half-batches on two streams)")
@@ -253,11 +274,11 @@ def main():
              vco=T(o.vcoord))
     stream = torch.cuda.current_stream()
 
-    def make_engine(prec, transform_kernel=0):
+    def make_engine(prec, transform_kernel=0, keep=False):
         import copy
         c = copy.copy(cfg)
         c.precision = prec
-        eng = engine.LetkfEngine(c, transform_kernel=transform_kernel)
+        eng = engine.LetkfEngine(c, transform_kernel=transform_kernel, keep_transforms=keep)
         eng.set_stream(stream.cuda_stream)
         eng.set_grid(inst.grid_lat, inst.grid_lon)
         return eng
@@ -269,10 +290,10 @@ def main():
         return eng.analyze_device(d["xb"].data_ptr(), d["axb"].data_ptr(), xa.data_ptr(),
                                   axa.data_ptr())
 
-    def run_mode(prec, transform_kernel=0, steps=None, warmup=None):
+    def run_mode(prec, transform_kernel=0, steps=None, warmup=None, counts=False):
         steps = args.steps if steps is None else steps
         warmup = args.warmup if warmup is None else warmup
-        eng = make_engine(prec, transform_kernel)
+        eng = make_engine(prec, transform_kernel, keep=counts)
         xa = torch.zeros_like(d["xb"])
         axa = torch.zeros_like(d["axb"])
         for _ in range(warmup):
@@ -295,11 +316,12 @@ def main():
         torch.cuda.synchronize()
         el = time.perf_counter() - ts
         stats = eng.stats.as_dict()
+        nob = eng.local_obs_counts() if counts else None
         eng.close()
         for kk in k_ms:
             k_ms[kk] /= max(steps, 1)
         return dict(elapsed=el, ms_step=1e3 * el / steps, value=N_A * steps / el,
-                    kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa)
+                    kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa, nobsl=nob)
 
     # timed steps: the eigensolver overlaps two half-batches on two streams
     # (transform_kernel 0), so its kernels are timed one at a time in a short
@@ -312,6 +334,10 @@ def main():
         if tk == 0:
             r["kernel_ms_overlapped"] = r["kernel_ms"]
             r["kernel_ms"] = run_mode(prec, 2, steps=2, warmup=1)["kernel_ms"]
+        if prec == args.mixed and not args.no_accuracy:
+            # local obs counts of the cycle (keep_transforms also writes every
+            # trans, so this step is not timed)
+            r["nobsl"] = run_mode(prec, 2, steps=1, warmup=0, counts=True)["nobsl"]
         res[prec] = r
         log(f"precision {prec}: {r['value']:.4g} analyses/s, {r['ms_step']:.2f} ms/step, "
             f"kernels (one stream) {r['kernel_ms']}, rows {r['stats']['total_rows']} "

@@ -327,6 +327,14 @@ class LetkfEngine:
                                                _p(nob, _ip)), "get_transforms")
         return np.transpose(trans, (0, 2, 1)).copy(), w, nob
 
+    def local_obs_counts(self):
+        """Local obs rows of each analysis of the last analyze call (needs
+        keep_transforms)."""
+        nob = np.zeros(self.n_analyses, np.int32)
+        _check(lib().dida_letkf_get_transforms(self.h, self.n_analyses, None, None,
+                                               _p(nob, _ip)), "get_transforms")
+        return nob
+
     def obs_operator_device(self, xb_ptr, axb_ptr, n_sites, lat_ptr, lon_ptr, row_ptr_ptr,
                             n_rows, row_var_ptr, row_ph_ptr, yo_ptr, yb_ptr, dep_ptr):
         """H(x) and Yb/d centring (gen_obs_struct) on device arrays."""

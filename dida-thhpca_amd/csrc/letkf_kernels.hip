@@ -427,46 +427,59 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
   double bacc[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) bacc[t] = 0.0;
-  // rows are loaded one group (4 rows) ahead of the MFMAs that use them
-  auto load = [&](long long rr, double* y, double& w, double& wd) {
-    w = 0.0;
-    wd = 0.0;
-    if (rr < r1) {
-      const int row = p.row_idx[rr];
-      if (p.pre_weighted) {
-        w = p.row_gc[rr];
-        wd = p.row_wd[rr];
-      } else {
-        w = row_weight(p, row, p.row_gc[rr], vc);
-        wd = w * p.dep[row];
-      }
-      const double* yr = p.yb64 + (long long)row * p.k;
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const int m = 16 * t + i16;
-        y[t] = (m < p.k) ? yr[m] : 0.0;
-      }
+  // G groups of 4 rows per step (lane q takes rows 4g + q), loaded one step
+  // ahead of the MFMAs that use them; positions past the end of the row list
+  // read its last row with zero weight (every load in bounds, no branch)
+  constexpr int G = KP <= 32 ? 4 : (KP <= 64 ? 2 : 1);
+  const int n = (int)(r1 - r0);
+  auto load = [&](int rl, double* y, double& w, double& wd) {
+    const int rc = rl < n ? rl : n - 1;
+    const int row = p.row_idx[r0 + rc];
+    double wv, wdv;
+    if (p.pre_weighted) {
+      wv = p.row_gc[r0 + rc];
+      wdv = p.row_wd[r0 + rc];
     } else {
+      wv = row_weight(p, row, p.row_gc[r0 + rc], vc);
+      wdv = wv * p.dep[row];
+    }
+    const bool in = rl < n;
+    w = in ? wv : 0.0;
+    wd = in ? wdv : 0.0;
+    const double* yr = p.yb64 + (long long)row * p.k;
 #pragma unroll
-      for (int t = 0; t < T; ++t) y[t] = 0.0;
+    for (int t = 0; t < T; ++t) {
+      const int m = 16 * t + i16;
+      y[t] = (m < p.k) ? yr[m < p.k ? m : 0] : 0.0;
     }
   };
-  double yn[T], wn, wdn;
-  load(r0 + q, yn, wn, wdn);
-  for (long long base = r0; base < r1; base += 4) {
-    double y[T];
+  double yn[G][T], wn[G], wdn[G];
+  if (n > 0) {
 #pragma unroll
-    for (int t = 0; t < T; ++t) y[t] = yn[t];
-    const double w = wn, wd = wdn;
-    load(base + 4 + q, yn, wn, wdn);
+    for (int g = 0; g < G; ++g) load(4 * g + q, yn[g], wn[g], wdn[g]);
+  }
+  for (int g0 = 0; g0 < n; g0 += 4 * G) {
+    double y[G][T], w[G], wd[G];
 #pragma unroll
-    for (int x = 0; x < NP; ++x) {
-      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
-      acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[I], w * y[J], acc[x], 0, 0, 0);
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) y[g][t] = yn[g][t];
+      w[g] = wn[g];
+      wd[g] = wdn[g];
     }
-    if (WS == 0) {
 #pragma unroll
-      for (int t = 0; t < T; ++t) bacc[t] = fma(wd, y[t], bacc[t]);
+    for (int g = 0; g < G; ++g) load(g0 + 4 * G + 4 * g + q, yn[g], wn[g], wdn[g]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int x = 0; x < NP; ++x) {
+        const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+        acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(y[g][I], w[g] * y[g][J], acc[x], 0, 0, 0);
+      }
+      if (WS == 0) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) bacc[t] = fma(wd[g], y[g][t], bacc[t]);
+      }
     }
   }
   double* A = p.A + (a - p.a0) * (long long)KP * KP;

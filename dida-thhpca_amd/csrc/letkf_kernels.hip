@@ -568,22 +568,36 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
       y[t] = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
     }
   };
-  float yn[T], wfn = 0.f, wdn = 0.f;
-  if (n > 0) load(q, yn, wfn, wdn);
-  for (int g0 = 0; g0 < n; g0 += 4) {
-    float y[T];
+  // G groups of 4 rows per step (lane q takes rows 4g + q), loaded one step
+  // ahead of the MFMAs that use them
+  constexpr int G = KP <= 64 ? 4 : 2;
+  float yn[G][T], wfn[G], wdn[G];
+  if (n > 0) {
 #pragma unroll
-    for (int t = 0; t < T; ++t) y[t] = yn[t];
-    const float wf = wfn, wd = wdn;
-    load(g0 + 4 + q, yn, wfn, wdn);
+    for (int g = 0; g < G; ++g) load(4 * g + q, yn[g], wfn[g], wdn[g]);
+  }
+  for (int g0 = 0; g0 < n; g0 += 4 * G) {
+    float y[G][T], wf[G], wd[G];
 #pragma unroll
-    for (int x = 0; x < NP; ++x) {
-      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x4f32(y[I], wf * y[J], acc[x], 0, 0, 0);
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) y[g][t] = yn[g][t];
+      wf[g] = wfn[g];
+      wd[g] = wdn[g];
     }
-    if (WS == 0) {
 #pragma unroll
-      for (int t = 0; t < T; ++t) bacc[t] = fmaf(wd, y[t], bacc[t]);
+    for (int g = 0; g < G; ++g) load(g0 + 4 * G + 4 * g + q, yn[g], wfn[g], wdn[g]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int x = 0; x < NP; ++x) {
+        const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+        acc[x] = __builtin_amdgcn_mfma_f32_16x16x4f32(y[g][I], wf[g] * y[g][J], acc[x], 0, 0, 0);
+      }
+      if (WS == 0) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) bacc[t] = fmaf(wd[g], y[g][t], bacc[t]);
+      }
     }
   }
   double* A = p.A + (a - p.a0) * (long long)KP * KP;

@@ -429,8 +429,10 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
   for (int t = 0; t < T; ++t) bacc[t] = 0.0;
   // G groups of 4 rows per step (lane q takes rows 4g + q), loaded one step
   // ahead of the MFMAs that use them; positions past the end of the row list
-  // read its last row with zero weight (every load in bounds, no branch)
-  constexpr int G = KP <= 32 ? 4 : (KP <= 64 ? 2 : 1);
+  // read its last row with zero weight (every load in bounds, no branch).
+  // More groups cost wavefronts per SIMD, which the per-row vertical weights
+  // of the k=32 level loop need more
+  constexpr int G = KP == 64 ? 2 : 1;
   const int n = (int)(r1 - r0);
   auto load = [&](int rl, double* y, double& w, double& wd) {
     const int rc = rl < n ? rl : n - 1;
@@ -570,7 +572,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
   };
   // G groups of 4 rows per step (lane q takes rows 4g + q), loaded one step
   // ahead of the MFMAs that use them
-  constexpr int G = KP <= 64 ? 4 : 2;
+  constexpr int G = KP == 64 ? 4 : 1;
   float yn[G][T], wfn[G], wdn[G];
   if (n > 0) {
 #pragma unroll

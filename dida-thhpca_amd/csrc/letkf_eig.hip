@@ -24,13 +24,106 @@
 // with half rows per thread (256 threads per matrix).
 //
 // Matrices whose QL did not converge or whose rotation log overflowed are
-// flagged (status != 0).  For k <= 64 k_vec and k_update skip them and the
-// host reruns them with the single-kernel transform; for k > 64 they are
-// applied with the inexact basis and the flag is returned to the caller.
+// flagged (status != 0) and appended to a device-side list; k_vec and
+// k_update skip them.  A second pass of k_ql_log, k_vec and k_update over
+// that list (list mode: a longer rotation log, more QL iterations) redoes
+// them from the stored tridiagonal form, with no host synchronisation.  An
+// analysis still flagged after it leaves xa = xb, axa = axb for its state and
+// NaN in trans/overline_w; the caller gets the count.
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
 namespace dida {
+
+// The analysis of workgroup (or lane) `slot`: a = global index, ai = index of
+// the batch-local inputs (A, b, refl, tau, d, e), ar = index of the pass's own
+// scratch (rot, hdr, nchase, lam).  False when the slot holds no analysis.
+__device__ __forceinline__ bool eig_slot(const EigParams& p, long long slot, long long& a,
+                                         long long& ai, long long& ar) {
+  if (p.list) {
+    const int n = min(*p.list_n, p.list_cap);
+    if (slot >= n) return false;
+    a = p.list[slot];
+    ai = a - p.a0;
+    ar = slot;
+  } else {
+    if (slot >= p.n_an) return false;
+    a = p.a0 + slot;
+    ai = slot;
+    ar = slot;
+  }
+  return true;
+}
+
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// State rows of one analysis: the columns of its cube and, on the vertical
+// loop, the levels of its chunk (with the 2-D variables on the top chunk).
+struct AnRows {
+  int sj, ej, si, ei, sk, ek;
+  long long lat_ext;
+};
+__device__ __forceinline__ AnRows an_rows(const EigParams& p, long long a) {
+  AnRows r;
+  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
+  r.sj = ic * p.g.cube_n + 1;
+  r.ej = min(r.sj + p.g.cube_n - 1, p.g.nlat);
+  r.si = jc * p.g.cube_n + 1;
+  r.ei = min(r.si + p.g.cube_n - 1, p.g.nlon);
+  r.lat_ext = p.g.nlat + 2 * p.halo;
+  r.sk = 1;
+  r.ek = p.nz;
+  if (p.vertical) {
+    r.sk = kc * p.cube_nz + 1;
+    r.ek = min(r.sk + p.cube_nz - 1, p.nz);
+  }
+  return r;
+}
+
+// An analysis whose eigensolve failed on both passes: its state rows keep the
+// background, xa = xb and axa = axb (the whole workgroup copies).
+__device__ void copy_background(const EigParams& p, long long a) {
+  const AnRows R = an_rows(p, a);
+  const int k = p.k;
+  for (int lo = R.si; lo <= R.ei; ++lo)
+    for (int la = R.sj; la <= R.ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * R.lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int vv = 0; vv < p.n_var; ++vv) {
+        int l0, l1;
+        const int start = p.var_start[vv] - 1;
+        if (p.var_lev[vv] == 1) {
+          if (p.vertical && R.ek != p.nz) continue;
+          l0 = l1 = start;
+        } else if (p.vertical) {
+          l0 = start + R.sk - 1;
+          l1 = start + R.ek - 1;
+        } else {
+          l0 = start;
+          l1 = start + p.var_lev[vv] - 1;
+        }
+        for (int e = threadIdx.x; e < (l1 - l0 + 1) * (k + 1); e += blockDim.x) {
+          const long long o = off + l0 + e / (k + 1);
+          const int m = e % (k + 1);
+          if (m < k) p.xa[o * k + m] = p.xb[o * k + m];
+          else p.axa[o] = p.axb[o];
+        }
+      }
+    }
+}
+
+// trans and overline_w of an analysis whose eigensolve failed on both passes
+__device__ void nan_transform(const EigParams& p, long long a) {
+  const int k = p.k;
+  const double nan = __longlong_as_double(0x7ff8000000000000LL);
+  if (p.trans_out)
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) p.trans_out[a * (long long)k * k + e] = nan;
+  if (p.w_out)
+    for (int e = threadIdx.x; e < k; e += blockDim.x) p.w_out[a * (long long)k + e] = nan;
+}
 
 // ---------------------------------------------------------------------------
 // k_tridiag
@@ -161,16 +254,15 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
 #define SE(i) se_l[(i) + 1][lane]
   SD(-1) = 0.0;
   SE(-1) = 0.0;
-  const long long a = p.a0 + (long long)blockIdx.x * 64 + lane;
-  const bool valid = a < p.a0 + p.n_an;
-  const long long al = a - p.a0;
+  long long a = 0, ai = 0, ar = 0;
+  const bool valid = eig_slot(p, (long long)blockIdx.x * 64 + lane, a, ai, ar);
   const int k = p.k, kp = p.kp;
   double tnorm = 0.0;
   for (int i = 0; i < KMAX; ++i) {
     double dv = 0.0, ev = 0.0;
     if (valid && i < k) {
-      dv = p.d[al * kp + i];
-      ev = p.e[al * kp + i];
+      dv = p.d[ai * kp + i];
+      ev = p.e[ai * kp + i];
     }
     SD(i) = dv;
     SE(i) = ev;
@@ -178,8 +270,8 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
   }
   if (!valid) return;
   const double eps = 2.220446049250313e-16;
-  double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
-  int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
+  double2* __restrict__ rot = p.rot + ar * (long long)p.rot_cap;
+  int4* __restrict__ hdr = p.hdr + ar * (long long)p.hdr_cap;
   int nrot = 0, nhdr = 0, status = 0;
   for (int l = 0; l < k && status == 0; ++l) {
     int iter = 0;
@@ -200,8 +292,8 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
         }
       }
       if (m == l) break;
-      if (++iter > 30) { status = l + 1; break; }
-      if (nhdr >= p.hdr_cap || nrot + (m - l) + 14 > p.rot_cap) { status = -1; break; }
+      if (++iter > p.max_iter) { status = l + 1; break; }
+      if (nhdr >= p.hdr_cap || nrot + (m - l) + 14 > p.rot_cap) { status = kStatusLogFull; break; }
       double g = (SD(l + 1) - SD(l)) / (2.0 * SE(l));
       double r = hypot(g, 1.0);
       g = SD(m) - SD(l) + SE(l) / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
@@ -268,9 +360,15 @@ __global__ __launch_bounds__(64) void k_ql_log(EigParams p) {
       SE(m) = 0.0;
     }
   }
-  for (int i = 0; i < k; ++i) p.lam[al * kp + i] = SD(i);
-  p.nchase[al] = nhdr;
+  for (int i = 0; i < k; ++i) p.lam[ar * kp + i] = SD(i);
+  p.nchase[ar] = nhdr;
   p.status[a] = status;
+  if (status != 0 && !p.list) {
+    const int s = atomicAdd(p.flag_n, 1);
+    if (s < p.flag_cap) p.flag_list[s] = (int)a;
+    else p.status[a] = kStatusNotRedone;
+    atomicAdd(p.flag_total, 1);
+  }
 }
 #undef SD
 #undef SE
@@ -355,14 +453,16 @@ void k_vec(EigParams p) {
   __shared__ __align__(16) double sbuf[L::N];
   __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
   const int lane = threadIdx.x;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  if (p.status[a] != 0) return;          // rerun by the single-kernel transform
+  long long a, ai, ar;
+  if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
+  if (p.status[a] != 0) {                // redone by the list pass, or failed
+    if (p.list || p.status[a] == kStatusNotRedone) nan_transform(p, a);
+    return;
+  }
   const int k = p.k, kp = p.kp;
   const bool in_r = lane < k;
-  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
-  const double* __restrict__ tg = p.tau + al * kp;
+  double* __restrict__ refl = p.refl + ai * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + ai * kp;
   const int nref = k > 2 ? k - 2 : 0;
 
   // ---- A. Z <- Z H_j, reflectors staged 16 at a time ----
@@ -414,9 +514,9 @@ void k_vec(EigParams p) {
   // ---- B. logged QL rotations streamed through an LDS ring of REFILL-entry refills ----
   {
     double2* sring = reinterpret_cast<double2*>(sbuf);
-    const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
-    const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
-    const int nch = uniform(min(p.nchase[al], p.hdr_cap));
+    const double2* __restrict__ rot = p.rot + ar * (long long)p.rot_cap;
+    const int4* __restrict__ hdr = p.hdr + ar * (long long)p.hdr_cap;
+    const int nch = uniform(min(p.nchase[ar], p.hdr_cap));
     int nrot = 0;
     if (nch > 0) {
       const int4 hl = hdr[nch - 1];
@@ -474,7 +574,7 @@ void k_vec(EigParams p) {
   }
 
   // ---- C. eigenvalue functions (letkf_core.F90:107-113, guard for reduced modes) ----
-  double lam = in_r ? p.lam[al * kp + lane] : 0.0;
+  double lam = in_r ? p.lam[ar * kp + lane] : 0.0;
   if (p.guard) {
     const double mn = wave_min_full(in_r ? lam : 1e300);
     if (mn < 0.0) lam = lam - mn;
@@ -483,7 +583,7 @@ void k_vec(EigParams p) {
   if (lane < KMAX) {
     sf[lane] = in_r ? sqrt((double)(k - 1) / lam) : 0.0;
     sg[lane] = in_r ? 1.0 / lam : 0.0;
-    sb[lane] = in_r ? p.b[al * kp + lane] : 0.0;
+    sb[lane] = in_r ? p.b[ai * kp + lane] : 0.0;
   }
   // T = V diag(f) V^T on f64 MFMA in IPP tile rows per pass; V is staged
   // 16 columns at a time ([column][row] in LDS).  The first pass also forms
@@ -589,13 +689,15 @@ void k_update(EigParams p) {
   static_assert(KMAX == 32 || KMAX == 64, "k_update covers k <= 64");
   __shared__ double sT[KMAX * LDT];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  if (p.status[a] != 0) return;
+  long long a, ai, ar;
+  if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
+  if (p.status[a] != 0) {                // redone by the list pass, or failed
+    if (p.list || p.status[a] == kStatusNotRedone) copy_background(p, a);
+    return;
+  }
   const int k = p.k;
   const int i16 = lane & 15, q = lane >> 4;
-  const double* __restrict__ tb = p.refl + al * (long long)p.kp * p.kp;
+  const double* __restrict__ tb = p.refl + ai * (long long)p.kp * p.kp;
   for (int idx = tid; idx < TT * TT * 4 * 64; idx += 256) {
     const int l = idx & 63, c = idx >> 6, r = c & 3, IJ = c >> 2;
     const int row = 16 * (IJ / TT) + 4 * r + (l >> 4), col = 16 * (IJ % TT) + (l & 15);
@@ -940,14 +1042,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int tid = threadIdx.x;
   const int r = tid & 127, h = uniform(tid >> 7);   // the half is one per wavefront
   const int lane = tid & 63, wv = uniform(tid >> 6);
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
-  // flagged analyses are applied too (inexact basis; the flag is returned)
+  long long a, ai, ar;
+  if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
+  if (p.status[a] != 0) {                // redone by the list pass, or failed
+    if (p.list || p.status[a] == kStatusNotRedone) nan_transform(p, a);
+    return;
+  }
   const int k = p.k, kp = p.kp;
   const bool in_r = r < k;
-  double* __restrict__ refl = p.refl + al * (long long)kp * kp;
-  const double* __restrict__ tg = p.tau + al * kp;
+  double* __restrict__ refl = p.refl + ai * (long long)kp * kp;
+  const double* __restrict__ tg = p.tau + ai * kp;
   const int nref = k > 2 ? k - 2 : 0;
 
   // ---- A. Z <- Z H_j: half dots exchanged through LDS ----
@@ -993,9 +1097,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   // ---- B. logged QL rotations ----
   {
     double2* sring = reinterpret_cast<double2*>(sbuf);
-    const double2* __restrict__ rot = p.rot + al * (long long)p.rot_cap;
-    const int4* __restrict__ hdr = p.hdr + al * (long long)p.hdr_cap;
-    const int nch = uniform(min(p.nchase[al], p.hdr_cap));
+    const double2* __restrict__ rot = p.rot + ar * (long long)p.rot_cap;
+    const int4* __restrict__ hdr = p.hdr + ar * (long long)p.hdr_cap;
+    const int nch = uniform(min(p.nchase[ar], p.hdr_cap));
     int nrot = 0;
     if (nch > 0) {
       const int4 hl = hdr[nch - 1];
@@ -1093,7 +1197,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
   // ---- C. eigenvalue functions, T = V diag(f) V^T on MFMA, overline_w ----
   if (tid < KMAX) {
-    double lam = tid < k ? p.lam[al * kp + tid] : 0.0;
+    double lam = tid < k ? p.lam[ar * kp + tid] : 0.0;
     sf[tid] = lam;
   }
   __syncthreads();
@@ -1108,7 +1212,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const bool ok = tid < k;
     const double lam = ok ? sf[tid] * 1.0 + (double)(k - 1) / p.inflation : 1.0;
     sg[tid] = ok ? 1.0 / lam : 0.0;
-    sb[tid] = ok ? p.b[al * kp + tid] : 0.0;
+    sb[tid] = ok ? p.b[ai * kp + tid] : 0.0;
     sf[tid] = ok ? sqrt((double)(k - 1) / lam) : 0.0;
   }
   // st = diag(1/lambda) V^T b: column sums over rows, V staged 16 columns at
@@ -1206,12 +1310,15 @@ __global__ __launch_bounds__(256) void k_update128(EigParams p) {
   __shared__ double spart[4][16];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  const long long a = p.a0 + blockIdx.x;
-  if (a >= p.a0 + p.n_an) return;
-  const long long al = a - p.a0;
+  long long a, ai, ar;
+  if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
+  if (p.status[a] != 0) {                // redone by the list pass, or failed
+    if (p.list || p.status[a] == kStatusNotRedone) copy_background(p, a);
+    return;
+  }
   const int k = p.k, kp = p.kp;
   const int i16 = lane & 15, q = lane >> 4;
-  const double* __restrict__ tb = p.refl + al * (long long)kp * kp;
+  const double* __restrict__ tb = p.refl + ai * (long long)kp * kp;
   d4 bop[8][2];
 #pragma unroll
   for (int I = 0; I < 8; ++I)
@@ -1484,6 +1591,38 @@ static void launch_eig_stage(hipStream_t st, const EigParams& p, int stage) {
   if (p.k <= 32) launch_eig_stage_k<32>(st, p, stage);
   else if (p.k <= 64) launch_eig_stage_k<64>(st, p, stage);
   else launch_eig_stage_k<128>(st, p, stage);
+}
+
+hipError_t launch_zero_ints(hipStream_t st, int* p, int n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// The list pass: QL, eigenvectors + trans and update over the analyses the
+// first pass flagged (p.list, *p.list_n of them, at most p.list_cap).  Grids
+// are sized for list_cap; workgroups past the count exit at once, so the
+// launch needs no count on the host.
+hipError_t launch_eig_rerun(hipStream_t st, const EigParams& p) {
+  if (!p.list || p.list_cap <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)p.list_cap;
+  const unsigned nq = (unsigned)((p.list_cap + 63) / 64);
+  for (int stage = 1; stage < 4; ++stage) {
+    if (p.k <= 32) {
+      if (stage == 1) hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
+      else if (stage == 2) hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
+      else if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
+    } else if (p.k <= 64) {
+      if (stage == 1) hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
+      else if (stage == 2) hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
+      else if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
+    } else {
+      if (stage == 1) hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
+      else if (stage == 2) hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
+      else if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+    }
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_first,

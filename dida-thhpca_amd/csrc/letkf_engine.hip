@@ -98,6 +98,10 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 // Scratch of the split eigensolver for one batch of analyses.
 struct EigScratch {
   DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
+  // flags: [0] analyses the batch's first pass flagged, [1] the same summed
+  // over the batches of a call, [2 ..] their list; rr_*: the list pass's own
+  // scratch (its longer rotation log, chase headers, eigenvalues)
+  DevBuf flags, rr_rot, rr_hdr, rr_nchase, rr_lam;
   std::vector<int> st_host;
   // second stream and events of the k = 65..128 half-batch pipeline, created
   // on first use (default priority)
@@ -148,13 +152,47 @@ size_t eig_bytes_per_analysis(int k) {
          16 * (size_t)kHdrCapPerK * k + sizeof(int);
 }
 
+// QL iterations per eigenvalue before a matrix is flagged (first pass; the
+// list pass allows twice as many), as the EISPACK/Numerical Recipes QL does
+constexpr int kQlMaxIter = 30;
+
+// Analyses the list pass can redo per batch, and its per-analysis scratch
+// (a rotation log and chase log twice the first pass's)
+long long rerun_cap_of(int k, long long n_an) {
+  return std::min<long long>(n_an, k > 64 ? 128 : 256);
+}
+size_t rerun_bytes_per_slot(int k) {
+  const int kp = kp_of(k);
+  return 16 * (size_t)(2 * rot_cap_of(k)) + 16 * (size_t)(2 * kHdrCapPerK * k) + sizeof(int) +
+         sizeof(double) * kp + sizeof(int);
+}
+
+// Optional positive integer from the environment (0 when unset).  The
+// eigensolver's limits can be lowered this way to make the flag paths run:
+// DIDA_ROT_CAP / DIDA_QL_MAXITER (first pass), DIDA_RERUN_ROT_CAP /
+// DIDA_RERUN_MAXITER (list pass).
+int env_int(const char* name) {
+  const char* e = std::getenv(name);
+  if (!e || !*e) return 0;
+  return std::max(0, std::atoi(e));
+}
+
+// Zeroes the call-wide flag count; run once per analysis call.
+int eig_begin_call(hipStream_t st, EigScratch& s, int k, long long max_batch) {
+  RC_TRY(s.flags.ensure(sizeof(int) * (2 + rerun_cap_of(k, max_batch))));
+  HIP_TRY(launch_zero_ints(st, s.flags.as<int>(), 2));
+  return DIDA_OK;
+}
+
 // Eigensolver parameters of the batch xp (its scratch allocated): the split
 // eigensolver (k_tridiag, k_ql_log, k_vec + k_update) over analyses
 // [xp.a0, xp.a0 + xp.n_an).
 int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   const long long n = xp.n_an;
   const int k = xp.k, kp = xp.kp;
-  const int rot_cap = rot_cap_of(k), hdr_cap = kHdrCapPerK * k;
+  int rot_cap = rot_cap_of(k);
+  if (int v = env_int("DIDA_ROT_CAP")) rot_cap = std::max(16, std::min(v, rot_cap));
+  const int hdr_cap = kHdrCapPerK * k;
   RC_TRY(s.refl.ensure(sizeof(double) * n * kp * kp));
   RC_TRY(s.tau.ensure(sizeof(double) * n * kp));
   RC_TRY(s.d.ensure(sizeof(double) * n * kp));
@@ -163,12 +201,23 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   RC_TRY(s.rot.ensure((size_t)16 * n * rot_cap));
   RC_TRY(s.hdr.ensure((size_t)16 * n * hdr_cap));
   RC_TRY(s.nchase.ensure(sizeof(int) * n));
+  const long long rcap = rerun_cap_of(k, n);
+  if (s.flags.bytes < sizeof(int) * (2 + rcap))
+    return set_error(DIDA_ERR_STATE, "eigensolver flags not set up for this batch");
   ep = EigParams{};
   ep.g = xp.g;
   ep.k = k;
   ep.kp = kp;
   ep.inflation = xp.inflation;
   ep.guard = xp.guard;
+  ep.max_iter = env_int("DIDA_QL_MAXITER") ? env_int("DIDA_QL_MAXITER") : kQlMaxIter;
+  ep.flag_n = s.flags.as<int>();
+  ep.flag_total = s.flags.as<int>() + 1;
+  ep.flag_list = s.flags.as<int>() + 2;
+  ep.flag_cap = (int)rcap;
+  ep.list = nullptr;
+  ep.list_n = nullptr;
+  ep.list_cap = 0;
   ep.a0 = xp.a0;
   ep.n_an = n;
   ep.A = xp.A;
@@ -207,37 +256,40 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   return DIDA_OK;
 }
 
-// After the split eigensolver (k <= 64): the single-kernel transform reruns
-// the analyses the split path flagged (k > 64 applies flagged analyses in
-// the split kernels themselves).  Synchronises the stream.
-int rerun_flagged(hipStream_t st, EigScratch& s, const XformParams& xp) {
-  if (xp.k > 64) return DIDA_OK;
-  const long long n = xp.n_an;
-  s.st_host.resize(n);
-  HIP_TRY(hipMemcpyAsync(s.st_host.data(), xp.status + xp.a0, sizeof(int) * n,
-                         hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  std::vector<int> flagged;
-  for (long long i = 0; i < n; ++i)
-    if (s.st_host[i] != 0) flagged.push_back((int)(xp.a0 + i));
-  if (!flagged.empty()) {
-    RC_TRY(s.list.ensure(sizeof(int) * flagged.size()));
-    HIP_TRY(hipMemcpyAsync(s.list.p, flagged.data(), sizeof(int) * flagged.size(),
-                           hipMemcpyHostToDevice, st));
-    XformParams xf = xp;
-    xf.list = s.list.as<int>();
-    xf.n_an = (long long)flagged.size();
-    HIP_TRY(launch_transform(st, xf));
-    HIP_TRY(hipStreamSynchronize(st));
-  }
+// The list pass over the analyses the first pass flagged: k_ql_log with a
+// rotation log and an iteration limit twice the first pass's, then k_vec and
+// k_update, all on st after the first pass (no host synchronisation).
+int rerun_flagged(hipStream_t st, EigScratch& s, const EigParams& ep) {
+  const int k = ep.k, kp = ep.kp;
+  const long long cap = ep.flag_cap;
+  if (cap <= 0) return DIDA_OK;
+  int rot_cap = 2 * rot_cap_of(k);
+  if (int v = env_int("DIDA_RERUN_ROT_CAP")) rot_cap = std::max(16, std::min(v, rot_cap));
+  const int hdr_cap = 2 * kHdrCapPerK * k;
+  RC_TRY(s.rr_rot.ensure((size_t)16 * cap * rot_cap));
+  RC_TRY(s.rr_hdr.ensure((size_t)16 * cap * hdr_cap));
+  RC_TRY(s.rr_nchase.ensure(sizeof(int) * cap));
+  RC_TRY(s.rr_lam.ensure(sizeof(double) * cap * kp));
+  EigParams rp = ep;
+  rp.list = ep.flag_list;
+  rp.list_n = ep.flag_n;
+  rp.list_cap = (int)cap;
+  rp.max_iter = env_int("DIDA_RERUN_MAXITER") ? env_int("DIDA_RERUN_MAXITER") : 2 * kQlMaxIter;
+  rp.rot = s.rr_rot.as<double2>();
+  rp.rot_cap = rot_cap;
+  rp.hdr = s.rr_hdr.as<int4>();
+  rp.hdr_cap = hdr_cap;
+  rp.nchase = s.rr_nchase.as<int>();
+  rp.lam = s.rr_lam.as<double>();
+  HIP_TRY(launch_eig_rerun(st, rp));
   return DIDA_OK;
 }
 
 // Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
-// eigensolver, then the single-kernel transform for analyses the split path
-// flagged; mode 1 runs only the single kernel (k <= 64).  Mode 0 runs batches
-// of 2 kEigPipeMinHalf analyses or more as two half-batches on two streams;
-// mode 2 keeps the split eigensolver on st (each kernel timed by itself).
+// eigensolver, then its list pass over the analyses it flagged; mode 1 runs
+// only the single kernel (k <= 64).  Mode 0 runs batches of 2 kEigPipeMinHalf
+// analyses or more as two half-batches on two streams; mode 2 keeps the split
+// eigensolver on st (each kernel timed by itself).
 // kev: null, or 5 events for the split kernels (recorded only if *split is set)
 int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
                   hipEvent_t* kev = nullptr, bool* split = nullptr) {
@@ -248,6 +300,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   }
   EigParams ep;
   RC_TRY(prepare_eig(s, xp, ep));
+  HIP_TRY(launch_zero_ints(st, ep.flag_n, 1));
   if (mode == 0 && xp.n_an >= 2 * kEigPipeMinHalf) {
     if (!s.st2) {
       HIP_TRY(hipStreamCreateWithFlags(&s.st2, hipStreamNonBlocking));
@@ -259,7 +312,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
     HIP_TRY(launch_eig(st, ep, kev));
   }
   if (split) *split = true;
-  return rerun_flagged(st, s, xp);
+  return rerun_flagged(st, s, ep);
 }
 
 namespace dida {
@@ -435,7 +488,9 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
-                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list};
+                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list,
+                    &h->eig.flags, &h->eig.rr_rot, &h->eig.rr_hdr, &h->eig.rr_nchase,
+                    &h->eig.rr_lam};
   for (DevBuf* b : bufs) b->release();
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -648,6 +703,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     RC_TRY(h->wout.ensure(sizeof(double) * h->n_an * c.k));
     h->nobsl_host.assign(h->n_an, 0);
   }
+  RC_TRY(eig_begin_call(st, h->eig, c.k, mb * h->nzc));
   long long total_rows = 0;
   int max_rows = 0, over_cap = 0;
   std::vector<long long> cnt_host;
@@ -860,9 +916,13 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     ms_xf += t3;
     HIP_TRY(hipEventRecord(ev[0], st));
   }
-  // eigensolver flags
+  // eigensolver flags (analyses still flagged after the list pass) and the
+  // count the list pass redid
   std::vector<int> stt(h->n_an);
+  int fl[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(stt.data(), h->status.p, sizeof(int) * h->n_an, hipMemcpyDeviceToHost, st));
+  if (c.transform_kernel != 1 || c.k > 64)
+    HIP_TRY(hipMemcpyAsync(fl, h->eig.flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   int flagged = 0;
   for (int s : stt) flagged += (s != 0);
@@ -880,6 +940,7 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
     stats->ms_ql = ms_k[1];
     stats->ms_apply = ms_k[2];
     stats->ms_update = ms_k[3];
+    stats->n_rerun = fl[1];
   }
   return flagged;
 }
@@ -1038,9 +1099,10 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   xp.status = d_s->as<int>();
   xp.list = nullptr;
   EigScratch es;
-  int rc_t = run_transform(st, es, xp, 0);
+  int rc_t = eig_begin_call(st, es, k, n_an);
+  if (rc_t == DIDA_OK) rc_t = run_transform(st, es, xp, 0);
   DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
-                     &es.list};
+                     &es.list, &es.flags, &es.rr_rot, &es.rr_hdr, &es.rr_nchase, &es.rr_lam};
   if (rc_t < 0) {
     (void)hipStreamSynchronize(st);
     es.release_streams();
@@ -1071,6 +1133,12 @@ void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,
   int st = 0;
   int rc = dida_letkf_core_batched(*nanals, 1, rp, hxens_in, rdiaginv, dep, rloc, trans,
                                    overline_w, *inflation_factor, DIDA_PREC_FP64, &st);
+  if (rc < 0 && *nanals > 0) {
+    // no result: the outputs are NaN, so the caller's update cannot pass for one
+    const double nan = std::nan("");
+    for (long long i = 0; i < (long long)*nanals * *nanals; ++i) trans[i] = nan;
+    for (int i = 0; i < *nanals; ++i) overline_w[i] = nan;
+  }
   if (rc != 0) fprintf(stderr, "letkf_core_: rc=%d %s\n", rc, g_last_error.c_str());
 }
 
@@ -1095,9 +1163,10 @@ size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows)
   const double th = 2.0 * c->local_dist / 6371.39306640625;
   const double frac = std::min(1.0, (1.0 - std::cos(std::min(th, M_PI))) / 2.0 * 2.0);
   const size_t rows_col = std::min<size_t>(nr, (size_t)(frac * nr) + 64);
-  b += 12 * mb * rows_col;
+  b += 20 * mb * rows_col;                                          // row_idx, row_gc, row_wd
   b += 16 * (mb + 1) * 2;
   b += mb * nzc * (8 * kp * kp + 8 * kp + eig_bytes_per_analysis(c->k));
+  b += rerun_cap_of(c->k, (long long)(mb * nzc)) * rerun_bytes_per_slot(c->k);
   b += 4 * n_an;
   if (c->keep_transforms) b += n_an * (8 * k * k + 8 * k);
   b += n_state * (k + 1) * 8;                                       // serving ping-pong state

@@ -16,6 +16,12 @@ constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis, pl
 // the identity entries that align each chase to blocks of 8 positions
 inline int rot_cap_of(int k) { return kRotCapPerK2 * k * k + 16 * k + 64; }
 constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
+// eigensolver status per analysis: 0 converged; l + 1 > 0: the QL of
+// eigenvalue l did not converge; kStatusLogFull: the rotation or chase log
+// filled up; kStatusNotRedone: flagged by the first pass when the list pass
+// was full (the analysis keeps its background)
+constexpr int kStatusLogFull = -1;
+constexpr int kStatusNotRedone = -2;
 
 struct Geom {
   const float* glat;   // nlat, real(4) degrees
@@ -126,7 +132,22 @@ struct EigParams {
   int k, kp;
   double inflation;
   int guard;
+  int max_iter;             // QL iterations per eigenvalue before the matrix is flagged
   long long a0, n_an;
+  // Flag list.  First pass (list == NULL): k_ql_log appends every analysis it
+  // flags to flag_list (global index, at most flag_cap entries; *flag_n
+  // counts all of them, *flag_total accumulates over the batches of a call).
+  // Rerun pass (list != NULL): the kernels take analysis list[slot] for
+  // slot < min(*list_n, list_cap); batch-local inputs (d, e, refl, tau, b) are
+  // indexed by list[slot] - a0, the pass's own scratch (rot, hdr, nchase, lam)
+  // by slot.
+  int* flag_list;
+  int* flag_n;
+  int* flag_total;
+  int flag_cap;
+  const int* list;
+  const int* list_n;
+  int list_cap;
   const double* A;          // batch-local, full symmetric kp x kp
   const double* b;          // batch-local, kp
   double* refl;             // batch-local, kp x kp reflectors (column j = v_j)
@@ -206,6 +227,9 @@ hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p);
 // halo of a [lon + 2h][lat + 2h][n] field with `inner` values per element
 hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);
 hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev = nullptr);
+// second pass over the analyses the first flagged (p.list / p.list_n / p.list_cap)
+hipError_t launch_eig_rerun(hipStream_t st, const EigParams& p);
+hipError_t launch_zero_ints(hipStream_t st, int* p, int n);
 // k = 65..128 batches of at least 2 kEigPipeMinHalf analyses: the batch in two
 // halves, the second on st2 (ordered after the first half's
 // tridiagonalisation, joined back into st through e_join); ev as above, with

@@ -53,7 +53,8 @@ struct dida_serve {
   std::condition_variable cv;          // job state changes
   std::deque<std::shared_ptr<Job>> queue;
   std::map<long long, std::shared_ptr<Job>> jobs;
-  std::vector<std::thread> workers;
+  std::map<long long, std::thread> workers;   // running (or just finished) jobs
+  std::vector<long long> finished;            // workers to join
   size_t used = 0;
   int active = 0;
   long long next_id = 1;
@@ -138,8 +139,21 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
                   : (rc < 0 ? DIDA_JOB_FAILED : DIDA_JOB_DONE);
     s->used -= j->st.bytes;
     s->active -= 1;
+    s->finished.push_back(j->id);
   }
   s->cv.notify_all();
+}
+
+// Joins the worker threads of finished jobs (called with s->mu held; a
+// finished worker only has its return left to run).
+static void reap_workers(dida_serve* s) {
+  for (long long id : s->finished) {
+    auto it = s->workers.find(id);
+    if (it == s->workers.end()) continue;
+    if (it->second.joinable()) it->second.join();
+    s->workers.erase(it);
+  }
+  s->finished.clear();
 }
 
 static void dispatch_loop(dida_serve* s) {
@@ -147,10 +161,15 @@ static void dispatch_loop(dida_serve* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     s->cv.wait(lk, [&] {
-      if (s->stopping) return true;
+      if (s->stopping || !s->finished.empty()) return true;
       if (s->queue.empty() || s->active >= s->cfg.max_active) return false;
       return s->used + s->queue.front()->st.bytes <= s->cfg.hbm_budget_bytes;
     });
+    reap_workers(s);
+    if (!s->stopping &&
+        (s->queue.empty() || s->active >= s->cfg.max_active ||
+         s->used + s->queue.front()->st.bytes > s->cfg.hbm_budget_bytes))
+      continue;
     if (s->stopping && s->queue.empty()) break;
     if (s->stopping) {
       // drain: cancel what was never admitted
@@ -174,7 +193,7 @@ static void dispatch_loop(dida_serve* s) {
     s->active += 1;
     j->st.state = DIDA_JOB_RUNNING;
     j->st.t_start = now_s();
-    s->workers.emplace_back(run_job, s, j);
+    s->workers.emplace(j->id, std::thread(run_job, s, j));
   }
 }
 
@@ -302,7 +321,7 @@ int dida_serve_destroy(dida_serve* s) {
   if (s->dispatcher.joinable()) s->dispatcher.join();
   // running jobs finish on their own threads
   for (auto& t : s->workers)
-    if (t.joinable()) t.join();
+    if (t.second.joinable()) t.second.join();
   delete s;
   return DIDA_OK;
 }

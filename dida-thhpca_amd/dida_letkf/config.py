@@ -34,9 +34,9 @@ import numpy as np
 
 # Precision modes of the engine (the reference selects letkf_core for
 # computing_precision >= 52 and letkf_core_mpal below, letkf_compute.F90:105-111).
-PREC_FP64 = 0      # obs-space products and update in FP64 (reference letkf_core)
-PREC_FP32 = 1      # FP32 MFMA products + update, FP64 eigensolve (mpal-style guard)
-PREC_BF16X3 = 2    # BF16 hi/lo split MFMA products, FP32 update, FP64 eigensolve
+PREC_FP64 = 0      # obs-space products in FP64 (reference letkf_core)
+PREC_FP32 = 1      # FP32 MFMA products, FP64 eigensolve (mpal-style guard)
+PREC_BF16X3 = 2    # BF16 hi/lo split MFMA products, FP64 eigensolve
 
 VCOORD_LEVEL = 0   # vertical coordinate is the level index (level units)
 VCOORD_LNP = 1     # vertical coordinate is ln p (ps2alnp with hybrid coefficients)

@@ -77,7 +77,7 @@ class DidaStats(C.Structure):
                 ("ms_select", C.c_double), ("ms_products", C.c_double),
                 ("ms_transform", C.c_double), ("ms_total", C.c_double),
                 ("ms_tridiag", C.c_double), ("ms_ql", C.c_double), ("ms_apply", C.c_double),
-                ("ms_update", C.c_double)]
+                ("ms_update", C.c_double), ("n_rerun", C.c_int)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -17,10 +17,11 @@
  * handle's stream.
  *
  * Errors: functions return DIDA_OK (0), a negative DIDA_ERR_* code, or, from
- * the analysis calls, a positive count of analyses whose eigensolver did not
- * converge (the reference prints dsyevd's ierr and continues,
- * letkf_core.F90:119-122).  dida_letkf_last_error() gives the message of the
- * last failure on the calling thread.  No call aborts the process.
+ * the analysis calls, a positive count of analyses whose eigensolve failed
+ * (the reference prints dsyevd's ierr and continues, letkf_core.F90:119-122);
+ * those analyses leave the background in place and give NaN transforms.
+ * dida_letkf_last_error() gives the message of the last failure on the
+ * calling thread.  No call aborts the process.
  *
  * Threading: a handle is used by one host thread at a time; different
  * handles are independent.
@@ -46,15 +47,16 @@ enum {
     DIDA_ERR_UNSUPPORTED = -6   /* configuration not supported */
 };
 
-/* Precision modes.  FP64 is the reference letkf_core (computing_precision
- * >= 52).  The reduced modes follow letkf_core_mpal's pattern
- * (letkf_core_mpal.F90): low-precision obs-space products, FP64
- * eigendecomposition and inverse square root, eigenvalue guard. */
+/* Precision modes of the obs-space products.  FP64 is the reference
+ * letkf_core (computing_precision >= 52).  The reduced modes follow
+ * letkf_core_mpal's pattern (letkf_core_mpal.F90): low-precision obs-space
+ * products, FP64 eigendecomposition and inverse square root, eigenvalue
+ * guard.  The ensemble update runs in FP64 on the double-precision state. */
 enum {
-    DIDA_PREC_FP64 = 0,     /* FP64 products (MFMA f64) and FP64 update */
-    DIDA_PREC_FP32 = 1,     /* FP32 MFMA products, FP64 eigensolve, FP32 update */
+    DIDA_PREC_FP64 = 0,     /* FP64 products (MFMA f64) */
+    DIDA_PREC_FP32 = 1,     /* FP32 MFMA products, FP64 eigensolve */
     DIDA_PREC_BF16X3 = 2    /* BF16 hi/lo split MFMA products (3 terms, FP32
-                               accumulation), FP64 eigensolve, FP32 update */
+                               accumulation), FP64 eigensolve */
 };
 
 /* Vertical coordinate of the vertical-localisation loop. */
@@ -106,7 +108,8 @@ typedef struct dida_letkf_stats {
     long long n_analyses;   /* local analyses (k x k transform solves) */
     long long total_rows;   /* sum of local obs rows over analyses */
     int max_rows;           /* largest local obs count */
-    int n_flagged;          /* analyses whose eigensolver did not converge */
+    int n_flagged;          /* analyses whose eigensolve failed on both passes (their
+                               state keeps the background: xa = xb, axa = axb) */
     int rows_over_10000;    /* analyses the reference's MAX_OBS would truncate */
     double ms_select;       /* device time, local obs selection */
     double ms_products;     /* device time, obs-space products */
@@ -117,6 +120,9 @@ typedef struct dida_letkf_stats {
     double ms_apply;        /* of ms_transform: eigenvector and transform kernel (k <= 64
                                with the fused kernel: also the update) */
     double ms_update;       /* of ms_transform: ensemble update kernel (0 when fused) */
+    int n_rerun;            /* analyses the eigensolver's first pass flagged (QL not
+                               converged in 30 iterations, or its rotation log full)
+                               and its list pass redid */
 } dida_letkf_stats;
 
 typedef struct dida_letkf_handle dida_letkf_handle;

@@ -244,7 +244,7 @@ def increment_error(inst, xa_m, axa_m, xa_r, axa_r):
     return out
 
 
-@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-4)])
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-5)])
 def test_reduced_precision_increment_error(prec, tol):
     import orc
     from dida_letkf import synth, engine
@@ -254,6 +254,25 @@ def test_reduced_precision_increment_error(prec, tol):
     err = increment_error(inst, gpu["xa"], gpu["axa"], ref["xa"], ref["axa"])
     print("precision", prec, "increment rel RMS err per var", err)
     assert max(err.values()) < tol
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_config3_shape_k64_reduced(prec):
+    """The c3 shape at k=64 (clustered obs, sigma log-uniform in [0.3, 3]) on a
+    reduced grid: FP64 at 1e-10 against the oracle, the reduced modes at the
+    1e-5 increment bar."""
+    import orc
+    from dida_letkf import synth, engine
+    inst = synth.generate("c3", seed=26, k=64, nlat=16, nlon=32, nz=3, n_obs=20000,
+                          local_dist=900.0)
+    ref = orc.letkf_compute(inst, want_trans=True)
+    gpu64 = engine.letkf_compute_nooverlap(inst, keep_transforms=True)
+    assert gpu64["stats"]["max_rows"] > 64
+    _assert_parity(gpu64, ref, what="c3-k64-reduced")
+    gpu = engine.letkf_compute_nooverlap(inst, precision=prec)
+    err = increment_error(inst, gpu["xa"], gpu["axa"], ref["xa"], ref["axa"])
+    print("c3 k=64 precision", prec, "increment rel RMS err per var", err)
+    assert max(err.values()) < 1e-5
 
 
 def test_epilogue_device():

@@ -84,37 +84,6 @@ __device__ __forceinline__ AnRows an_rows(const EigParams& p, long long a) {
   return r;
 }
 
-// An analysis whose eigensolve failed on both passes: its state rows keep the
-// background, xa = xb and axa = axb (the whole workgroup copies).
-__device__ void copy_background(const EigParams& p, long long a) {
-  const AnRows R = an_rows(p, a);
-  const int k = p.k;
-  for (int lo = R.si; lo <= R.ei; ++lo)
-    for (int la = R.sj; la <= R.ej; ++la) {
-      const long long off = ((long long)(lo - 1 + p.halo) * R.lat_ext + (la - 1 + p.halo)) * p.n;
-      for (int vv = 0; vv < p.n_var; ++vv) {
-        int l0, l1;
-        const int start = p.var_start[vv] - 1;
-        if (p.var_lev[vv] == 1) {
-          if (p.vertical && R.ek != p.nz) continue;
-          l0 = l1 = start;
-        } else if (p.vertical) {
-          l0 = start + R.sk - 1;
-          l1 = start + R.ek - 1;
-        } else {
-          l0 = start;
-          l1 = start + p.var_lev[vv] - 1;
-        }
-        for (int e = threadIdx.x; e < (l1 - l0 + 1) * (k + 1); e += blockDim.x) {
-          const long long o = off + l0 + e / (k + 1);
-          const int m = e % (k + 1);
-          if (m < k) p.xa[o * k + m] = p.xb[o * k + m];
-          else p.axa[o] = p.axb[o];
-        }
-      }
-    }
-}
-
 // trans and overline_w of an analysis whose eigensolve failed on both passes
 __device__ void nan_transform(const EigParams& p, long long a) {
   const int k = p.k;
@@ -654,7 +623,7 @@ void k_vec(EigParams p) {
             if (f < k && e < k) tr[(long long)e * k + f] = acc[I][J][r];
           }
     }
-    if (p.xb) {
+    if (p.xb || p.xb32) {
       // the B operand of the update, over the (consumed) reflector block
 #pragma unroll
       for (int I = 0; I < IPP; ++I)
@@ -665,6 +634,89 @@ void k_vec(EigParams p) {
     }
   }
   if (p.w_out && in_r) p.w_out[a * (long long)k + lane] = wbar;
+}
+
+// State precision of the update: FP64 (xb/xa) or FP32 (xb32/xa32) members on
+// f64 or f32 MFMA 16x16x4 (the two share their operand layout).
+template <class TS> struct StateT;
+template <> struct StateT<double> {
+  using acc = d4;
+  static __device__ __forceinline__ const double* xb(const EigParams& p) { return p.xb; }
+  static __device__ __forceinline__ double* xa(const EigParams& p) { return p.xa; }
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct StateT<float> {
+  using acc = f4;
+  static __device__ __forceinline__ const float* xb(const EigParams& p) { return p.xb32; }
+  static __device__ __forceinline__ float* xa(const EigParams& p) { return p.xa32; }
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// One state row o (k members at o*k) of a fused-epilogue column: the
+// background mean ab, the analysis members xe (k of them, FP64) with mean aa;
+// writes xa_out and the two spreads (module_da_parallel.F90:222-258).
+__device__ __forceinline__ long long spread_index(const EigParams& p, int lo, int la, int v) {
+  return ((long long)(lo - 1) * p.g.nlat + (la - 1)) * p.n + v;
+}
+
+// An analysis whose eigensolve failed on both passes (or found no room in the
+// list pass): its state rows keep the background -- xa = xb, axa = axb, or
+// with the fused epilogue xa_out = xb + axb and spread_a = spread_b.
+template <class TS, int EPI>
+__device__ void copy_background_t(const EigParams& p, long long a) {
+  const AnRows R = an_rows(p, a);
+  const int k = p.k;
+  const TS* __restrict__ xb = StateT<TS>::xb(p);
+  for (int lo = R.si; lo <= R.ei; ++lo)
+    for (int la = R.sj; la <= R.ej; ++la) {
+      const long long off = ((long long)(lo - 1 + p.halo) * R.lat_ext + (la - 1 + p.halo)) * p.n;
+      for (int vv = 0; vv < p.n_var; ++vv) {
+        int l0, l1;
+        const int start = p.var_start[vv] - 1;
+        if (p.var_lev[vv] == 1) {
+          if (p.vertical && R.ek != p.nz) continue;
+          l0 = l1 = start;
+        } else if (p.vertical) {
+          l0 = start + R.sk - 1;
+          l1 = start + R.ek - 1;
+        } else {
+          l0 = start;
+          l1 = start + p.var_lev[vv] - 1;
+        }
+        for (int nk = l0 + (int)threadIdx.x; nk <= l1; nk += blockDim.x) {
+          const long long o = off + nk;
+          const double ab = p.axb[o];
+          double sb = 0.0;
+          for (int m = 0; m < k; ++m) {
+            const double x = (double)xb[o * k + m];
+            if (EPI) p.xa_out[o * k + m] = x + ab;
+            else StateT<TS>::xa(p)[o * k + m] = xb[o * k + m];
+            sb += x * x;
+          }
+          if (EPI) {
+            const double sp = sqrt(sb / (double)(float)k);
+            p.spread_b[spread_index(p, lo, la, nk)] = sp;
+            p.spread_a[spread_index(p, lo, la, nk)] = sp;
+          } else {
+            p.axa[o] = ab;
+          }
+        }
+      }
+    }
+}
+
+__device__ void copy_background(const EigParams& p, long long a) {
+  if (p.epi) {
+    if (p.state_f32) copy_background_t<float, 1>(p, a);
+    else copy_background_t<double, 1>(p, a);
+  } else {
+    if (p.state_f32) copy_background_t<float, 0>(p, a);
+    else copy_background_t<double, 0>(p, a);
+  }
 }
 
 // k_update: xa(:, v) = trans^T xb(:, v) + axb(v), then re-centred, for the
@@ -678,42 +730,49 @@ void k_vec(EigParams p) {
 // members of one variable (16-byte loads); B takes the same permutation from
 // LDS.  Registers stay near 100, so four blocks share a CU and the loads of
 // one wavefront overlap the MFMAs of the others.
-template <int KMAX>
+//
+// TS is the storage precision of the perturbations (FP32: f32 MFMA with FP32
+// accumulation; the means and the re-centring stay FP64).  EPI = 1 fuses the
+// caller's epilogue into the output pass: full members gamma1 (axb + xb) +
+// gamma2 (axa + xa) (the background where |lat| > polar_lat) and the spreads,
+// written in place of xa/axa.
+template <int KMAX, class TS, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void k_update(EigParams p) {
+  using ST = StateT<TS>;
+  using acc_t = typename ST::acc;
   constexpr int TT = KMAX / 16;
   constexpr int S = KMAX / 4;
-  // S * LDT = 16 (mod 32): the four lane quarters of a B read fall on
-  // disjoint banks in each half-wavefront
+  constexpr int VEC = 16 / sizeof(TS);       // members per 16-byte load
+  // S * LDT = 16 (mod 32) for FP64: the four lane quarters of a B read fall
+  // on disjoint banks in each half-wavefront
   constexpr int LDT = KMAX == 64 ? KMAX + 1 : KMAX + 2;
   static_assert(KMAX == 32 || KMAX == 64, "k_update covers k <= 64");
-  __shared__ double sT[KMAX * LDT];
+  __shared__ TS sT[KMAX * LDT];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   long long a, ai, ar;
   if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
   if (p.status[a] != 0) {                // redone by the list pass, or failed
-    if (p.list || p.status[a] == kStatusNotRedone) copy_background(p, a);
+    if (p.list || p.status[a] == kStatusNotRedone) copy_background_t<TS, EPI>(p, a);
     return;
   }
   const int k = p.k;
   const int i16 = lane & 15, q = lane >> 4;
+  // the row of a 16x16 MFMA result that register r of lane quarter q holds:
+  // q + 4 r for f64 16x16x4, 4 q + r for f32 16x16x4
+  constexpr bool F32 = sizeof(TS) == 4;
+  auto out_row = [q](int r) { return F32 ? 4 * q + r : q + 4 * r; };
+  const TS* __restrict__ xb = ST::xb(p);
   const double* __restrict__ tb = p.refl + ai * (long long)p.kp * p.kp;
   for (int idx = tid; idx < TT * TT * 4 * 64; idx += 256) {
     const int l = idx & 63, c = idx >> 6, r = c & 3, IJ = c >> 2;
     const int row = 16 * (IJ / TT) + 4 * r + (l >> 4), col = 16 * (IJ % TT) + (l & 15);
-    sT[row * LDT + col] = (row < k && col < k) ? tb[idx] : 0.0;
+    sT[row * LDT + col] = (row < k && col < k) ? (TS)tb[idx] : (TS)0;
   }
 
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
-  const long long lat_ext = p.g.nlat + 2 * p.halo;
-  int sk = 1, ek = p.nz;
-  if (p.vertical) {
-    sk = kc * p.cube_nz + 1;
-    ek = min(sk + p.cube_nz - 1, p.nz);
-  }
+  const AnRows R = an_rows(p, a);
+  const int sj = R.sj, ej = R.ej, si = R.si, ei = R.ei, sk = R.sk, ek = R.ek;
+  const long long lat_ext = R.lat_ext;
   // the analysis's state rows: all n of the column, or (vertical loop) the
   // levels sk..ek of each 3-D variable and the 2-D variables on the top chunk;
   // sidx maps the v-th of them to its offset in the column
@@ -739,36 +798,37 @@ void k_update(EigParams p) {
   const int ngrp = (nvec + 15) / 16;
   const int ncl = ej - sj + 1;
   const int n_items = (ei - si + 1) * ncl * ngrp;
-  const bool vec_ld = (k == KMAX) && ((reinterpret_cast<uintptr_t>(p.xb) & 15) == 0);
+  const bool vec_ld = (k == KMAX) && ((reinterpret_cast<uintptr_t>(xb) & 15) == 0);
   auto item_off = [&](int it) {
     const int cl = it / ngrp;
     const int lo = si + cl / ncl, la = sj + cl % ncl;
     return ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
   };
   // A operands of item it: variable 16 g + i16, members S q .. S q + S - 1
-  auto load_a = [&](int it, double (&af)[S]) {
+  auto load_a = [&](int it, TS (&af)[S]) {
     const long long off = item_off(it);
     const int s_a = sidx(16 * (it % ngrp) + i16);
     if (s_a >= 0 && vec_ld) {
-      const double2* xr = reinterpret_cast<const double2*>(p.xb + (off + s_a) * k + S * q);
+      const uint4* xr = reinterpret_cast<const uint4*>(xb + (off + s_a) * k + S * q);
 #pragma unroll
-      for (int u = 0; u < S / 2; ++u) {
-        const double2 t2 = xr[u];
-        af[2 * u] = t2.x;
-        af[2 * u + 1] = t2.y;
+      for (int u = 0; u < S / VEC; ++u) {
+        const uint4 t4 = xr[u];
+        const TS* tv = reinterpret_cast<const TS*>(&t4);
+#pragma unroll
+        for (int x = 0; x < VEC; ++x) af[VEC * u + x] = tv[x];
       }
     } else {
-      const double* xr = p.xb + (off + (s_a >= 0 ? s_a : 0)) * k;
+      const TS* xr = xb + (off + (s_a >= 0 ? s_a : 0)) * k;
 #pragma unroll
       for (int u = 0; u < S; ++u) {
         const int m = S * q + u;
-        af[u] = (s_a >= 0 && m < k) ? xr[m] : 0.0;
+        af[u] = (s_a >= 0 && m < k) ? xr[m] : (TS)0;
       }
     }
   };
   // the first item's A operands are read before the staging barrier, each
   // later item's as soon as the previous item's MFMAs have consumed theirs
-  double af[S];
+  TS af[S];
   if (w < n_items) load_a(w, af);
   __syncthreads();
   for (int it = w; it < n_items; it += 4) {
@@ -778,22 +838,22 @@ void k_update(EigParams p) {
     int st4[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      st4[r] = sidx(v0 + q + 4 * r);
+      st4[r] = sidx(v0 + out_row(r));
       ab4[r] = (st4[r] >= 0) ? p.axb[off + st4[r]] : 0.0;
     }
-    d4 out[TT];
+    acc_t out[TT];
 #pragma unroll
-    for (int J = 0; J < TT; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int J = 0; J < TT; ++J) out[J] = acc_t{0, 0, 0, 0};
     // B of step s + 1 is read while the MFMAs of step s run; the scheduling
     // barriers keep the compiler from hoisting every B read (and its
     // registers) to the top of the loop
-    const double* bq = sT + S * q * LDT + i16;
-    double bn[TT];
+    const TS* bq = sT + S * q * LDT + i16;
+    TS bn[TT];
 #pragma unroll
     for (int J = 0; J < TT; ++J) bn[J] = bq[16 * J];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      double b[TT];
+      TS b[TT];
 #pragma unroll
       for (int J = 0; J < TT; ++J) b[J] = bn[J];
       if (s + 1 < S) {
@@ -801,12 +861,19 @@ void k_update(EigParams p) {
         for (int J = 0; J < TT; ++J) bn[J] = bq[(s + 1) * LDT + 16 * J];
       }
 #pragma unroll
-      for (int J = 0; J < TT; ++J)
-        out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], b[J], out[J], 0, 0, 0);
+      for (int J = 0; J < TT; ++J) out[J] = ST::mma(af[s], b[J], out[J]);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (it + 4 < n_items) load_a(it + 4, af);
-    // lane (i16, q) holds xa of variable v0 + q + 4 r, member 16 J + i16
+    // lane (i16, q) holds xa of variable v0 + out_row(r), member 16 J + i16
+    int lo_c = 0, la_c = 0;
+    bool polar = false;
+    if (EPI) {
+      const int cl = it / ngrp;
+      lo_c = si + cl / ncl;
+      la_c = sj + cl % ncl;
+      polar = fabs((double)p.g.glat[la_c - 1]) > p.polar_lat;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double sum = 0.0;
@@ -814,20 +881,62 @@ void k_update(EigParams p) {
 #pragma unroll
       for (int J = 0; J < TT; ++J) {
         const int e = 16 * J + i16;
-        xe[J] = (e < k) ? out[J][r] + ab4[r] : 0.0;
+        xe[J] = (e < k) ? (double)out[J][r] + ab4[r] : 0.0;
         sum += xe[J];
       }
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
       const double mean = sum / k;
-      if (st4[r] >= 0) {
-        const long long o = off + st4[r];
+      if (!EPI) {
+        if (st4[r] >= 0) {
+          const long long o = off + st4[r];
+          TS* __restrict__ xa = ST::xa(p);
+#pragma unroll
+          for (int J = 0; J < TT; ++J) {
+            const int e = 16 * J + i16;
+            if (e < k) xa[o * k + e] = (TS)(xe[J] - mean);
+          }
+          if (i16 == 0) p.axa[o] = mean;
+        }
+      } else {
+        // the caller's epilogue (module_da_parallel.F90:219-261)
+        const long long o = off + (st4[r] >= 0 ? st4[r] : 0);
+        double sb = 0.0, sa = 0.0;
+        double xo[TT];
 #pragma unroll
         for (int J = 0; J < TT; ++J) {
           const int e = 16 * J + i16;
-          if (e < k) p.xa[o * k + e] = xe[J] - mean;
+          const bool in = e < k && st4[r] >= 0;
+          const double xbe = in ? (double)xb[o * k + e] : 0.0;
+          const double xae = xe[J] - mean;
+          double v;
+          if (polar) {
+            v = xbe + ab4[r];
+            sa += xbe * xbe;
+          } else {
+            v = p.gamma1 * (ab4[r] + xbe) + p.gamma2 * (mean + xae);
+            if (e < k) sa += (v - mean) * (v - mean);
+          }
+          xo[J] = v;
+          sb += xbe * xbe;
         }
-        if (i16 == 0) p.axa[o] = mean;
+#pragma unroll
+        for (int o2 = 1; o2 < 16; o2 <<= 1) {
+          sb += __shfl_xor(sb, o2, 64);
+          sa += __shfl_xor(sa, o2, 64);
+        }
+        if (st4[r] >= 0) {
+#pragma unroll
+          for (int J = 0; J < TT; ++J) {
+            const int e = 16 * J + i16;
+            if (e < k) p.xa_out[o * k + e] = xo[J];
+          }
+          if (i16 == 0) {
+            const long long so = spread_index(p, lo_c, la_c, st4[r]);
+            p.spread_b[so] = sqrt(sb / (double)(float)k);
+            p.spread_a[so] = sqrt(sa / (double)(float)k);
+          }
+        }
       }
     }
   }
@@ -1288,7 +1397,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (f < k && e < k) tr[(long long)e * k + f] = acc[I][J][rr];
           }
     }
-    if (p.xb) {
+    if (p.xb || p.xb32) {
       __syncthreads();                             // every read of the reflector block is done
 #pragma unroll
       for (int I = 0; I < 4; ++I)
@@ -1303,40 +1412,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // Xa = Xb trans for k = 65..128: wave w produces members 32w .. 32w+31 from the
-// trans tiles J in {2w, 2w+1}; 16 state vectors per MFMA group.
+// trans tiles J in {2w, 2w+1}; 16 state vectors per MFMA group.  TS and EPI
+// as k_update.
+template <class TS, int EPI>
 __global__ __launch_bounds__(256) void k_update128(EigParams p) {
+  using ST = StateT<TS>;
+  using acc_t = typename ST::acc;
   constexpr int KMAX = 128, LDV = KMAX + 1, NT = 256;
-  __shared__ double sV[16 * LDV];
-  __shared__ double spart[4][16];
+  __shared__ TS sV[16 * LDV];
+  __shared__ double spart[3][4][16];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   long long a, ai, ar;
   if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
   if (p.status[a] != 0) {                // redone by the list pass, or failed
-    if (p.list || p.status[a] == kStatusNotRedone) copy_background(p, a);
+    if (p.list || p.status[a] == kStatusNotRedone) copy_background_t<TS, EPI>(p, a);
     return;
   }
   const int k = p.k, kp = p.kp;
   const int i16 = lane & 15, q = lane >> 4;
+  // the row of a 16x16 MFMA result that register rr of lane quarter q holds:
+  // q + 4 rr for f64 16x16x4, 4 q + rr for f32 16x16x4
+  constexpr bool F32 = sizeof(TS) == 4;
+  auto out_row = [q](int rr) { return F32 ? 4 * q + rr : q + 4 * rr; };
+  const TS* __restrict__ xb = ST::xb(p);
   const double* __restrict__ tb = p.refl + ai * (long long)kp * kp;
-  d4 bop[8][2];
+  TS bop[8][2][4];
 #pragma unroll
   for (int I = 0; I < 8; ++I)
 #pragma unroll
     for (int J = 0; J < 2; ++J)
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) bop[I][J][rr] = tb[tile_off<KMAX>(I, 2 * wv + J, rr, lane)];
+      for (int rr = 0; rr < 4; ++rr) bop[I][J][rr] = (TS)tb[tile_off<KMAX>(I, 2 * wv + J, rr, lane)];
 
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
-  const long long lat_ext = p.g.nlat + 2 * p.halo;
-  int sk = 1, ek = p.nz;
-  if (p.vertical) {
-    sk = kc * p.cube_nz + 1;
-    ek = min(sk + p.cube_nz - 1, p.nz);
-  }
+  const AnRows R = an_rows(p, a);
+  const int sj = R.sj, ej = R.ej, si = R.si, ei = R.ei, sk = R.sk, ek = R.ek;
+  const long long lat_ext = R.lat_ext;
   int rs[kMaxVars], rl[kMaxVars];
   int nrun = 0, nvec = 0;
   if (!p.vertical) {
@@ -1363,58 +1474,114 @@ __global__ __launch_bounds__(256) void k_update128(EigParams p) {
   for (int lo = si; lo <= ei; ++lo) {
     for (int la = sj; la <= ej; ++la) {
       const long long off = ((long long)(lo - 1 + p.halo) * lat_ext + (la - 1 + p.halo)) * p.n;
+      const bool polar = EPI && fabs((double)p.g.glat[la - 1]) > p.polar_lat;
       for (int v0 = 0; v0 < nvec; v0 += 16) {
         __syncthreads();
         for (int e = tid; e < 16 * KMAX; e += NT) {
           const int u = e / KMAX, m = e % KMAX;
           const int s_i = (v0 + u < nvec) ? sidx(v0 + u) : -1;
-          sV[u * LDV + m] = (s_i >= 0 && m < k) ? p.xb[(off + s_i) * k + m] : 0.0;
+          sV[u * LDV + m] = (s_i >= 0 && m < k) ? xb[(off + s_i) * k + m] : (TS)0;
         }
         __syncthreads();
-        d4 out[2];
+        acc_t out[2];
 #pragma unroll
-        for (int J = 0; J < 2; ++J) out[J] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int J = 0; J < 2; ++J) out[J] = acc_t{0, 0, 0, 0};
 #pragma unroll
         for (int sK = 0; sK < KMAX / 4; ++sK) {
-          const double xa_frag = sV[i16 * LDV + 4 * sK + q];
+          const TS xa_frag = sV[i16 * LDV + 4 * sK + q];
 #pragma unroll
-          for (int J = 0; J < 2; ++J)
-            out[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa_frag, bop[sK >> 2][J][sK & 3],
-                                                          out[J], 0, 0, 0);
+          for (int J = 0; J < 2; ++J) out[J] = ST::mma(xa_frag, bop[sK >> 2][J][sK & 3], out[J]);
         }
-        // lane holds Xa[vector q + 4rr][member 32w + 16J + i16]
+        // lane holds Xa[vector out_row(rr)][member 32w + 16J + i16]
         double ab4[4], xe[4][2];
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int v = v0 + q + 4 * rr;
+          const int v = v0 + out_row(rr);
           const int s_i = (v < nvec) ? sidx(v) : -1;
           ab4[rr] = (s_i >= 0) ? p.axb[off + s_i] : 0.0;
           double sum = 0.0;
 #pragma unroll
           for (int J = 0; J < 2; ++J) {
             const int e = 32 * wv + 16 * J + i16;
-            xe[rr][J] = (e < k) ? out[J][rr] + ab4[rr] : 0.0;
+            xe[rr][J] = (e < k) ? (double)out[J][rr] + ab4[rr] : 0.0;
             sum += xe[rr][J];
           }
 #pragma unroll
           for (int o = 1; o < 16; o <<= 1) sum += __shfl_xor(sum, o, 64);
-          if (i16 == 0) spart[wv][q + 4 * rr] = sum;
+          if (i16 == 0) spart[0][wv][out_row(rr)] = sum;
         }
         __syncthreads();
+        if (!EPI) {
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int v = v0 + q + 4 * rr;
-          const int s_i = (v < nvec) ? sidx(v) : -1;
-          if (s_i < 0) continue;
-          const int vi = q + 4 * rr;
-          const double mean = ((spart[0][vi] + spart[1][vi]) + (spart[2][vi] + spart[3][vi])) / k;
-          const long long o = off + s_i;
+          for (int rr = 0; rr < 4; ++rr) {
+            const int v = v0 + out_row(rr);
+            const int s_i = (v < nvec) ? sidx(v) : -1;
+            if (s_i < 0) continue;
+            const int vi = out_row(rr);
+            const double mean =
+                ((spart[0][0][vi] + spart[0][1][vi]) + (spart[0][2][vi] + spart[0][3][vi])) / k;
+            const long long o = off + s_i;
+            TS* __restrict__ xa = ST::xa(p);
 #pragma unroll
-          for (int J = 0; J < 2; ++J) {
-            const int e = 32 * wv + 16 * J + i16;
-            if (e < k) p.xa[o * k + e] = xe[rr][J] - mean;
+            for (int J = 0; J < 2; ++J) {
+              const int e = 32 * wv + 16 * J + i16;
+              if (e < k) xa[o * k + e] = (TS)(xe[rr][J] - mean);
+            }
+            if (wv == 0 && i16 == 0) p.axa[o] = mean;
           }
-          if (wv == 0 && i16 == 0) p.axa[o] = mean;
+        } else {
+          // the caller's epilogue (module_da_parallel.F90:219-261); the
+          // spreads' member sums go through LDS like the mean
+          int sidx4[4];
+          double mean4[4];
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int v = v0 + out_row(rr);
+            sidx4[rr] = (v < nvec) ? sidx(v) : -1;
+            const int vi = out_row(rr);
+            mean4[rr] =
+                ((spart[0][0][vi] + spart[0][1][vi]) + (spart[0][2][vi] + spart[0][3][vi])) / k;
+            const long long o = off + (sidx4[rr] >= 0 ? sidx4[rr] : 0);
+            double sb = 0.0, sa = 0.0;
+#pragma unroll
+            for (int J = 0; J < 2; ++J) {
+              const int e = 32 * wv + 16 * J + i16;
+              const bool in = e < k && sidx4[rr] >= 0;
+              const double xbe = in ? (double)xb[o * k + e] : 0.0;
+              double vv;
+              if (polar) {
+                vv = xbe + ab4[rr];
+                sa += xbe * xbe;
+              } else {
+                vv = p.gamma1 * (ab4[rr] + xbe) + p.gamma2 * (mean4[rr] + (xe[rr][J] - mean4[rr]));
+                if (e < k) sa += (vv - mean4[rr]) * (vv - mean4[rr]);
+              }
+              sb += xbe * xbe;
+              if (in) p.xa_out[o * k + e] = vv;
+            }
+#pragma unroll
+            for (int o2 = 1; o2 < 16; o2 <<= 1) {
+              sb += __shfl_xor(sb, o2, 64);
+              sa += __shfl_xor(sa, o2, 64);
+            }
+            if (i16 == 0) {
+              spart[1][wv][vi] = sb;
+              spart[2][wv][vi] = sa;
+            }
+          }
+          __syncthreads();
+          if (wv == 0 && i16 == 0) {
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+              if (sidx4[rr] < 0) continue;
+              const int vi = out_row(rr);
+              const double sb = (spart[1][0][vi] + spart[1][1][vi]) + (spart[1][2][vi] + spart[1][3][vi]);
+              const double sa = (spart[2][0][vi] + spart[2][1][vi]) + (spart[2][2][vi] + spart[2][3][vi]);
+              const long long so = spread_index(p, lo, la, sidx4[rr]);
+              p.spread_b[so] = sqrt(sb / (double)(float)k);
+              p.spread_a[so] = sqrt(sa / (double)(float)k);
+            }
+          }
         }
       }
     }
@@ -1505,6 +1672,31 @@ __global__ __launch_bounds__(KMAX < 64 ? 64 : KMAX) void k_tridiag_lds(EigParams
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// true when the batch updates a state (not only transforms)
+static bool has_state(const EigParams& p) { return p.xb || p.xb32; }
+
+// the update kernel for the batch's width, state precision and output mode
+template <int KMAX>
+static void launch_update(hipStream_t st, const EigParams& p, unsigned nb) {
+  if constexpr (KMAX == 128) {
+    if (p.state_f32) {
+      if (p.epi) hipLaunchKernelGGL((k_update128<float, 1>), dim3(nb), dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_update128<float, 0>), dim3(nb), dim3(256), 0, st, p);
+    } else {
+      if (p.epi) hipLaunchKernelGGL((k_update128<double, 1>), dim3(nb), dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_update128<double, 0>), dim3(nb), dim3(256), 0, st, p);
+    }
+  } else {
+    if (p.state_f32) {
+      if (p.epi) hipLaunchKernelGGL((k_update<KMAX, float, 1>), dim3(nb), dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_update<KMAX, float, 0>), dim3(nb), dim3(256), 0, st, p);
+    } else {
+      if (p.epi) hipLaunchKernelGGL((k_update<KMAX, double, 1>), dim3(nb), dim3(256), 0, st, p);
+      else hipLaunchKernelGGL((k_update<KMAX, double, 0>), dim3(nb), dim3(256), 0, st, p);
+    }
+  }
+}
+
 hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
   if (p.n_an <= 0) return hipSuccess;
   const unsigned nb = (unsigned)p.n_an;
@@ -1520,7 +1712,7 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(3)) != hipSuccess) return e;
-    if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
+    if (has_state(p)) launch_update<32>(st, p, nb);
   } else if (p.k <= 64) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<64>, dim3(nb), dim3(64), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag<64>, dim3(nb), dim3(64), 0, st, p);
@@ -1529,7 +1721,7 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
     if ((e = mark(3)) != hipSuccess) return e;
-    if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
+    if (has_state(p)) launch_update<64>(st, p, nb);
   } else if (p.k <= 128) {
     if (p.tridiag_lds) hipLaunchKernelGGL(k_tridiag_lds<128>, dim3(nb), dim3(128), 0, st, p);
     else hipLaunchKernelGGL(k_tridiag128, dim3(nb), dim3(256), 0, st, p);
@@ -1538,7 +1730,7 @@ hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev) {
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
     if ((e = mark(3)) != hipSuccess) return e;
-    if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+    if (has_state(p)) launch_update<128>(st, p, nb);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1581,9 +1773,8 @@ static void launch_eig_stage_k(hipStream_t st, const EigParams& p, int stage) {
   } else if (stage == 2) {
     if constexpr (KMAX == 128) hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
     else hipLaunchKernelGGL(k_vec<KMAX>, dim3(nb), dim3(64), 0, st, p);
-  } else if (p.xb) {
-    if constexpr (KMAX == 128) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
-    else hipLaunchKernelGGL(k_update<KMAX>, dim3(nb), dim3(256), 0, st, p);
+  } else if (has_state(p)) {
+    launch_update<KMAX>(st, p, nb);
   }
 }
 
@@ -1611,15 +1802,15 @@ hipError_t launch_eig_rerun(hipStream_t st, const EigParams& p) {
     if (p.k <= 32) {
       if (stage == 1) hipLaunchKernelGGL(k_ql_log<32>, dim3(nq), dim3(64), 0, st, p);
       else if (stage == 2) hipLaunchKernelGGL(k_vec<32>, dim3(nb), dim3(64), 0, st, p);
-      else if (p.xb) hipLaunchKernelGGL(k_update<32>, dim3(nb), dim3(256), 0, st, p);
+      else if (has_state(p)) launch_update<32>(st, p, nb);
     } else if (p.k <= 64) {
       if (stage == 1) hipLaunchKernelGGL(k_ql_log<64>, dim3(nq), dim3(64), 0, st, p);
       else if (stage == 2) hipLaunchKernelGGL(k_vec<64>, dim3(nb), dim3(64), 0, st, p);
-      else if (p.xb) hipLaunchKernelGGL(k_update<64>, dim3(nb), dim3(256), 0, st, p);
+      else if (has_state(p)) launch_update<64>(st, p, nb);
     } else {
       if (stage == 1) hipLaunchKernelGGL(k_ql_log<128>, dim3(nq), dim3(64), 0, st, p);
       else if (stage == 2) hipLaunchKernelGGL(k_vec128, dim3(nb), dim3(256), 0, st, p);
-      else if (p.xb) hipLaunchKernelGGL(k_update128, dim3(nb), dim3(256), 0, st, p);
+      else if (has_state(p)) launch_update<128>(st, p, nb);
     }
   }
   return hipGetLastError();

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,12 +58,17 @@ namespace {
     if (rc_ < 0) return rc_;    \
   } while (0)
 
-// Growable device buffer.
+// Growable device buffer, or (wrap) a fixed piece of memory it does not own:
+// ensure() then succeeds only within the piece and never allocates.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   int ensure(size_t n) {
     if (n <= bytes && p) return DIDA_OK;
+    if (!owned)
+      return set_error(DIDA_ERR_ARG, "fixed scratch piece of %zu bytes is too small (%zu needed)",
+                       bytes, n);
     if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
     if (n == 0) n = 16;
     hipError_t e = hipMalloc(&p, n);
@@ -73,10 +79,16 @@ struct DevBuf {
     bytes = n;
     return DIDA_OK;
   }
+  void wrap(void* q, size_t n) {
+    p = q;
+    bytes = n;
+    owned = false;
+  }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -97,7 +109,7 @@ int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 
 // Scratch of the split eigensolver for one batch of analyses.
 struct EigScratch {
-  DevBuf refl, tau, d, e, lam, rot, hdr, nchase, list;
+  DevBuf refl, tau, d, e, lam, rot, hdr, nchase;
   // flags: [0] analyses the batch's first pass flagged, [1] the same summed
   // over the batches of a call, [2 ..] their list; rr_*: the list pass's own
   // scratch (its longer rotation log, chase headers, eigenvalues)
@@ -253,6 +265,16 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   ep.axa = xp.axa;
   ep.trans_out = xp.trans_out;
   ep.w_out = xp.w_out;
+  ep.state_f32 = xp.state_f32;
+  ep.xb32 = xp.xb32;
+  ep.xa32 = xp.xa32;
+  ep.epi = xp.epi;
+  ep.polar_lat = xp.polar_lat;
+  ep.gamma1 = xp.gamma1;
+  ep.gamma2 = xp.gamma2;
+  ep.xa_out = xp.xa_out;
+  ep.spread_b = xp.spread_b;
+  ep.spread_a = xp.spread_a;
   return DIDA_OK;
 }
 
@@ -294,7 +316,8 @@ int rerun_flagged(hipStream_t st, EigScratch& s, const EigParams& ep) {
 int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode,
                   hipEvent_t* kev = nullptr, bool* split = nullptr) {
   if (split) *split = false;
-  if (mode == 1 && xp.k <= 64) {
+  // the single-kernel transform updates an FP64 state only
+  if (mode == 1 && xp.k <= 64 && !xp.state_f32 && !xp.epi) {
     HIP_TRY(launch_transform(st, xp));
     return DIDA_OK;
   }
@@ -488,7 +511,7 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->row_site, &h->nobsl_dev, &h->row_wd,
                     &h->Abuf, &h->bbuf, &h->status, &h->trans, &h->wout, &h->x_xb, &h->x_axb,
                     &h->x_xa, &h->x_axa, &h->eig.refl, &h->eig.tau, &h->eig.d, &h->eig.e,
-                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase, &h->eig.list,
+                    &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase,
                     &h->eig.flags, &h->eig.rr_rot, &h->eig.rr_hdr, &h->eig.rr_nchase,
                     &h->eig.rr_lam};
   for (DevBuf* b : bufs) b->release();
@@ -662,8 +685,24 @@ int dida_letkf_set_obs_device(dida_letkf_handle* h, int n_sites, const double* l
 }  // extern "C"
 
 // Shared analysis driver.  xb etc. are device pointers.
-static int run_analysis(dida_letkf_handle* h, const double* xb, const double* axb, double* xa,
-                        double* axa, dida_letkf_stats* stats) {
+// The state an analysis call reads and writes: FP64 or FP32 perturbations,
+// FP64 means, and either xa/axa or the fused caller epilogue's outputs.
+struct StateArgs {
+  const double* xb = nullptr;
+  const float* xb32 = nullptr;
+  const double* axb = nullptr;
+  double* xa = nullptr;
+  float* xa32 = nullptr;
+  double* axa = nullptr;
+  int epi = 0;
+  double polar_lat = 90.0, gamma1 = 0.0, gamma2 = 1.0;
+  double* xa_out = nullptr;
+  double* spread_b = nullptr;
+  double* spread_a = nullptr;
+};
+
+static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_stats* stats) {
+  const double* axb = sa.axb;
   const auto& c = h->cfg;
   hipStream_t st = h->stream;
   if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
@@ -878,10 +917,20 @@ static int run_analysis(dida_letkf_handle* h, const double* xb, const double* ax
         cur += c.nz;
       }
     }
-    xp.xb = xb;
+    xp.xb = sa.xb;
     xp.axb = axb;
-    xp.xa = xa;
-    xp.axa = axa;
+    xp.xa = sa.xa;
+    xp.axa = sa.axa;
+    xp.state_f32 = sa.xb32 != nullptr;
+    xp.xb32 = sa.xb32;
+    xp.xa32 = sa.xa32;
+    xp.epi = sa.epi;
+    xp.polar_lat = sa.polar_lat;
+    xp.gamma1 = sa.gamma1;
+    xp.gamma2 = sa.gamma2;
+    xp.xa_out = sa.xa_out;
+    xp.spread_b = sa.spread_b;
+    xp.spread_a = sa.spread_a;
     xp.trans_out = c.keep_transforms ? h->trans.as<double>() : nullptr;
     xp.w_out = c.keep_transforms ? h->wout.as<double>() : nullptr;
     xp.status = h->status.as<int>();
@@ -950,7 +999,65 @@ extern "C" {
 int dida_letkf_analyze_device(dida_letkf_handle* h, const double* xb, const double* axb,
                               double* xa, double* axa, dida_letkf_stats* stats) {
   if (!h || !xb || !axb || !xa || !axa) return set_error(DIDA_ERR_ARG, "null argument");
-  return run_analysis(h, xb, axb, xa, axa, stats);
+  StateArgs sa;
+  sa.xb = xb;
+  sa.axb = axb;
+  sa.xa = xa;
+  sa.axa = axa;
+  return run_analysis(h, sa, stats);
+}
+
+int dida_letkf_analyze_device_f32(dida_letkf_handle* h, const float* xb, const double* axb,
+                                  float* xa, double* axa, dida_letkf_stats* stats) {
+  if (!h || !xb || !axb || !xa || !axa) return set_error(DIDA_ERR_ARG, "null argument");
+  StateArgs sa;
+  sa.xb32 = xb;
+  sa.axb = axb;
+  sa.xa32 = xa;
+  sa.axa = axa;
+  return run_analysis(h, sa, stats);
+}
+
+// analysis + fused epilogue; xb_f32 selects which of xb / xb32 is the state
+static int analyze_epilogue(dida_letkf_handle* h, const double* xb, const float* xb32,
+                            const double* axb, double polar_lat, double gamma1, double gamma2,
+                            double* xa_out, double* spread_b, double* spread_a,
+                            dida_letkf_stats* stats) {
+  if (!h || (!xb && !xb32) || !axb || !xa_out || !spread_b || !spread_a)
+    return set_error(DIDA_ERR_ARG, "null argument");
+  const auto& c = h->cfg;
+  StateArgs sa;
+  sa.xb = xb;
+  sa.xb32 = xb32;
+  sa.axb = axb;
+  sa.epi = 1;
+  sa.polar_lat = polar_lat;
+  sa.gamma1 = gamma1;
+  sa.gamma2 = gamma2;
+  sa.xa_out = xa_out;
+  sa.spread_b = spread_b;
+  sa.spread_a = spread_a;
+  // the halo columns of xa_out are zero (module_da_parallel.F90:222-223)
+  HIP_TRY(launch_zero_halo(h->stream, xa_out, c.nlat, c.nlon, c.halo, (long long)h->n * c.k));
+  return run_analysis(h, sa, stats);
+}
+
+int dida_letkf_analyze_epilogue_device(dida_letkf_handle* h, const double* xb, const double* axb,
+                                       double polar_lat, double gamma1, double gamma2,
+                                       double* xa_out, double* spread_b, double* spread_a,
+                                       dida_letkf_stats* stats) {
+  if (!xb) return set_error(DIDA_ERR_ARG, "null argument");
+  return analyze_epilogue(h, xb, nullptr, axb, polar_lat, gamma1, gamma2, xa_out, spread_b,
+                          spread_a, stats);
+}
+
+int dida_letkf_analyze_epilogue_device_f32(dida_letkf_handle* h, const float* xb,
+                                           const double* axb, double polar_lat, double gamma1,
+                                           double gamma2, double* xa_out, double* spread_b,
+                                           double* spread_a, dida_letkf_stats* stats) {
+  if (!xb) return set_error(DIDA_ERR_ARG, "null argument");
+  return analyze_epilogue(h, nullptr, xb, axb, polar_lat, gamma1, gamma2, xa_out, spread_b,
+                          spread_a, stats);
 }
 
 int dida_letkf_analyze(dida_letkf_handle* h, const double* xb, const double* axb, double* xa,
@@ -969,8 +1076,12 @@ int dida_letkf_analyze(dida_letkf_handle* h, const double* xb, const double* axb
   // outputs start as the caller's arrays so that halo entries are left as they are
   HIP_TRY(hipMemcpyAsync(h->x_xa.p, xa, nx * 8, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(h->x_axa.p, axa, ncol * 8, hipMemcpyHostToDevice, st));
-  int rc = run_analysis(h, h->x_xb.as<double>(), h->x_axb.as<double>(), h->x_xa.as<double>(),
-                        h->x_axa.as<double>(), stats);
+  StateArgs sa;
+  sa.xb = h->x_xb.as<double>();
+  sa.axb = h->x_axb.as<double>();
+  sa.xa = h->x_xa.as<double>();
+  sa.axa = h->x_axa.as<double>();
+  int rc = run_analysis(h, sa, stats);
   if (rc < 0) return rc;
   HIP_TRY(hipMemcpyAsync(xa, h->x_xa.p, nx * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(axa, h->x_axa.p, ncol * 8, hipMemcpyDeviceToHost, st));
@@ -995,10 +1106,58 @@ int dida_letkf_get_transforms(dida_letkf_handle* h, long long n_an, double* tran
   return DIDA_OK;
 }
 
-int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, const double* hxens,
-                            const float* rdiaginv, const double* dep, const double* rloc,
-                            double* trans, double* overline_w, double inflation, int precision,
-                            int* status) {
+}  // extern "C"
+
+namespace {
+
+// Device buffers of one batched letkf_core call: inputs staged from the host
+// (Yb, rdiaginv, dep, rloc, the row list and offsets, the FP32 copy of Yb for
+// the reduced modes), the products A and b, and the outputs.
+struct CoreBufs {
+  DevBuf y, r, d, g, idx, off, A, b, t, w, s, y32;
+};
+
+// Every device buffer the call needs with its size in bytes, in a fixed
+// order: the core buffers, then the eigensolver's (as prepare_eig,
+// eig_begin_call and rerun_flagged request them).
+int core_layout(CoreBufs& cb, EigScratch& es, int k, long long n_an, long long nr, int precision,
+                DevBuf** bufs, size_t* sizes) {
+  const size_t kp = kp_of(k), n = n_an, nrr = std::max(nr, 1LL);
+  const size_t cap = rerun_cap_of(k, n_an);
+  int c = 0;
+  auto add = [&](DevBuf& b, size_t bytes) { bufs[c] = &b; sizes[c] = bytes; ++c; };
+  add(cb.y, 8 * nrr * k);
+  add(cb.r, 4 * nrr);
+  add(cb.d, 8 * nrr);
+  add(cb.g, 8 * nrr);
+  add(cb.idx, 4 * nrr);
+  add(cb.off, 8 * (n + 1));
+  add(cb.A, 8 * n * kp * kp);
+  add(cb.b, 8 * n * kp);
+  add(cb.t, 8 * n * k * k);
+  add(cb.w, 8 * n * k);
+  add(cb.s, 4 * n);
+  if (precision != DIDA_PREC_FP64) add(cb.y32, 4 * nrr * k);
+  add(es.refl, 8 * n * kp * kp);
+  add(es.tau, 8 * n * kp);
+  add(es.d, 8 * n * kp);
+  add(es.e, 8 * n * kp);
+  add(es.lam, 8 * n * kp);
+  add(es.rot, 16 * n * (size_t)rot_cap_of(k));
+  add(es.hdr, 16 * n * (size_t)(kHdrCapPerK * k));
+  add(es.nchase, 4 * n);
+  add(es.flags, 4 * (2 + cap));
+  add(es.rr_rot, 16 * cap * (size_t)(2 * rot_cap_of(k)));
+  add(es.rr_hdr, 16 * cap * (size_t)(2 * kHdrCapPerK * k));
+  add(es.rr_nchase, 4 * cap);
+  add(es.rr_lam, 8 * cap * kp);
+  return c;
+}
+constexpr int kCoreMaxBufs = 32;
+
+int core_check(int k, long long n_an, const long long* row_ptr, const double* hxens,
+               const float* rdiaginv, const double* dep, const double* rloc, double* trans,
+               double* overline_w, double inflation, int precision) {
   if (k < 2 || k > 128) return set_error(DIDA_ERR_UNSUPPORTED, "k=%d: supported 2..128", k);
   if (n_an < 0 || !row_ptr || !trans || !overline_w)
     return set_error(DIDA_ERR_ARG, "bad arguments");
@@ -1009,57 +1168,41 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   if (row_ptr[0] != 0) return set_error(DIDA_ERR_ARG, "row_ptr[0] must be 0");
   for (long long a = 0; a < n_an; ++a)
     if (row_ptr[a + 1] < row_ptr[a]) return set_error(DIDA_ERR_ARG, "row_ptr decreasing");
-  const long long nr = row_ptr[n_an];
-  if (nr > 0 && (!hxens || !rdiaginv || !dep || !rloc))
+  if (row_ptr[n_an] > 0 && (!hxens || !rdiaginv || !dep || !rloc))
     return set_error(DIDA_ERR_ARG, "null input arrays");
+  return DIDA_OK;
+}
+
+// n_an independent letkf_core calls on st: stage the inputs, products,
+// eigensolver (mode as run_transform), outputs back to the host.  The buffers
+// of cb and es are sized here (a no-op for pieces they already hold).
+int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long long n_an,
+             const long long* row_ptr, const double* hxens, const float* rdiaginv,
+             const double* dep, const double* rloc, double* trans, double* overline_w,
+             double inflation, int precision, int* status) {
+  const long long nr = row_ptr[n_an];
   const int kp = kp_of(k);
-  hipStream_t st;
-  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  struct S {
-    hipStream_t s;
-    std::vector<DevBuf*> b;
-    ~S() {
-      (void)hipStreamSynchronize(s);
-      for (auto* x : b) { x->release(); delete x; }
-      (void)hipStreamDestroy(s);
-    }
-  } scope{st, {}};
-  auto mk = [&](size_t bytes, DevBuf** out) -> int {
-    DevBuf* b = new DevBuf();
-    scope.b.push_back(b);
-    *out = b;
-    return b->ensure(bytes);
-  };
-  const long long nrr = std::max(nr, 1LL);
-  DevBuf *d_y, *d_r, *d_d, *d_g, *d_idx, *d_off, *d_A, *d_b, *d_t, *d_w, *d_s, *d_y32 = nullptr;
-  RC_TRY(mk(sizeof(double) * nrr * k, &d_y));
-  RC_TRY(mk(sizeof(float) * nrr, &d_r));
-  RC_TRY(mk(sizeof(double) * nrr, &d_d));
-  RC_TRY(mk(sizeof(double) * nrr, &d_g));
-  RC_TRY(mk(sizeof(int) * nrr, &d_idx));
-  RC_TRY(mk(sizeof(long long) * (n_an + 1), &d_off));
-  RC_TRY(mk(sizeof(double) * n_an * kp * kp, &d_A));
-  RC_TRY(mk(sizeof(double) * n_an * kp, &d_b));
-  RC_TRY(mk(sizeof(double) * n_an * k * k, &d_t));
-  RC_TRY(mk(sizeof(double) * n_an * k, &d_w));
-  RC_TRY(mk(sizeof(int) * n_an, &d_s));
+  DevBuf* bufs[kCoreMaxBufs];
+  size_t sizes[kCoreMaxBufs];
+  const int nb = core_layout(cb, es, k, n_an, nr, precision, bufs, sizes);
+  // the core buffers; the eigensolver sizes its own
+  for (int i = 0; i < nb; ++i)
+    if (bufs[i] == &es.refl) break;
+    else RC_TRY(bufs[i]->ensure(sizes[i]));
   if (nr > 0) {
-    HIP_TRY(hipMemcpyAsync(d_y->p, hxens, sizeof(double) * nr * k, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_r->p, rdiaginv, sizeof(float) * nr, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_d->p, dep, sizeof(double) * nr, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_g->p, rloc, sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(cb.y.p, hxens, sizeof(double) * nr * k, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(cb.r.p, rdiaginv, sizeof(float) * nr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(cb.d.p, dep, sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(cb.g.p, rloc, sizeof(double) * nr, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, nr,
-                       d_idx->as<int>());
+                       cb.idx.as<int>());
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipMemcpyAsync(d_off->p, row_ptr, sizeof(long long) * (n_an + 1), hipMemcpyHostToDevice, st));
-  if (precision != DIDA_PREC_FP64) {
-    RC_TRY(mk(sizeof(float) * nrr * k, &d_y32));
-    if (nr > 0) {
-      hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((nr * k + 255) / 256)), dim3(256), 0, st,
-                         nr * k, d_y->as<double>(), d_y32->as<float>());
-      HIP_TRY(hipGetLastError());
-    }
+  HIP_TRY(hipMemcpyAsync(cb.off.p, row_ptr, sizeof(long long) * (n_an + 1), hipMemcpyHostToDevice, st));
+  if (precision != DIDA_PREC_FP64 && nr > 0) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((nr * k + 255) / 256)), dim3(256), 0, st,
+                       nr * k, cb.y.as<double>(), cb.y32.as<float>());
+    HIP_TRY(hipGetLastError());
   }
   Geom g{};
   g.nlat = 1; g.nlon = 1; g.cube_n = 1; g.nlat_c = 1; g.nlon_c = 1;
@@ -1069,18 +1212,18 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   pp.kp = kp;
   pp.nzc = 1;
   pp.vertical = 0;
-  pp.row_idx = d_idx->as<int>();
-  pp.row_gc = d_g->as<double>();
-  pp.row_off = d_off->as<long long>();
-  pp.yb64 = d_y->as<double>();
-  pp.yb32 = d_y32 ? d_y32->as<float>() : nullptr;
-  pp.dep = d_d->as<double>();
-  pp.rinv32 = d_r->as<float>();
+  pp.row_idx = cb.idx.as<int>();
+  pp.row_gc = cb.g.as<double>();
+  pp.row_off = cb.off.as<long long>();
+  pp.yb64 = cb.y.as<double>();
+  pp.yb32 = precision != DIDA_PREC_FP64 ? cb.y32.as<float>() : nullptr;
+  pp.dep = cb.d.as<double>();
+  pp.rinv32 = cb.r.as<float>();
   pp.a0 = 0;
   pp.n_an = n_an;
   pp.h0 = 0;
-  pp.A = d_A->as<double>();
-  pp.b = d_b->as<double>();
+  pp.A = cb.A.as<double>();
+  pp.b = cb.b.as<double>();
   HIP_TRY(launch_products(st, pp, precision));
   XformParams xp{};
   xp.g = g;
@@ -1094,35 +1237,56 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   xp.n_an = n_an;
   xp.nzc = 1;
   xp.xb = nullptr;
-  xp.trans_out = d_t->as<double>();
-  xp.w_out = d_w->as<double>();
-  xp.status = d_s->as<int>();
+  xp.trans_out = cb.t.as<double>();
+  xp.w_out = cb.w.as<double>();
+  xp.status = cb.s.as<int>();
   xp.list = nullptr;
-  EigScratch es;
-  int rc_t = eig_begin_call(st, es, k, n_an);
-  if (rc_t == DIDA_OK) rc_t = run_transform(st, es, xp, 0);
-  DevBuf* ebufs[] = {&es.refl, &es.tau, &es.d, &es.e, &es.lam, &es.rot, &es.hdr, &es.nchase,
-                     &es.list, &es.flags, &es.rr_rot, &es.rr_hdr, &es.rr_nchase, &es.rr_lam};
-  if (rc_t < 0) {
-    (void)hipStreamSynchronize(st);
-    es.release_streams();
-    for (DevBuf* b : ebufs) b->release();
-    return rc_t;
-  }
-  HIP_TRY(hipStreamSynchronize(st));
-  es.release_streams();
-  for (DevBuf* b : ebufs) b->release();
-  HIP_TRY(hipMemcpyAsync(trans, d_t->p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(overline_w, d_w->p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost, st));
-  std::vector<int> sh(n_an);
-  HIP_TRY(hipMemcpyAsync(sh.data(), d_s->p, sizeof(int) * n_an, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  RC_TRY(eig_begin_call(st, es, k, n_an));
+  RC_TRY(run_transform(st, es, xp, mode));
+  HIP_TRY(hipMemcpyAsync(trans, cb.t.p, sizeof(double) * n_an * k * k, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(overline_w, cb.w.p, sizeof(double) * n_an * k, hipMemcpyDeviceToHost, st));
+  // status in chunks through a stack buffer when the caller passes none
   int flagged = 0;
-  for (long long a = 0; a < n_an; ++a) {
-    flagged += sh[a] != 0;
-    if (status) status[a] = sh[a];
+  int chunk[1024];
+  for (long long a0 = 0; a0 < n_an; a0 += 1024) {
+    const long long c = std::min<long long>(1024, n_an - a0);
+    int* dst = status ? status + a0 : chunk;
+    HIP_TRY(hipMemcpyAsync(dst, cb.s.as<int>() + a0, sizeof(int) * c, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (long long i = 0; i < c; ++i) flagged += dst[i] != 0;
   }
   return flagged;
+}
+
+void release_core(CoreBufs& cb, EigScratch& es) {
+  DevBuf* bufs[kCoreMaxBufs];
+  size_t sizes[kCoreMaxBufs];
+  const int nb = core_layout(cb, es, 2, 1, 1, DIDA_PREC_BF16X3, bufs, sizes);
+  for (int i = 0; i < nb; ++i) bufs[i]->release();
+}
+
+}  // namespace
+
+extern "C" {
+
+int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, const double* hxens,
+                            const float* rdiaginv, const double* dep, const double* rloc,
+                            double* trans, double* overline_w, double inflation, int precision,
+                            int* status) {
+  RC_TRY(core_check(k, n_an, row_ptr, hxens, rdiaginv, dep, rloc, trans, overline_w, inflation,
+                    precision));
+  if (n_an == 0) return DIDA_OK;
+  hipStream_t st;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  CoreBufs cb;
+  EigScratch es;
+  const int rc = core_run(st, cb, es, 0, k, n_an, row_ptr, hxens, rdiaginv, dep, rloc, trans,
+                          overline_w, inflation, precision, status);
+  (void)hipStreamSynchronize(st);
+  es.release_streams();
+  release_core(cb, es);
+  (void)hipStreamDestroy(st);
+  return rc;
 }
 
 void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,

@@ -125,6 +125,15 @@ struct XformParams {
   double* w_out;             // optional, k per analysis
   int* status;               // optional, per analysis
   const int* list;           // optional: analyses to run (global indices)
+  // FP32 state and fused epilogue, as in EigParams (split eigensolver only)
+  int state_f32;
+  const float* xb32;
+  float* xa32;
+  int epi;
+  double polar_lat, gamma1, gamma2;
+  double* xa_out;
+  double* spread_b;
+  double* spread_a;
 };
 
 struct EigParams {
@@ -170,6 +179,20 @@ struct EigParams {
   double* axa;
   double* trans_out;        // optional, global index
   double* w_out;            // optional, global index
+  // FP32 state (state_f32 != 0): the perturbations are xb32/xa32 (member
+  // fastest, as xb/xa); the means stay FP64 in axb/axa
+  int state_f32;
+  const float* xb32;
+  float* xa32;
+  // Fused caller epilogue (epi != 0, module_da_parallel.F90:219-261): instead
+  // of xa/axa the update writes xa_out = full members
+  // gamma1 (axb + xb) + gamma2 (axa + xa) (xb + axb where |lat| > polar_lat)
+  // and spread_b/spread_a [lon][lat][n]
+  int epi;
+  double polar_lat, gamma1, gamma2;
+  double* xa_out;
+  double* spread_b;
+  double* spread_a;
 };
 
 // observation operator + centring (gen_obs_struct)
@@ -226,6 +249,8 @@ hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);
 hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p);
 // halo of a [lon + 2h][lat + 2h][n] field with `inner` values per element
 hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);
+// zeroes the halo columns of such a field
+hipError_t launch_zero_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner);
 hipError_t launch_eig(hipStream_t st, const EigParams& p, hipEvent_t* ev = nullptr);
 // second pass over the analyses the first flagged (p.list / p.list_n / p.list_cap)
 hipError_t launch_eig_rerun(hipStream_t st, const EigParams& p);

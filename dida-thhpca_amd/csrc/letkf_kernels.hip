@@ -400,6 +400,13 @@ __device__ __forceinline__ long long analysis_at(const ProdParams& p, long long 
   return (b * B + rem % Bb) * L + rem / Bb;
 }
 
+// Reduced-precision products: row lists longer than kFlushMinRows add their
+// FP32 partial sums to FP64 every kFlushRows rows; at k > 64 lists longer
+// than kFp32MinRowsWide use FP32 instead of BF16 hi/lo products.
+constexpr int kFlushMinRows = 3072;
+constexpr int kFlushRows = 1024;
+constexpr int kFp32MinRowsWide = 4096;
+
 template <int T>
 struct PairTable {
   static constexpr int N = T * (T + 1) / 2;
@@ -578,7 +585,34 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
 #pragma unroll
     for (int g = 0; g < G; ++g) load(4 * g + q, yn[g], wfn[g], wdn[g]);
   }
+  // Long row lists: the FP32 sums of every kFlushRows rows are added to FP64
+  // (A in memory, b in registers), so the FP32 accumulation error grows with
+  // kFlushRows instead of p
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+  const bool chunked = n > kFlushMinRows;
+  bool flushed = false;
+  double bsum[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) bsum[t] = 0.0;
   for (int g0 = 0; g0 < n; g0 += 4 * G) {
+    if (chunked && g0 > 0 && g0 % kFlushRows == 0) {
+#pragma unroll
+      for (int x = 0; x < NP; ++x) {
+        const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double* e = A + (long long)(16 * J + i16) * KP + 16 * I + 4 * q + r;
+          *e = (flushed ? *e : 0.0) + (double)acc[x][r];
+        }
+        acc[x] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        bsum[t] += (double)bacc[t];
+        bacc[t] = 0.f;
+      }
+      flushed = true;
+    }
     float y[G][T], wf[G], wd[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -602,22 +636,22 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
       }
     }
   }
-  double* A = p.A + (a - p.a0) * (long long)KP * KP;
 #pragma unroll
   for (int x = 0; x < NP; ++x) {
     const int I = PT::I(P0 + x), J = PT::J(P0 + x);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
-      A[(long long)col * KP + row] = (double)acc[x][r];
-      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
+      const double v = (flushed ? A[(long long)col * KP + row] : 0.0) + (double)acc[x][r];
+      A[(long long)col * KP + row] = v;
+      if (I != J) A[(long long)row * KP + col] = v;
     }
   }
   if (WS == 0) {
     double* b = p.b + (a - p.a0) * KP;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      double v = bacc[t];
+      double v = bsum[t] + (double)bacc[t];
       v += __shfl_xor(v, 16, 64);
       v += __shfl_xor(v, 32, 64);
       if (q == 0) b[16 * t + i16] = v;
@@ -870,6 +904,18 @@ void k_products_bf16(ProdParams p) {
     products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
   } else {
     static_assert(WPA == 4 && KP % 64 == 0, "four wavefronts share an analysis");
+    // Long row lists at k > 64: the hi/lo split keeps 16 bits of each value,
+    // and with thousands of rows that error, amplified by the k = 65..128
+    // eigenvalue spread, passes 1e-5 of the analysis increment; these
+    // analyses take the FP32 products (24 bits, FP64 flushes).  Every
+    // wavefront of the block's one analysis takes the same branch.
+    if (r1 - r0 > kFp32MinRowsWide) {
+      if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+      return;
+    }
     __shared__ bf8 sh[2 * 2 * (KP / 16) * 64];
     if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
     if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);

@@ -9,6 +9,8 @@
 // continues from the previous row of the same site, as the reference's does
 // across the levels of one obs.  Finally Yb = H(x) - mean, d = yo - mean
 // (letkf_obs.F90:158-170).
+#include <algorithm>
+
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
@@ -153,6 +155,27 @@ __global__ void k_fill_halo(double* __restrict__ x, int nlat, int nlon, int halo
   const double* src = x + ((long long)ls * lat_ext + as) * inner;
   double* dst = x + ((long long)lo * lat_ext + la) * inner;
   for (long long i = threadIdx.x; i < inner; i += blockDim.x) dst[i] = src[i];
+}
+
+// Zeroes the halo columns of a [lon + 2h][lat + 2h][inner] field.
+__global__ void k_zero_halo(double* __restrict__ x, int nlat, int nlon, int halo, long long inner) {
+  const long long lat_ext = nlat + 2 * halo, lon_ext = nlon + 2 * halo;
+  const long long total = lat_ext * lon_ext * inner;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long col = i / inner;
+    const long long la = col % lat_ext, lo = col / lat_ext;
+    const bool interior = la >= halo && la < nlat + halo && lo >= halo && lo < nlon + halo;
+    if (!interior) x[i] = 0.0;
+  }
+}
+
+hipError_t launch_zero_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner) {
+  if (halo <= 0) return hipSuccess;
+  const long long total = (long long)(nlat + 2 * halo) * (nlon + 2 * halo) * inner;
+  const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_zero_halo, dim3(nb), dim3(256), 0, st, x, nlat, nlon, halo, inner);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_halo(hipStream_t st, double* x, int nlat, int nlon, int halo, long long inner) {

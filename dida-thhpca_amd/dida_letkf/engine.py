@@ -87,6 +87,8 @@ EXPORTS = [
     "dida_letkf_config_default", "dida_letkf_create", "dida_letkf_destroy",
     "dida_letkf_set_stream", "dida_letkf_set_grid", "dida_letkf_set_obs",
     "dida_letkf_set_obs_device", "dida_letkf_analyze", "dida_letkf_analyze_device",
+    "dida_letkf_analyze_device_f32", "dida_letkf_analyze_epilogue_device",
+    "dida_letkf_analyze_epilogue_device_f32",
     "dida_letkf_get_transforms", "dida_letkf_n_analyses", "dida_letkf_core_batched",
     "letkf_core_", "dida_letkf_epilogue_device", "dida_letkf_last_error",
     "dida_letkf_abi_version", "dida_serve_create", "dida_serve_submit", "dida_serve_poll",
@@ -121,6 +123,12 @@ def lib(path: Optional[str] = None):
                                             _vp]
     L.dida_letkf_analyze.argtypes = [_vp, _dp, _dp, _dp, _dp, C.POINTER(DidaStats)]
     L.dida_letkf_analyze_device.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(DidaStats)]
+    L.dida_letkf_analyze_device_f32.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(DidaStats)]
+    L.dida_letkf_analyze_epilogue_device.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double,
+                                                     C.c_double, _vp, _vp, _vp,
+                                                     C.POINTER(DidaStats)]
+    L.dida_letkf_analyze_epilogue_device_f32.argtypes = \
+        L.dida_letkf_analyze_epilogue_device.argtypes
     L.dida_letkf_get_transforms.argtypes = [_vp, C.c_longlong, _dp, _dp, _ip]
     L.dida_letkf_n_analyses.argtypes = [C.POINTER(DidaConfig)]
     L.dida_letkf_n_analyses.restype = C.c_longlong
@@ -316,6 +324,23 @@ class LetkfEngine:
         rc = lib().dida_letkf_analyze_device(self.h, _vp(xb_ptr), _vp(axb_ptr), _vp(xa_ptr),
                                              _vp(axa_ptr), C.byref(self.stats))
         return _check(rc, "dida_letkf_analyze_device")
+
+    def analyze_device_f32(self, xb_ptr: int, axb_ptr: int, xa_ptr: int, axa_ptr: int) -> int:
+        """The analysis on an FP32 perturbation state (FP64 means)."""
+        rc = lib().dida_letkf_analyze_device_f32(self.h, _vp(xb_ptr), _vp(axb_ptr), _vp(xa_ptr),
+                                                 _vp(axa_ptr), C.byref(self.stats))
+        return _check(rc, "dida_letkf_analyze_device_f32")
+
+    def analyze_epilogue_device(self, xb_ptr: int, axb_ptr: int, polar_lat: float,
+                                gamma1: float, gamma2: float, xa_out_ptr: int, sb_ptr: int,
+                                sa_ptr: int, f32: bool = False) -> int:
+        """The analysis with run_da_parallel's epilogue fused into the update
+        (module_da_parallel.F90:219-261); f32: the perturbations are FP32."""
+        fn = (lib().dida_letkf_analyze_epilogue_device_f32 if f32
+              else lib().dida_letkf_analyze_epilogue_device)
+        rc = fn(self.h, _vp(xb_ptr), _vp(axb_ptr), float(polar_lat), float(gamma1),
+                float(gamma2), _vp(xa_out_ptr), _vp(sb_ptr), _vp(sa_ptr), C.byref(self.stats))
+        return _check(rc, "dida_letkf_analyze_epilogue_device" + ("_f32" if f32 else ""))
 
     def get_transforms(self):
         k = self.cfg.k

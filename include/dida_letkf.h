@@ -171,6 +171,29 @@ int dida_letkf_analyze(dida_letkf_handle *h, const double *xb, const double *axb
 /* Same with device-resident arrays (the serving and benchmark path). */
 int dida_letkf_analyze_device(dida_letkf_handle *h, const double *xb, const double *axb,
                               double *xa, double *axa, dida_letkf_stats *stats);
+/* Same with the perturbations stored in FP32 (xb, xa: float, member fastest,
+ * the xb layout) and the means in FP64: the reduced-precision state the
+ * reference's xb_type may be (xb_2_xa's sgemm branch, letkf_compute.F90:293).
+ * The ensemble update runs on f32 MFMA with FP32 accumulation; the means,
+ * the re-centring and the eigensolver stay FP64. */
+int dida_letkf_analyze_device_f32(dida_letkf_handle *h, const float *xb, const double *axb,
+                                  float *xa, double *axa, dida_letkf_stats *stats);
+/* The analysis with the caller's epilogue (run_da_parallel,
+ * module_da_parallel.F90:219-261) fused into the update's output pass, on
+ * device arrays: instead of xa/axa it writes xa_out (full members, the xb
+ * layout, FP64) = gamma1 (axb + xb) + gamma2 (axa + xa) on interior columns
+ * with |grid_lat| <= polar_lat, xb + axb on the others, 0 on halo columns,
+ * and spread_b/spread_a (n, nlat, nlon) = RMS over members (divisor ens) of
+ * xb and of xa_out - axa (xb where |grid_lat| > polar_lat).  The _f32 form
+ * reads FP32 perturbations as dida_letkf_analyze_device_f32. */
+int dida_letkf_analyze_epilogue_device(dida_letkf_handle *h, const double *xb, const double *axb,
+                                       double polar_lat, double gamma1, double gamma2,
+                                       double *xa_out, double *spread_b, double *spread_a,
+                                       dida_letkf_stats *stats);
+int dida_letkf_analyze_epilogue_device_f32(dida_letkf_handle *h, const float *xb,
+                                           const double *axb, double polar_lat, double gamma1,
+                                           double gamma2, double *xa_out, double *spread_b,
+                                           double *spread_a, dida_letkf_stats *stats);
 
 /* Per-analysis outputs of the last analyze call when keep_transforms was
  * set: trans (k*k per analysis, column-major), overline_w (k), nobsl.

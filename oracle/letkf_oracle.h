@@ -96,6 +96,16 @@ int orc_obs_operator(const orc_problem *pb, const double *xb, const double *axb,
                      const int *row_var, const double *row_ph, const double *yo, double *yb,
                      double *dep);
 
+/* CPU baseline (letkf_baseline.c): the horizontal loop on nthreads ranks of
+ * the reference's block split, each scanning its comm_info_init candidate
+ * sites, letkf_core/xb_2_xa on MKL dgemm/dsyevd when MKL loads.  Every
+ * sample-th column of each block.  Returns 0, or -1 for an unsupported
+ * problem (vertical loop, cube_n > 1). */
+int orc_baseline_run(const orc_problem *pb, const double *xb, const double *axb, double *xa,
+                     double *axa, int nthreads, int sample, long long *n_an, long long *n_cand,
+                     double *wall_s, int *split);
+const char *orc_baseline_provider(void);
+
 #ifdef __cplusplus
 }
 #endif

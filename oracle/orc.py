@@ -76,6 +76,10 @@ def lib():
                                         _ip, _ip, C.c_int, C.c_int]
         L.orc_epilogue.argtypes = [C.POINTER(OrcProblem), _dp, _dp, _dp, _dp, C.c_double,
                                    C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_baseline_run.argtypes = [C.POINTER(OrcProblem), _dp, _dp, _dp, _dp, C.c_int,
+                                       C.c_int, C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_longlong), C.POINTER(C.c_double), _ip]
+        L.orc_baseline_provider.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -283,3 +287,26 @@ def obs_operator(case):
     if rc != 0:
         raise ValueError("orc_obs_operator: bad arguments")
     return yb, dep
+
+
+def baseline(inst, nthreads: int, sample: int = 1):
+    """The CPU baseline of bench.py (letkf_baseline.c): the horizontal loop on
+    nthreads ranks of the reference's block split with their comm_info_init
+    candidate obs, letkf_core/xb_2_xa on MKL when it loads, every sample-th
+    column of each block.  Returns dict(xa, axa, n_analyses, n_candidates,
+    wall_s, split, provider)."""
+    pb, keep = _problem(inst)
+    xb = np.ascontiguousarray(inst.xb, np.float64)
+    axb = np.ascontiguousarray(inst.axb, np.float64)
+    xa = np.zeros_like(xb)
+    axa = np.zeros_like(axb)
+    na, nc, wall = C.c_longlong(), C.c_longlong(), C.c_double()
+    split = np.zeros(2, np.int32)
+    rc = lib().orc_baseline_run(C.byref(pb), _p(xb, _dp), _p(axb, _dp), _p(xa, _dp),
+                                _p(axa, _dp), int(nthreads), int(sample), C.byref(na),
+                                C.byref(nc), C.byref(wall), _p(split, _ip))
+    if rc != 0:
+        raise ValueError("orc_baseline_run: unsupported problem (vertical loop or cube_n > 1)")
+    return dict(xa=xa, axa=axa, n_analyses=int(na.value), n_candidates=int(nc.value),
+                wall_s=float(wall.value), split=(int(split[0]), int(split[1])),
+                provider=lib().orc_baseline_provider().decode())

@@ -337,7 +337,7 @@ def test_vertical_k100_padded():
     _assert_parity(gpu, ref, what="c1-k100")
 
 
-@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-4)])
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-5)])
 def test_reduced_precision_k128(prec, tol):
     import orc
     from dida_letkf import synth, engine

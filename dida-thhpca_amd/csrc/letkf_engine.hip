@@ -1289,6 +1289,94 @@ int dida_letkf_core_batched(int k, long long n_an, const long long* row_ptr, con
   return rc;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// dida_letkf_core_batched_noalloc: dida_letkf_core_batched with every device
+// buffer carved from `scratch`, a fixed array in the library's code object, so
+// the call itself allocates no device memory.
+// ---------------------------------------------------------------------------
+
+// 64 MiB on each device that runs the library, held until the process ends.
+// In host code this name is only a handle for hipGetSymbolAddress: the
+// device's copy is reached through the pointer that call returns.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocDevice {
+  std::mutex mu;            // calls on one device run one after another
+  void* ptr = nullptr;      // the device's copy of scratch
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+constexpr size_t kNoAllocAlign = 256;
+size_t round_up(size_t n) { return (n + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; }
+
+// Scratch bytes a call needs: every buffer of core_layout, 256-byte aligned.
+size_t noalloc_bytes(int k, long long n_an, long long nr, int precision) {
+  CoreBufs cb;
+  EigScratch es;
+  DevBuf* bufs[kCoreMaxBufs];
+  size_t sizes[kCoreMaxBufs];
+  const int nb = core_layout(cb, es, k, n_an, nr, precision, bufs, sizes);
+  size_t total = 0;
+  for (int i = 0; i < nb; ++i) total += round_up(sizes[i]);
+  return total;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dida_letkf_core_batched_noalloc(int k, long long n_an, const long long* row_ptr,
+                                    const double* hxens, const float* rdiaginv, const double* dep,
+                                    const double* rloc, double* trans, double* overline_w,
+                                    double inflation, int precision, int* status) {
+  RC_TRY(core_check(k, n_an, row_ptr, hxens, rdiaginv, dep, rloc, trans, overline_w, inflation,
+                    precision));
+  if (n_an == 0) return DIDA_OK;
+  const size_t need = noalloc_bytes(k, n_an, row_ptr[n_an], precision);
+  if (need > kNoAllocScratchBytes)
+    return set_error(DIDA_ERR_ARG,
+                     "dida_letkf_core_batched_noalloc: %lld analyses at k=%d with %lld obs rows "
+                     "need %zu bytes of scratch, more than its fixed %zu",
+                     n_an, k, row_ptr[n_an], need, kNoAllocScratchBytes);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return set_error(DIDA_ERR_UNSUPPORTED, "device %d: the no-alloc path covers devices 0..%d", dev,
+                     kNoAllocMaxDevices - 1);
+  NoAllocDevice& D = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(D.mu);
+  if (!D.ptr) {
+    // set-up, once per device with the device current
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(DIDA_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null pointer");
+    D.ptr = p;
+  }
+  CoreBufs cb;
+  EigScratch es;
+  DevBuf* bufs[kCoreMaxBufs];
+  size_t sizes[kCoreMaxBufs];
+  const int nb = core_layout(cb, es, k, n_an, row_ptr[n_an], precision, bufs, sizes);
+  unsigned char* base = static_cast<unsigned char*>(D.ptr);
+  size_t off = 0;
+  for (int i = 0; i < nb; ++i) {
+    bufs[i]->wrap(base + off, sizes[i]);
+    off += round_up(sizes[i]);
+  }
+  // one stream, the null stream: no stream or event is created
+  const int rc = core_run(nullptr, cb, es, 2, k, n_an, row_ptr, hxens, rdiaginv, dep, rloc,
+                          trans, overline_w, inflation, precision, status);
+  (void)hipStreamSynchronize(nullptr);
+  release_core(cb, es);       // forgets the wrapped pieces; frees nothing
+  return rc;
+}
+
 void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,
                  const float* rdiaginv, const double* dep, const double* rloc, double* trans,
                  double* overline_w, const double* inflation_factor, const int* pflag) {

@@ -90,6 +90,7 @@ EXPORTS = [
     "dida_letkf_analyze_device_f32", "dida_letkf_analyze_epilogue_device",
     "dida_letkf_analyze_epilogue_device_f32",
     "dida_letkf_get_transforms", "dida_letkf_n_analyses", "dida_letkf_core_batched",
+    "dida_letkf_core_batched_noalloc",
     "letkf_core_", "dida_letkf_epilogue_device", "dida_letkf_last_error",
     "dida_letkf_abi_version", "dida_serve_create", "dida_serve_submit", "dida_serve_poll",
     "dida_serve_wait", "dida_serve_cancel", "dida_serve_destroy", "dida_serve_now",
@@ -134,6 +135,7 @@ def lib(path: Optional[str] = None):
     L.dida_letkf_n_analyses.restype = C.c_longlong
     L.dida_letkf_core_batched.argtypes = [C.c_int, C.c_longlong, _lp, _dp, _fp, _dp, _dp, _dp, _dp,
                                           C.c_double, C.c_int, _ip]
+    L.dida_letkf_core_batched_noalloc.argtypes = L.dida_letkf_core_batched.argtypes
     L.letkf_core_.argtypes = [_ip, _ip, _dp, _fp, _dp, _dp, _dp, _dp, _dp, _ip]
     L.letkf_core_.restype = None
     L.dida_letkf_epilogue_device.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
@@ -206,9 +208,11 @@ def letkf_core(hxens: np.ndarray, rdiaginv: np.ndarray, dep: np.ndarray, rloc: n
 
 
 def letkf_core_batched(hxens_list, rdiaginv_list, dep_list, rloc_list,
-                       inflation_factor: float = 1.0, precision: int = PREC_FP64):
+                       inflation_factor: float = 1.0, precision: int = PREC_FP64,
+                       noalloc: bool = False):
     """Many independent letkf_core calls in one launch.  Returns
-    (trans [n, k, k], overline_w [n, k], status [n])."""
+    (trans [n, k, k], overline_w [n, k], status [n]).  noalloc selects the
+    entry point that allocates no device memory."""
     n = len(hxens_list)
     k = hxens_list[0].shape[0]
     sizes = np.array([h.shape[1] for h in hxens_list], dtype=np.int64)
@@ -228,10 +232,10 @@ def letkf_core_batched(hxens_list, rdiaginv_list, dep_list, rloc_list,
     trans = np.zeros((n, k, k))
     w = np.zeros((n, k))
     st = np.zeros(n, np.int32)
-    rc = lib().dida_letkf_core_batched(k, n, _p(row_ptr, _lp), _p(hx, _dp), _p(rd, _fp),
-                                       _p(dp, _dp), _p(rl, _dp), _p(trans, _dp), _p(w, _dp),
-                                       float(inflation_factor), int(precision), _p(st, _ip))
-    _check(rc, "dida_letkf_core_batched")
+    fn = lib().dida_letkf_core_batched_noalloc if noalloc else lib().dida_letkf_core_batched
+    rc = fn(k, n, _p(row_ptr, _lp), _p(hx, _dp), _p(rd, _fp), _p(dp, _dp), _p(rl, _dp),
+            _p(trans, _dp), _p(w, _dp), float(inflation_factor), int(precision), _p(st, _ip))
+    _check(rc, "dida_letkf_core_batched" + ("_noalloc" if noalloc else ""))
     # stored Fortran (k, k) per analysis: [a, j, i] is element (i, j)
     return np.transpose(trans, (0, 2, 1)).copy(), w, st
 

@@ -215,6 +215,24 @@ int dida_letkf_core_batched(int nanals, long long n_analyses, const long long *r
                             const double *rloc, double *trans, double *overline_w,
                             double inflation_factor, int precision, int *status);
 
+/* dida_letkf_core_batched without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as
+ * dida_letkf_core_batched; every device buffer (the staged inputs, the
+ * products, the eigensolver's scratch, the outputs) is carved from one fixed
+ * 64 MiB array that each device running the library holds until the process
+ * ends.  Calls needing more scratch than that return DIDA_ERR_ARG with the
+ * size in the message.  The work runs on the null stream and the call returns
+ * when it is done.  Calls on one device share the array: within a process
+ * they run one after another (a lock per device); each call starts from
+ * cleared state, nothing of an earlier call is read. */
+int dida_letkf_core_batched_noalloc(int nanals, long long n_analyses, const long long *row_ptr,
+                                    const double *hxens, const float *rdiaginv,
+                                    const double *dep, const double *rloc, double *trans,
+                                    double *overline_w, double inflation_factor, int precision,
+                                    int *status);
+
 /* Drop-in for the external letkf_core (letkf_core.F90:14-33), the same
  * by-reference signature the Sunway C kernel uses
  * (sw/DA/LETKF/src/letkf_half/letkf_core_half_master.c:24).  Runs one

@@ -6,10 +6,12 @@ n = 97 per column), k = 64, 500,000 obs rows (70% on 40 great-circle swaths,
 (support 600 km), one local analysis per column: N_A = 64,800.
 
 A step is one analysis cycle with every input already in HBM: the obs-site
-index build (set_obs_device) plus the analysis (analyze_device: selection,
-obs-space products, FP64 eigensolve + transform, ensemble update).  The
-mixed-precision form (BF16 hi/lo split products, FP32 accumulation, FP64
-eigensolve) is the headline; the plain FP64 form is timed beside it.
+index build (set_obs_device) plus the analysis (selection, obs-space
+products, FP64 eigensolve + transform, ensemble update).  The mixed-precision
+form is the headline: BF16 split products with FP32 accumulation, FP64
+eigensolve and inverse square root, and the perturbations stored in FP32
+with the update on f32 MFMA (analyze_device_f32).  The plain FP64 form
+(analyze_device, FP64 throughout) is timed beside it.
 
 Prints ONE JSON line (rank 0).  --gpus N > 1 is refused: the engine runs on
 one GPU.
@@ -37,39 +39,54 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-KERNEL_NAMES = {"products": "k_products", "tridiag": "k_tridiag", "ql": "k_ql_log",
-                "apply": "k_vec", "update": "k_update"}
+KERNEL_NAMES = {"select": "k_select", "products": "k_products", "tridiag": "k_tridiag",
+                "ql": "k_ql_log", "apply": "k_vec", "update": "k_update"}
 
 
 def kernel_work(name, stats, cfg, n_an, mixed):
-    """Algorithmic work of one cycle's launch of a kernel: (amount, peak, bound, unit).
+    """Algorithmic work of one cycle's launch of a kernel, as the cycle's
+    flop and byte counts F and B are defined for this workload:
+    (flops, bytes, peak, bound, unit).
 
-    products: the symmetric SYRK k(k+1) p + Yb^T R^-1 d 2k p flops per analysis
-              (BF16 MFMA peak in the split-bf16 mode, else FP64);
-    tridiag:  Householder tridiagonalisation 4/3 k^3 flops (Golub & Van Loan 8.3.1);
-    ql:       the QL recurrence, ~30 flops per Givens step, ~1.1 k^2 steps;
-    apply:    eigenvectors and transform: Q = H_0..H_{k-3} 4/3 k^3, rotation
-              accumulation 6 k^3 (GvL 8.3.5), T = V f V^T 2k^3, overline_w 4k^2;
-              (with the fused kernel, k <= 64 only under DIDA_APPLY=fused, this
-              also carries the update's 2 k^2 m);
-    update:   the ensemble update, HBM-bound: Xb read and Xa written (2*8*k*m
-              bytes) plus the means (16 m bytes) per analysis.
+    products: the symmetric SYRK k(k+1) p plus Yb^T R^-1 d 2 k p flops per
+              analysis (p local obs rows), on MFMA: the BF16 peak in the split
+              mode (the three-term split's extra products are not counted),
+              FP64 otherwise;
+    apply:    T = V f V^T (2 k^3) and overline_w = Pa b (4 k^2) per analysis,
+              FP64 MFMA; the eigendecomposition itself (tridiag, QL, the
+              eigenvector accumulation) is excluded from F and reported as a
+              time share;
+    update:   Xa = Xb trans, 2 k^2 m flops, HBM-bound: Xb read and Xa written
+              (2 s_x k m bytes, s_x = 4 for the FP32 state of the mixed form,
+              8 for FP64) plus the means (16 m bytes) per analysis;
+    select, tridiag, ql: no F (index work, eigendecomposition).
     The FP64 vector and FP64 matrix peaks of MI355X are equal (78.6 TFLOP/s)."""
     k, m = cfg.k, cfg.n
     if name == "products":
         p = stats["total_rows"]
         fl = k * (k + 1) * p + 2 * k * p
-        return fl, (PEAK_BF16_TFLOPS if mixed == 2 else PEAK_FP64_TFLOPS), "mfma", "TFLOP/s"
-    if name == "tridiag":
-        return n_an * (4.0 / 3.0) * k ** 3, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
-    if name == "ql":
-        return n_an * 30 * 1.1 * k ** 2, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
+        return fl, 0, (PEAK_BF16_TFLOPS if mixed == 2 else PEAK_FP64_TFLOPS), "mfma", "TFLOP/s"
+    if name == "apply":
+        return n_an * (2 * k ** 3 + 4 * k ** 2), 0, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
     if name == "update":
-        return n_an * (2 * 8 * k * m + 16 * m), PEAK_HBM_GBS / 1000.0, "hbm", "TB/s"
-    fl = n_an * ((4.0 / 3.0) * k ** 3 + 6 * k ** 3 + 2 * k ** 3 + 4 * k ** 2)
-    if stats.get("ms_update", 0.0) == 0.0:
-        fl += n_an * 2 * k ** 2 * m
-    return fl, PEAK_FP64_TFLOPS, "mfma", "TFLOP/s"
+        sx = 4 if mixed else 8
+        return (n_an * 2 * k ** 2 * m, n_an * (2 * sx * k * m + 16 * m), PEAK_HBM_GBS / 1000.0,
+                "hbm", "TB/s")
+    return 0, 0, PEAK_FP64_TFLOPS, "none", "TFLOP/s"
+
+
+def cycle_work(stats, cfg, n_an, n_obs_rows, mixed):
+    """F and B of one cycle: F = sum_i [k(k+1) p_i + 2k p_i + 2k^3 + 4k^2 + 2k^2 m_i]
+    (the eigendecomposition excluded), B = 2 s_x k N_state + 16 N_state +
+    N_obs (s_y k + 40)."""
+    k, m = cfg.k, cfg.n
+    p = stats["total_rows"]
+    F = k * (k + 1) * p + 2 * k * p + n_an * (2 * k ** 3 + 4 * k ** 2 + 2 * k ** 2 * m)
+    n_state = cfg.nlat * cfg.nlon * m
+    sx = 4 if mixed else 8
+    sy = 4 if mixed else 8
+    B = 2 * sx * k * n_state + 16 * n_state + n_obs_rows * (sy * k + 40)
+    return F, B
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "letkf_c2_k64_traffic.json")
@@ -91,17 +108,41 @@ def measured_traffic(kernel, workload):
     return None if k is None else k.get("bytes")
 
 
-def roofline(km, stats, cfg, n_an, mixed, workload=None):
-    cands = [x for x in ("products", "tridiag", "ql", "apply", "update") if km.get(x, 0.0) > 0]
+def roofline(km, stats, cfg, n_an, n_obs_rows, mixed, ms_step, workload=None):
+    """The dominant kernel (largest measured time) against its roofline, and
+    the whole cycle's F and B against the peaks, with the eigensolver's time
+    share (its work is not in F)."""
+    cands = [x for x in ("select", "products", "tridiag", "ql", "apply", "update")
+             if km.get(x, 0.0) > 0]
     dom = max(cands, key=lambda x: km[x])
-    amount, peak, bound, unit = kernel_work(dom, stats, cfg, n_an, mixed)
-    achieved = amount / (km[dom] * 1e-3) / 1e12
+    fl, by, peak, bound, unit = kernel_work(dom, stats, cfg, n_an, mixed)
+    t = km[dom] * 1e-3
+    amount = by if bound == "hbm" else fl
+    achieved = amount / t / 1e12
     out = dict(bound=bound, kernel=KERNEL_NAMES[dom], achieved=round(achieved, 4), peak=peak,
                unit=unit, frac=round(achieved / peak, 5),
                traffic=measured_traffic(KERNEL_NAMES[dom], workload),
                ms_per_launch=round(km[dom], 4),
                kernel_ms={KERNEL_NAMES[x]: round(km[x], 4) for x in cands})
     out["bytes_per_launch" if bound == "hbm" else "flops_per_launch"] = amount
+    per = {}
+    for x in cands:
+        f, b, pk, bd, u = kernel_work(x, stats, cfg, n_an, mixed)
+        if bd == "none" or km[x] <= 0:
+            continue
+        a = (b if bd == "hbm" else f) / (km[x] * 1e-3) / 1e12
+        per[KERNEL_NAMES[x]] = dict(achieved=round(a, 4), unit=u, frac=round(a / pk, 5))
+    out["per_kernel"] = per
+    F, B = cycle_work(stats, cfg, n_an, n_obs_rows, mixed)
+    tc = ms_step * 1e-3
+    eig_ms = km.get("tridiag", 0.0) + km.get("ql", 0.0) + km.get("apply", 0.0)
+    out["cycle"] = dict(
+        flops=F, bytes=B, ms=round(ms_step, 3),
+        tflops=round(F / tc / 1e12, 3), tbps=round(B / tc / 1e12, 4),
+        frac_fp64_peak=round(F / tc / 1e12 / PEAK_FP64_TFLOPS, 4),
+        frac_hbm=round(B / tc / 1e12 / (PEAK_HBM_GBS / 1000.0), 4),
+        eigensolver_ms=round(eig_ms, 3),
+        eigensolver_share=round(eig_ms / max(sum(km[x] for x in cands), 1e-9), 4))
     return out
 
 
@@ -166,7 +207,8 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
     st = head["stats"]
     workload = (f"{args.config}: {cfg.nlon}x{cfg.nlat}x{cfg.nz} grid, n={cfg.n}, k={cfg.k}, "
                 f"{n_obs_rows} clustered obs rows, c={cfg.local_dist:g} km, N_A={n_a} analyses/cycle")
-    roof = roofline(head["kernel_ms"], st, cfg, n_a, args.mixed, workload)
+    roof = roofline(head["kernel_ms"], st, cfg, n_a, n_obs_rows, args.mixed, head["ms_step"],
+                    workload)
     line = {
         "metric": "LETKF local analyses/sec at k=64 (mixed precision; FP64 form beside it)",
         "value": round(head["value"], 2),
@@ -178,11 +220,18 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3+f64" if args.mixed == 2 else "f32+f64",
+        "dtype": ("bf16x3" if args.mixed == 2 else "f32") + "+f64" +
+                 ("+f32state" if args.mixed_state == "f32" else ""),
         "data": "synthetic",
         "config": {"workload": workload,
                    "n_analyses": n_a, "k": cfg.k, "obs_rows": n_obs_rows,
                    "local_obs_rows_total": st["total_rows"], "local_obs_rows_max": st["max_rows"],
+                   "rows_over_10000": st["rows_over_10000"],
+                   "precision": (("BF16x3" if args.mixed == 2 else "FP32") + " MFMA obs-space "
+                                 "products with FP32 accumulation, FP64 eigensolve and inverse "
+                                 "square root, " + ("FP32 perturbation state, update on f32 MFMA"
+                                                    if args.mixed_state == "f32" else
+                                                    "FP64 state and update")),
                    "parallelism": "1 GPU"},
         "kernel_ms": head["kernel_ms"],
         "kernel_ms_note": ("HIP-event time of each kernel over the whole batch, from two extra "
@@ -212,9 +261,13 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--k", type=int, default=0, help="ensemble size (0: the config's)")
     ap.add_argument("--mixed", type=int, default=2, help="precision of the headline mode (1 fp32, 2 bf16x3)")
+    ap.add_argument("--mixed-state", choices=("f32", "f64"), default="f32",
+                    help="storage of the perturbations in the headline mode (the update runs "
+                         "on f32 MFMA for f32)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="every Nth column for the CPU baseline")
+    ap.add_argument("--cpu-sample", type=int, default=8,
+                    help="every Nth column of each rank's block for the CPU baseline")
     ap.add_argument("--parity-cols", type=int, default=48)
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs per arrival rate in the serving leg (0: no serving leg)")
@@ -272,6 +325,8 @@ def main():
     d = dict(xb=T(inst.xb), axb=T(inst.axb), lat=T(o.site_lat), lon=T(o.site_lon),
              rp=T(o.site_row_ptr.astype(np.int32)), yb=T(o.yb), dep=T(o.dep), rinv=T(o.rinv),
              vco=T(o.vcoord))
+    if args.mixed_state == "f32":
+        d["xb32"] = d["xb"].to(torch.float32)
     stream = torch.cuda.current_stream()
 
     def make_engine(prec, transform_kernel=0, keep=False):
@@ -287,6 +342,9 @@ def main():
         eng.set_obs_device(o.n_sites, d["lat"].data_ptr(), d["lon"].data_ptr(), d["rp"].data_ptr(),
                            o.n_rows, d["yb"].data_ptr(), d["dep"].data_ptr(), d["rinv"].data_ptr(),
                            d["vco"].data_ptr())
+        if xa.dtype == torch.float32:
+            return eng.analyze_device_f32(d["xb32"].data_ptr(), d["axb"].data_ptr(),
+                                          xa.data_ptr(), axa.data_ptr())
         return eng.analyze_device(d["xb"].data_ptr(), d["axb"].data_ptr(), xa.data_ptr(),
                                   axa.data_ptr())
 
@@ -294,7 +352,8 @@ def main():
         steps = args.steps if steps is None else steps
         warmup = args.warmup if warmup is None else warmup
         eng = make_engine(prec, transform_kernel, keep=counts)
-        xa = torch.zeros_like(d["xb"])
+        f32 = prec != PREC_FP64 and args.mixed_state == "f32"
+        xa = torch.zeros_like(d["xb32"] if f32 else d["xb"])
         axa = torch.zeros_like(d["axb"])
         for _ in range(warmup):
             step(eng, xa, axa)
@@ -360,7 +419,7 @@ def main():
         g64 = (res[PREC_FP64]["xa"] + res[PREC_FP64]["axa"][..., None]).cpu().numpy()
         diff = np.abs(g64[lon_i, lat_i] - ref_full[lon_i, lat_i]).max()
         acc["fp64_vs_oracle_max_rel"] = float(diff / np.abs(ref_full[lon_i, lat_i]).max())
-        gm = (res[args.mixed]["xa"] + res[args.mixed]["axa"][..., None])
+        gm = (res[args.mixed]["xa"].double() + res[args.mixed]["axa"][..., None])
         g64t = res[PREC_FP64]["xa"] + res[PREC_FP64]["axa"][..., None]
         xbf = d["xb"] + d["axb"][..., None]
         per_var = {}
@@ -375,19 +434,22 @@ def main():
         acc["mixed_increment_rms_rel_err"] = per_var
     log(f"accuracy: {acc}")
 
-    # ---- CPU baseline: the oracle (port of the reference loop) on a sample
+    # ---- CPU baseline: the reference's step on the host cores (letkf_baseline.c)
     cpu = None
     if not args.no_cpu:
+        # 16 threads: the GPU box's CPU share for one GPU
         ncore = min(16, os.cpu_count() or 1)
-        samp = np.arange(0, cfg.nlat * cfg.nlon, args.cpu_sample, dtype=np.int32)
-        tc = time.perf_counter()
-        orc.letkf_compute(inst, columns=samp, nthreads=ncore)
-        tcpu = time.perf_counter() - tc
-        n_samp = samp.size * (N_A // (cfg.nlat * cfg.nlon))   # analyses per column: vertical chunks
-        cpu = dict(value=float(n_samp / tcpu), unit="local analyses/s", cores=ncore, kind="port",
-                   sample=f"every {args.cpu_sample}th column of the same cycle ({n_samp} analyses, "
-                          f"{tcpu:.1f} s wall); brute-force gen_yb_dis over all obs as on one rank, "
-                          f"own Householder+QL eigensolver, FP64")
+        b = orc.baseline(inst, nthreads=ncore, sample=args.cpu_sample)
+        n_an_cpu = b["n_analyses"] * (N_A // (cfg.nlat * cfg.nlon))
+        cpu = dict(value=float(n_an_cpu / b["wall_s"]), unit="local analyses/s", cores=ncore,
+                   kind="port",
+                   sample=(f"every {args.cpu_sample}th column of each rank's block of the same "
+                           f"cycle ({n_an_cpu} analyses, {b['wall_s']:.1f} s wall to the slowest "
+                           f"rank); {b['split'][0]}x{b['split'][1]} (lat x lon) rank blocks as "
+                           f"MPI_Dims_create({ncore}, 2), each scanning the obs of its "
+                           f"comm_info_init neighbour blocks "
+                           f"({b['n_candidates'] / max(b['n_analyses'], 1):.0f} sites per "
+                           f"analysis); letkf_core and xb_2_xa on {b['provider']}, FP64"))
         log(f"cpu baseline: {cpu}")
 
     line = make_line(args, cfg, N_A, o.n_rows, res, acc, cpu)

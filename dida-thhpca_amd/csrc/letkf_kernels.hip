@@ -401,11 +401,9 @@ __device__ __forceinline__ long long analysis_at(const ProdParams& p, long long 
 }
 
 // Reduced-precision products: row lists longer than kFlushMinRows add their
-// FP32 partial sums to FP64 every kFlushRows rows; at k > 64 lists longer
-// than kFp32MinRowsWide use FP32 instead of BF16 hi/lo products.
+// FP32 partial sums to FP64 every kFlushRows rows.
 constexpr int kFlushMinRows = 3072;
 constexpr int kFlushRows = 1024;
-constexpr int kFp32MinRowsWide = 4096;
 
 template <int T>
 struct PairTable {
@@ -682,9 +680,11 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   }
 }
 
-// BF16 hi/lo split: yhat = sqrt(w) y (FP32) = hi + lo with hi = bf16(yhat),
-// lo = bf16(yhat - hi); A ~ hi hi^T + hi lo^T + lo hi^T, FP32 accumulation
-// (b = sum w d y too).
+// BF16 split of yhat = sqrt(w) y (FP32), FP32 accumulation (b = sum w d y
+// too).  k <= 64: two terms, hi = bf16(yhat), lo = bf16(yhat - hi) (16 bits),
+// A ~ hi hi^T + hi lo^T + lo hi^T.  k = 65..128, where the increment's
+// sensitivity to A grows with the eigenvalue spread: three terms
+// (hi, mid, lo: 24 bits) and six products (products_bf16_shared).
 
 // One 32-row group position of an analysis' row list (rl, with n rows):
 // sqrt(w) and w d in FP32, zero past the end of the list (whose last row is
@@ -816,9 +816,35 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
 #pragma unroll
     for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(8 * q + u, n - 1)];
   }
+  // long lists add their FP32 partial sums to FP64 every kFlushRows rows
+  // (A in memory, b in registers)
+  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+  const bool chunked = n > kFlushMinRows;
+  bool flushed = false;
+  double bsum[TW];
+#pragma unroll
+  for (int t = 0; t < TW; ++t) bsum[t] = 0.0;
   int buf = 0;
   for (int g0 = 0; g0 < n; g0 += 32, buf ^= 1) {
-    bf8 hi[TW], lo[TW];
+    if (chunked && g0 > 0 && g0 % kFlushRows == 0) {
+#pragma unroll
+      for (int x = 0; x < NP; ++x) {
+        const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double* e = A + (long long)(16 * J + i16) * KP + 16 * I + 4 * q + r;
+          *e = (flushed ? *e : 0.0) + (double)acc[x][r];
+        }
+        acc[x] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        bsum[t] += (double)bacc[t];
+        bacc[t] = 0.f;
+      }
+      flushed = true;
+    }
+    bf8 hi[TW], mi[TW], lo[TW];
     int idx[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
@@ -833,54 +859,71 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
       for (int tt = 0; tt < TW; ++tt) {
         const int m = 16 * (WS * TW + tt) + i16;
         const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
+        // yh = hi + mid + lo: 24 bits in three BF16 terms (each difference
+        // is exact in FP32)
         const float yh = s * yv;
         const __bf16 h = (__bf16)yh;
+        const float r1v = yh - (float)h;
+        const __bf16 m1 = (__bf16)r1v;
         hi[tt][u] = h;
-        lo[tt][u] = (__bf16)(yh - (float)h);
+        mi[tt][u] = m1;
+        lo[tt][u] = (__bf16)(r1v - (float)m1);
         bacc[tt] = fmaf(wdf, yv, bacc[tt]);
       }
     }
-    bf8* sb = sh + buf * (2 * T * 64);
+    bf8* sb = sh + buf * (3 * T * 64);
 #pragma unroll
     for (int tt = 0; tt < TW; ++tt) {
       sb[(WS * TW + tt) * 64 + lane] = hi[tt];
-      sb[(T + WS * TW + tt) * 64 + lane] = lo[tt];
+      sb[(T + WS * TW + tt) * 64 + lane] = mi[tt];
+      sb[(2 * T + WS * TW + tt) * 64 + lane] = lo[tt];
     }
     __syncthreads();
     // the buffer written next is the other one; it was last read before this
-    // barrier, so one barrier per group suffices
-    bf8 H[T], L[T];
+    // barrier, so one barrier per group suffices.  The six products that
+    // carry 24 bits -- hi.hi, hi.mid, mid.hi, mid.mid, hi.lo, lo.hi -- run in
+    // two rounds, the second reading lo into mid's registers
+    bf8 H[T], M[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       H[t] = sb[t * 64 + lane];
-      L[t] = sb[(T + t) * 64 + lane];
+      M[t] = sb[(T + t) * 64 + lane];
     }
 #pragma unroll
     for (int x = 0; x < NP; ++x) {
       const int I = PT::I(P0 + x), J = PT::J(P0 + x);
       acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], H[J], acc[x], 0, 0, 0);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], L[J], acc[x], 0, 0, 0);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(L[I], H[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], M[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[I], H[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[I], M[J], acc[x], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) M[t] = sb[(2 * T + t) * 64 + lane];
+#pragma unroll
+    for (int x = 0; x < NP; ++x) {
+      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(H[I], M[J], acc[x], 0, 0, 0);
+      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M[I], H[J], acc[x], 0, 0, 0);
     }
   }
-  double* A = p.A + (a - p.a0) * (long long)KP * KP;
 #pragma unroll
   for (int x = 0; x < NP; ++x) {
     const int I = PT::I(P0 + x), J = PT::J(P0 + x);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
-      A[(long long)col * KP + row] = (double)acc[x][r];
-      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
+      const double v = (flushed ? A[(long long)col * KP + row] : 0.0) + (double)acc[x][r];
+      A[(long long)col * KP + row] = v;
+      if (I != J) A[(long long)row * KP + col] = v;
     }
   }
   double* b = p.b + (a - p.a0) * KP;
 #pragma unroll
   for (int tt = 0; tt < TW; ++tt) {
-    float v = bacc[tt];
+    double v = bsum[tt] + (double)bacc[tt];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
-    if (q == 0) b[16 * (WS * TW + tt) + i16] = (double)v;
+    if (q == 0) b[16 * (WS * TW + tt) + i16] = v;
   }
 }
 
@@ -904,19 +947,7 @@ void k_products_bf16(ProdParams p) {
     products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
   } else {
     static_assert(WPA == 4 && KP % 64 == 0, "four wavefronts share an analysis");
-    // Long row lists at k > 64: the hi/lo split keeps 16 bits of each value,
-    // and with thousands of rows that error, amplified by the k = 65..128
-    // eigenvalue spread, passes 1e-5 of the analysis increment; these
-    // analyses take the FP32 products (24 bits, FP64 flushes).  Every
-    // wavefront of the block's one analysis takes the same branch.
-    if (r1 - r0 > kFp32MinRowsWide) {
-      if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
-      return;
-    }
-    __shared__ bf8 sh[2 * 2 * (KP / 16) * 64];
+    __shared__ bf8 sh[2 * 3 * (KP / 16) * 64];
     if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
     if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);
     if (wsub == 2) products_bf16_shared<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane, sh);

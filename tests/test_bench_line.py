@@ -13,9 +13,9 @@ def test_line_has_contract_keys():
     import bench
     from dida_letkf import synth
     cfg = synth.make_config("c2")
-    args = argparse.Namespace(mixed=2, steps=5, warmup=1, config="c2")
+    args = argparse.Namespace(mixed=2, mixed_state="f32", steps=5, warmup=1, config="c2")
     km = dict(select=2.8, products=9.6, transform=41.9, tridiag=8.5, ql=9.0, apply=24.0, update=1.5)
-    st = dict(total_rows=72_613_192, max_rows=2656, ms_update=1.5)
+    st = dict(total_rows=72_613_192, max_rows=2656, ms_update=1.5, rows_over_10000=0)
     res = {2: dict(value=1.18e6, ms_step=54.9, kernel_ms=km, stats=st),
            0: dict(value=1.0e6, ms_step=64.8, kernel_ms=dict(km, products=19.9), stats=st)}
     line = bench.make_line(args, cfg, cfg.n_analyses(), 500_000, res, {}, None)
@@ -23,16 +23,25 @@ def test_line_has_contract_keys():
                 "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
                 "roofline", "cpu_baseline"):
         assert key in line
+    assert line["config"]["rows_over_10000"] == 0
     r = line["roofline"]
     assert r["kernel"] == "k_vec" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-4
     k, m, n_a = cfg.k, cfg.n, cfg.n_analyses()
-    fl = n_a * ((4 / 3) * k**3 + 6 * k**3 + 2 * k**3 + 4 * k**2)
+    # the eigenvector kernel is priced at T = V f V^T and overline_w only
+    fl = n_a * (2 * k**3 + 4 * k**2)
     assert abs(r["achieved"] - fl / 24.0e-3 / 1e12) < 1e-3
-    # an HBM-bound update, when it dominates, is priced in bytes
-    r2 = bench.roofline(dict(update=3.0, apply=1.0), st, cfg, n_a, 2)
+    # the cycle's F and B, and the eigensolver's share of the kernel time
+    p = st["total_rows"]
+    F = k * (k + 1) * p + 2 * k * p + n_a * (2 * k**3 + 4 * k**2 + 2 * k**2 * m)
+    n_state = cfg.nlat * cfg.nlon * m
+    B = 2 * 4 * k * n_state + 16 * n_state + 500_000 * (4 * k + 40)
+    assert r["cycle"]["flops"] == F and r["cycle"]["bytes"] == B
+    assert abs(r["cycle"]["eigensolver_share"] - (8.5 + 9.0 + 24.0) / 55.4) < 1e-3
+    # an HBM-bound update, when it dominates, is priced in bytes (FP32 state)
+    r2 = bench.roofline(dict(update=3.0, apply=1.0), st, cfg, n_a, 500_000, 2, 4.0)
     assert r2["bound"] == "hbm" and r2["unit"] == "TB/s"
-    assert abs(r2["achieved"] - n_a * (16 * k * m + 16 * m) / 3.0e-3 / 1e12) < 1e-3
+    assert abs(r2["achieved"] - n_a * (8 * k * m + 16 * m) / 3.0e-3 / 1e12) < 1e-3
     assert line["mixed_over_fp64"] == round(1.18e6 / 1.0e6, 3)
     json.dumps(line)
 

@@ -15,9 +15,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "dida_letkf.h"
 #include "letkf_device.h"
@@ -187,6 +192,27 @@ int env_int(const char* name) {
   const char* e = std::getenv(name);
   if (!e || !*e) return 0;
   return std::max(0, std::atoi(e));
+}
+
+// Sizes every eigensolver buffer for batches of up to n analyses (the list
+// pass's too), so that the launches of a call allocate nothing.
+int eig_prepare(EigScratch& s, int k, long long n) {
+  const int kp = kp_of(k);
+  const long long cap = rerun_cap_of(k, n);
+  RC_TRY(s.refl.ensure(sizeof(double) * n * kp * kp));
+  RC_TRY(s.tau.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.d.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.e.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.lam.ensure(sizeof(double) * n * kp));
+  RC_TRY(s.rot.ensure((size_t)16 * n * rot_cap_of(k)));
+  RC_TRY(s.hdr.ensure((size_t)16 * n * kHdrCapPerK * k));
+  RC_TRY(s.nchase.ensure(sizeof(int) * n));
+  RC_TRY(s.flags.ensure(sizeof(int) * (2 + cap)));
+  RC_TRY(s.rr_rot.ensure((size_t)16 * cap * 2 * rot_cap_of(k)));
+  RC_TRY(s.rr_hdr.ensure((size_t)16 * cap * 2 * kHdrCapPerK * k));
+  RC_TRY(s.rr_nchase.ensure(sizeof(int) * cap));
+  RC_TRY(s.rr_lam.ensure(sizeof(double) * cap * kp));
+  return DIDA_OK;
 }
 
 // Zeroes the call-wide flag count; run once per analysis call.
@@ -389,7 +415,33 @@ struct dida_letkf_handle {
   std::vector<int> nobsl_host;
   long long max_batch = 0;
   bool tiled = false;                 // tile-union products
+  // row lists: capacity per batch (grown when a call finds more rows, which
+  // then reruns), per-batch totals and per-column counts read after a call
+  long long row_cap = 0;
+  DevBuf batch_tot;
+  // per-batch timing events (9 per batch), created once
+  std::vector<hipEvent_t> ev;
+  // hipGraph of a call's launches, replayed while every pointer and size it
+  // captured is unchanged
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uintptr_t> gkey;
+  hipEvent_t g_ev[2] = {nullptr, nullptr};
+  // options (dida_letkf_set_option)
+  int opt_schedule = 1, opt_timing = 0, opt_tune = 0, opt_graph = 0;
+  // bumped by set_grid / set_obs: a captured graph holds their buffers and sizes
+  long long obs_version = 0;
+  // length buckets: order list, block counts, segment starts, queue counters,
+  // per-batch bucket statistics and timing ticks
+  DevBuf b_order, b_blk, b_seg, b_q, b_stats, b_ticks;
+  int variant[kBuckets] = {0};        // products variant per bucket position
+  std::vector<bool> split_timed;      // per batch: split eigensolver events recorded
+  std::string arch;                   // device architecture name (tuning cache key)
+  double clock_khz = 100000.0;        // device wall clock (in-kernel timing)
 };
+
+// products variant per bucket position, from the tuning cache
+static void resolve_variants(dida_letkf_handle* h);
 
 extern "C" {
 
@@ -479,6 +531,19 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
     return set_error(DIDA_ERR_HIP, "HIP init failed: %s", hipGetErrorString(e));
   }
   h->stream = h->own_stream;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) {
+      h->arch = prop.gcnArchName;
+      const size_t cut = h->arch.find(':');
+      if (cut != std::string::npos) h->arch.resize(cut);
+    }
+    if (h->arch.empty()) h->arch = "unknown";
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device) == hipSuccess &&
+        khz > 0)
+      h->clock_khz = khz;
+  }
   // obs-space support angle 2c/a (radians -> degrees) with margin for the
   // real(4) pi scaling of the reference's distance
   const double a = 6371.39306640625;
@@ -494,6 +559,7 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
     // tiles never straddle batches, so results do not depend on the batching
     h->max_batch = std::max<long long>(kTile, (h->max_batch / kTile) * kTile);
   }
+  resolve_variants(h);
   *out = h;
   return DIDA_OK;
 }
@@ -515,6 +581,14 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->eig.flags, &h->eig.rr_rot, &h->eig.rr_hdr, &h->eig.rr_nchase,
                     &h->eig.rr_lam};
   for (DevBuf* b : bufs) b->release();
+  h->batch_tot.release();
+  for (DevBuf* b : {&h->b_order, &h->b_blk, &h->b_seg, &h->b_q, &h->b_stats, &h->b_ticks})
+    b->release();
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->g_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return DIDA_OK;
@@ -524,6 +598,35 @@ int dida_letkf_set_stream(dida_letkf_handle* h, void* s) {
   if (!h) return set_error(DIDA_ERR_ARG, "null handle");
   h->stream = s ? (hipStream_t)s : h->own_stream;
   return DIDA_OK;
+}
+
+int dida_letkf_set_option(dida_letkf_handle* h, int option, long long value) {
+  if (!h) return set_error(DIDA_ERR_ARG, "null handle");
+  switch (option) {
+    case DIDA_OPT_SCHEDULE:
+      if (value != 0 && value != 1) break;
+      h->opt_schedule = (int)value;
+      return DIDA_OK;
+    case DIDA_OPT_TIMING:
+      if (value != 0 && value != 1) break;
+      h->opt_timing = (int)value;
+      return DIDA_OK;
+    case DIDA_OPT_TUNE:
+      if (value != 0 && value != 1) break;
+      h->opt_tune = (int)value;
+      return DIDA_OK;
+    case DIDA_OPT_GRAPH:
+      if (value != 0 && value != 1) break;
+      h->opt_graph = (int)value;
+      return DIDA_OK;
+    case DIDA_OPT_ROW_CAP:
+      if (value < 0) break;
+      h->row_cap = value;
+      return DIDA_OK;
+    default:
+      return set_error(DIDA_ERR_ARG, "unknown option %d", option);
+  }
+  return set_error(DIDA_ERR_ARG, "bad value %lld for option %d", value, option);
 }
 
 int dida_letkf_set_grid(dida_letkf_handle* h, const float* lat, const float* lon) {
@@ -543,6 +646,7 @@ int dida_letkf_set_grid(dida_letkf_handle* h, const float* lat, const float* lon
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->have_grid = true;
+  ++h->obs_version;
   return DIDA_OK;
 }
 
@@ -609,6 +713,7 @@ static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
   }
   HIP_TRY(hipGetLastError());
   h->have_obs = true;
+  ++h->obs_version;
   return DIDA_OK;
 }
 
@@ -701,23 +806,202 @@ struct StateArgs {
   double* spread_a = nullptr;
 };
 
-static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_stats* stats) {
+// ---------------------------------------------------------------------------
+// Tuning cache: the products variant (32-row groups per step) of each
+// (device, k, precision, length bucket), measured on a live call and kept in
+// a text file, one "arch k precision bucket groups microseconds" line each.
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, double>> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_path() {
+  if (const char* f = std::getenv("DIDA_TUNE_FILE")) return f;
+  if (const char* x = std::getenv("XDG_CACHE_HOME")) return std::string(x) + "/dida_letkf/tune.tsv";
+  if (const char* hm = std::getenv("HOME")) return std::string(hm) + "/.cache/dida_letkf/tune.tsv";
+  return std::string();
+}
+
+std::string tune_key(const std::string& arch, int k, int prec, int bucket) {
+  return arch + " " + std::to_string(k) + " " + std::to_string(prec) + " " + std::to_string(bucket);
+}
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const std::string path = tune_path();
+  if (path.empty()) return;
+  std::ifstream f(path);
+  std::string arch;
+  int k, prec, bucket, groups;
+  double us;
+  while (f >> arch >> k >> prec >> bucket >> groups >> us)
+    if (groups == 1 || groups == 2) g_tune[tune_key(arch, k, prec, bucket)] = {groups, us};
+}
+
+// rewrites the whole file (entries of every device and shape seen)
+int tune_save_locked() {
+  const std::string path = tune_path();
+  if (path.empty()) return DIDA_OK;
+  std::error_code ec;
+  std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+  const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+  {
+    std::ofstream f(tmp);
+    if (!f) return set_error(DIDA_ERR_ARG, "cannot write the tuning cache %s", tmp.c_str());
+    for (const auto& kv : g_tune) f << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
+  }
+  std::filesystem::rename(tmp, path, ec);
+  if (ec) return set_error(DIDA_ERR_ARG, "cannot replace the tuning cache %s", path.c_str());
+  return DIDA_OK;
+}
+
+// products variants worth measuring: row groups per step, BF16 at k <= 64
+std::vector<int> products_variants(int kp, int precision) {
+  if (kp <= 64 && precision == DIDA_PREC_BF16X3) return {1, 2};
+  return {1};
+}
+}  // namespace
+
+// variant per bucket position from the cache (default 1 row group);
+// DIDA_PRODUCTS_GROUPS = 1 or 2 sets every bucket's (a test hook)
+static void resolve_variants(dida_letkf_handle* h) {
+  const std::vector<int> vars = products_variants(h->kp, h->cfg.precision);
+  const int forced = env_int("DIDA_PRODUCTS_GROUPS");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  for (int j = 0; j < kBuckets; ++j) {
+    const int b = kBuckets - 1 - j;
+    auto it = g_tune.find(tune_key(h->arch, h->cfg.k, h->cfg.precision, b));
+    int v = it != g_tune.end() ? it->second.first : 1;
+    if (forced) v = forced;
+    h->variant[j] = std::find(vars.begin(), vars.end(), v) != vars.end() ? v : 1;
+  }
+}
+
+// Row-list capacity per batch before any call has measured one: the expected
+// rows of uniformly spread obs within the support radius 2c, twice over, plus
+// a margin (a call that finds more runs again with the measured need).
+static long long estimate_row_cap(const dida_letkf_handle* h) {
+  const double ang = std::min(M_PI, 2.0 * h->cfg.local_dist / kEarthF32);
+  const double frac = 0.5 * (1.0 - std::cos(ang));
+  const double per_col = (double)h->n_rows * frac;
+  const double lists = h->tiled ? (double)h->max_batch / kTile * 2.0 : (double)h->max_batch;
+  const double est = 2.0 * per_col * lists + 65536.0;
+  const double most = (double)h->n_rows * (double)h->max_batch + 1.0;
+  return (long long)std::min(est, most);
+}
+
+static long long n_batches_of(const dida_letkf_handle* h) {
+  return (h->n_h + h->max_batch - 1) / h->max_batch;
+}
+
+// Every buffer of a call, so that its launches allocate nothing.
+static int prepare_call(dida_letkf_handle* h) {
+  const auto& c = h->cfg;
+  const long long mb = h->max_batch, nb = n_batches_of(h);
+  if (h->row_cap <= 0) h->row_cap = estimate_row_cap(h);
+  const long long rc = std::max(h->row_cap, 1LL);
+  RC_TRY(h->cnt.ensure(sizeof(long long) * (mb + 1)));
+  RC_TRY(h->off.ensure(sizeof(long long) * (mb + 1)));
+  RC_TRY(h->scan_tmp.ensure(sizeof(long long) * scan_tmp_elems(mb + 1)));
+  RC_TRY(h->Abuf.ensure(sizeof(double) * mb * h->nzc * h->kp * h->kp));
+  RC_TRY(h->bbuf.ensure(sizeof(double) * mb * h->nzc * h->kp));
+  RC_TRY(h->status.ensure(sizeof(int) * h->n_an));
+  RC_TRY(h->nobsl_dev.ensure(sizeof(int) * std::max(h->n_h, 1LL)));
+  RC_TRY(h->batch_tot.ensure(sizeof(long long) * nb));
+  RC_TRY(h->row_idx.ensure(sizeof(int) * rc));
+  if (h->tiled) {
+    RC_TRY(h->row_site.ensure(sizeof(int) * rc));
+  } else {
+    RC_TRY(h->row_gc.ensure(sizeof(double) * rc));
+    if (h->nzc == 1) RC_TRY(h->row_wd.ensure(sizeof(double) * rc));
+  }
+  if (c.keep_transforms) {
+    RC_TRY(h->trans.ensure(sizeof(double) * h->n_an * c.k * c.k));
+    RC_TRY(h->wout.ensure(sizeof(double) * h->n_an * c.k));
+    h->nobsl_host.assign(h->n_an, 0);
+  }
+  RC_TRY(eig_prepare(h->eig, c.k, mb * h->nzc));
+  if (h->opt_schedule == 1 && !h->tiled) {
+    RC_TRY(h->b_order.ensure(sizeof(int) * mb * h->nzc));
+    RC_TRY(h->b_blk.ensure(sizeof(long long) * std::max(bucket_blk_elems((int)mb), 1LL)));
+    RC_TRY(h->b_seg.ensure(sizeof(long long) * (kBuckets + 1)));
+    RC_TRY(h->b_q.ensure(sizeof(int) * kBuckets));
+    RC_TRY(h->b_stats.ensure(sizeof(long long) * nb * 2 * kBuckets));
+    RC_TRY(h->b_ticks.ensure(sizeof(unsigned long long) * nb * 5 * kBuckets));
+  }
+  while ((long long)h->ev.size() < 9 * nb) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    h->ev.push_back(e);
+  }
+  return DIDA_OK;
+}
+
+// Measures each products variant on every non-empty bucket of the batch in
+// pp (buckets already built), keeps the fastest per bucket in the handle and
+// the cache, and leaves the queue counters zeroed.  Eager calls only.
+static int tune_products(dida_letkf_handle* h, hipStream_t st, ProdParams pp) {
+  const auto& c = h->cfg;
+  long long seg[kBuckets + 1];
+  HIP_TRY(hipMemcpyAsync(seg, h->b_seg.p, sizeof(seg), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const std::vector<int> vars = products_variants(h->kp, c.precision);
+  for (auto& e : h->g_ev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  hipEvent_t e0 = h->g_ev[0], e1 = h->g_ev[1];
+  double best_us[kBuckets];
+  int best[kBuckets];
+  for (int j = 0; j < kBuckets; ++j) {
+    best_us[j] = 1e300;
+    best[j] = h->variant[j];
+  }
+  pp.bucket_ticks = nullptr;
+  pp.launch_ticks = nullptr;
+  for (int rep = 0; rep < 2; ++rep) {
+    for (int g : vars) {
+      for (int j = 0; j < kBuckets; ++j) {
+        if (seg[j + 1] <= seg[j]) continue;
+        pp.seg_lo = j;
+        pp.seg_hi = j + 1;
+        pp.q_counter = h->b_q.as<int>() + j;
+        pp.groups = g;
+        HIP_TRY(launch_zero_ints(st, pp.q_counter, 1));
+        HIP_TRY(hipEventRecord(e0, st));
+        HIP_TRY(launch_products_queue(st, pp, c.precision));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (1000.0 * ms < best_us[j]) {
+          best_us[j] = 1000.0 * ms;
+          best[j] = g;
+        }
+      }
+    }
+  }
+  HIP_TRY(launch_zero_ints(st, h->b_q.as<int>(), kBuckets));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  for (int j = 0; j < kBuckets; ++j) {
+    if (seg[j + 1] <= seg[j]) continue;
+    h->variant[j] = best[j];
+    g_tune[tune_key(h->arch, c.k, c.precision, kBuckets - 1 - j)] = {best[j], best_us[j]};
+  }
+  return tune_save_locked();
+}
+
+// The launches of one analysis call, every batch in turn on the handle's
+// stream: no allocation and no host synchronisation (unless a tuning pass
+// runs), so that a call can be captured in a hipGraph.  timed: record the
+// per-batch events; tmode: run_transform's mode.
+static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, int tmode,
+                        bool tune) {
   const double* axb = sa.axb;
   const auto& c = h->cfg;
   hipStream_t st = h->stream;
-  if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
-  if (!h->have_obs) return set_error(DIDA_ERR_STATE, "set_obs was not called");
-  hipEvent_t ev[9];
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() {
-      for (int i = 0; i < 9; ++i) (void)hipEventDestroy(e[i]);
-    }
-  } guard{ev};
-  float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[4] = {0, 0, 0, 0};
-  HIP_TRY(hipEventRecord(ev[0], st));
-
   Geom g;
   g.glat = h->glat.as<float>();
   g.glon = h->glon.as<float>();
@@ -726,33 +1010,17 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
   g.cube_n = c.cube_n;
   g.nlat_c = h->nlat_c;
   g.nlon_c = h->nlon_c;
-
   const long long mb = h->max_batch;
-  RC_TRY(h->cnt.ensure(sizeof(long long) * (mb + 1)));
-  RC_TRY(h->off.ensure(sizeof(long long) * (mb + 1)));
-  size_t scan_bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->cnt.as<long long>(),
-                                           h->off.as<long long>(), (int)(mb + 1), st));
-  RC_TRY(h->scan_tmp.ensure(scan_bytes));
-  RC_TRY(h->Abuf.ensure(sizeof(double) * mb * h->nzc * h->kp * h->kp));
-  RC_TRY(h->bbuf.ensure(sizeof(double) * mb * h->nzc * h->kp));
-  RC_TRY(h->status.ensure(sizeof(int) * h->n_an));
-  if (c.keep_transforms) {
-    RC_TRY(h->trans.ensure(sizeof(double) * h->n_an * c.k * c.k));
-    RC_TRY(h->wout.ensure(sizeof(double) * h->n_an * c.k));
-    h->nobsl_host.assign(h->n_an, 0);
-  }
-  RC_TRY(eig_begin_call(st, h->eig, c.k, mb * h->nzc));
-  long long total_rows = 0;
-  int max_rows = 0, over_cap = 0;
-  std::vector<long long> cnt_host;
-  std::vector<int> nob_host;
-  // tiles of analyses share their staged obs rows (horizontal loop, k <= 64)
   const bool tiled = h->tiled;
+  const bool queue = h->opt_schedule == 1 && !tiled;
+  HIP_TRY(launch_zero_ints(st, h->eig.flags.as<int>(), 2));
 
-  for (long long h0 = 0; h0 < h->n_h; h0 += mb) {
+  long long bi = 0;
+  for (long long h0 = 0; h0 < h->n_h; h0 += mb, ++bi) {
     const long long h1 = std::min(h->n_h, h0 + mb);
     const int ncol = (int)(h1 - h0);
+    hipEvent_t* ev = h->ev.data() + 9 * bi;
+    if (timed) HIP_TRY(hipEventRecord(ev[0], st));
     SelectParams sp{};
     sp.g = g;
     sp.s_cos = h->s_cos.as<double>();
@@ -783,56 +1051,35 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
     sp.h1 = (int)h1;
     sp.cnt = h->cnt.as<long long>();
     sp.off = h->off.as<long long>();
-    sp.row_idx = nullptr;
-    sp.row_gc = nullptr;
-    sp.cap = 0;
     sp.tile = tiled ? kTile : 0;
-    sp.row_site = nullptr;
     // horizontal loop: weights precomputed by the fill pass
     sp.weights = h->nzc == 1 ? 1 : 0;
     sp.rinv32 = h->rinv32.as<float>();
     sp.dep = h->o_dep.as<double>();
-    sp.row_wd = nullptr;
-    // count pass -> offsets -> fill pass; per analysis (column mode) or per
-    // tile of kTile analyses (tile mode: the union of their candidate rows)
+    // count pass -> offsets -> fill pass into the fixed-capacity lists; per
+    // analysis (column mode) or per tile of kTile analyses (tile mode: the
+    // union of their candidate rows).  The list total goes to batch_tot and
+    // the rows per column to nobsl_dev (tile mode: from the products kernel).
     const int nlist = tiled ? (ncol + kTile - 1) / kTile : ncol;
-    HIP_TRY(hipMemsetAsync(h->cnt.p, 0, sizeof(long long) * (nlist + 1), st));
+    HIP_TRY(launch_zero_ll(st, h->cnt.as<long long>(), nlist + 1));
     if (tiled) HIP_TRY(launch_select_tile(st, sp, false));
     else HIP_TRY(launch_select(st, sp, false));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(h->scan_tmp.p, scan_bytes, h->cnt.as<long long>(),
-                                             h->off.as<long long>(), nlist + 1, st));
-    cnt_host.resize(nlist + 1);
-    HIP_TRY(hipMemcpyAsync(cnt_host.data(), h->off.p, sizeof(long long) * (nlist + 1),
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    const long long rows = cnt_host[nlist];
-    if (!tiled) {
-      for (int i = 0; i < ncol; ++i) {
-        long long r = cnt_host[i + 1] - cnt_host[i];
-        max_rows = std::max<long long>(max_rows, r);
-        if (r > 10000) ++over_cap;
-        if (c.keep_transforms)
-          for (int kc = 0; kc < h->nzc; ++kc) h->nobsl_host[(h0 + i) * h->nzc + kc] = (int)r;
-      }
-      total_rows += rows * h->nzc;
-    }
-    RC_TRY(h->row_idx.ensure(sizeof(int) * std::max(rows, 1LL)));
+    HIP_TRY(launch_exclusive_scan(st, h->cnt.as<long long>(), h->off.as<long long>(), nlist + 1,
+                                  h->scan_tmp.as<long long>()));
+    HIP_TRY(launch_list_counts(st, h->off.as<long long>(), nlist,
+                               tiled ? nullptr : h->nobsl_dev.as<int>() + h0,
+                               h->batch_tot.as<long long>() + bi));
     sp.row_idx = h->row_idx.as<int>();
-    sp.cap = rows;
+    sp.cap = h->row_cap;
     if (tiled) {
-      RC_TRY(h->row_site.ensure(sizeof(int) * std::max(rows, 1LL)));
       sp.row_site = h->row_site.as<int>();
       HIP_TRY(launch_select_tile(st, sp, true));
     } else {
-      RC_TRY(h->row_gc.ensure(sizeof(double) * std::max(rows, 1LL)));
       sp.row_gc = h->row_gc.as<double>();
-      if (sp.weights) {
-        RC_TRY(h->row_wd.ensure(sizeof(double) * std::max(rows, 1LL)));
-        sp.row_wd = h->row_wd.as<double>();
-      }
+      sp.row_wd = sp.weights ? h->row_wd.as<double>() : nullptr;
       HIP_TRY(launch_select(st, sp, true));
     }
-    HIP_TRY(hipEventRecord(ev[1], st));
+    if (timed) HIP_TRY(hipEventRecord(ev[1], st));
 
     ProdParams pp{};
     pp.g = g;
@@ -858,6 +1105,7 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
     pp.row_wd = h->row_wd.as<double>();
     pp.pre_weighted = (!tiled && h->nzc == 1) ? 1 : 0;
     pp.row_off = h->off.as<long long>();
+    pp.row_cap = h->row_cap;
     pp.yb64 = h->o_yb64.as<double>();
     pp.yb32 = h->yb32.as<float>();
     pp.dep = h->o_dep.as<double>();
@@ -865,10 +1113,7 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
     pp.vco = h->o_vco.as<double>();
     pp.a0 = h0 * h->nzc;
     pp.n_an = (h1 - h0) * h->nzc;
-    if (products_band() > 0 && h->nzc == 1 && h0 % g.nlat_c == 0 && (h1 - h0) % g.nlat_c == 0) {
-      pp.band_w = products_band();
-      pp.band_l = g.nlat_c;
-    }
+    pp.groups = 1;
     pp.h0 = (int)h0;
     pp.A = h->Abuf.as<double>();
     pp.b = h->bbuf.as<double>();
@@ -880,16 +1125,46 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
       pp.s_lonr = h->s_lonr.as<double>();
       pp.local_dist = c.local_dist;
       pp.gc_filter = c.gc_filter;
-      RC_TRY(h->nobsl_dev.ensure(sizeof(int) * std::max(ncol, 1)));
-      pp.nobsl = h->nobsl_dev.as<int>();
+      pp.nobsl = h->nobsl_dev.as<int>() + h0;
       HIP_TRY(launch_products_tile(st, pp, c.precision));
-      nob_host.resize(ncol);
-      HIP_TRY(hipMemcpyAsync(nob_host.data(), h->nobsl_dev.p, sizeof(int) * ncol,
-                             hipMemcpyDeviceToHost, st));
+    } else if (queue) {
+      BucketParams bp{};
+      bp.counts = h->nobsl_dev.as<int>() + h0;
+      bp.n_cols = ncol;
+      bp.nzc = h->nzc;
+      bp.blk = h->b_blk.as<long long>();
+      bp.seg_off = h->b_seg.as<long long>();
+      bp.bucket_stats = h->b_stats.as<long long>() + bi * 2 * kBuckets;
+      bp.order = h->b_order.as<int>();
+      bp.q_counters = h->b_q.as<int>();
+      bp.ticks = h->opt_timing ? h->b_ticks.as<unsigned long long>() + bi * 5 * kBuckets : nullptr;
+      HIP_TRY(launch_buckets(st, bp));
+      pp.order = bp.order;
+      pp.seg_off = bp.seg_off;
+      if (tune && bi == 0 && products_variants(h->kp, c.precision).size() > 1)
+        RC_TRY(tune_products(h, st, pp));
+      // runs of bucket positions that share a variant: one persistent launch each
+      for (int j0 = 0; j0 < kBuckets;) {
+        int j1 = j0 + 1;
+        while (j1 < kBuckets && h->variant[j1] == h->variant[j0]) ++j1;
+        pp.seg_lo = j0;
+        pp.seg_hi = j1;
+        pp.q_counter = bp.q_counters + j0;
+        pp.groups = h->variant[j0];
+        pp.bucket_ticks = bp.ticks;
+        pp.launch_ticks = bp.ticks ? bp.ticks + kBuckets + 4 * j0 : nullptr;
+        HIP_TRY(launch_products_queue(st, pp, c.precision));
+        j0 = j1;
+      }
     } else {
+      if (products_band() > 0 && h->nzc == 1 && h0 % g.nlat_c == 0 &&
+          (h1 - h0) % g.nlat_c == 0) {
+        pp.band_w = products_band();
+        pp.band_l = g.nlat_c;
+      }
       HIP_TRY(launch_products(st, pp, c.precision));
     }
-    HIP_TRY(hipEventRecord(ev[2], st));
+    if (timed) HIP_TRY(hipEventRecord(ev[2], st));
 
     XformParams xp{};
     xp.g = g;
@@ -936,62 +1211,221 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
     xp.status = h->status.as<int>();
     xp.list = nullptr;
     bool split = false;
-    RC_TRY(run_transform(st, h->eig, xp, c.transform_kernel, ev + 4, &split));
-    HIP_TRY(hipEventRecord(ev[3], st));
-    HIP_TRY(hipEventSynchronize(ev[3]));
-    if (tiled) {
-      // local obs counts of this batch (copied after the products kernel)
-      for (int i = 0; i < ncol; ++i) {
-        const long long r = nob_host[i];
-        max_rows = std::max<long long>(max_rows, r);
-        if (r > 10000) ++over_cap;
-        total_rows += r;
-        if (c.keep_transforms) h->nobsl_host[h0 + i] = (int)r;
-      }
-    }
-    if (split) {
-      for (int i = 0; i < 4; ++i) {
-        float t = 0;
-        HIP_TRY(hipEventElapsedTime(&t, ev[4 + i], ev[5 + i]));
-        ms_k[i] += t;
-      }
-    }
-    float t1 = 0, t2 = 0, t3 = 0;
-    HIP_TRY(hipEventElapsedTime(&t1, ev[0], ev[1]));
-    HIP_TRY(hipEventElapsedTime(&t2, ev[1], ev[2]));
-    HIP_TRY(hipEventElapsedTime(&t3, ev[2], ev[3]));
-    ms_sel += t1;
-    ms_prod += t2;
-    ms_xf += t3;
-    HIP_TRY(hipEventRecord(ev[0], st));
+    RC_TRY(run_transform(st, h->eig, xp, tmode, timed ? ev + 4 : nullptr, &split));
+    h->split_timed[bi] = timed && split;
+    if (timed) HIP_TRY(hipEventRecord(ev[3], st));
   }
-  // eigensolver flags (analyses still flagged after the list pass) and the
-  // count the list pass redid
-  std::vector<int> stt(h->n_an);
+  return DIDA_OK;
+}
+
+// After the call's launches: one readback (row totals and counts, status,
+// flags, bucket statistics), the stats, and the largest row total of a batch
+// in need (the call's lists were cut if it exceeds row_cap).  Returns the
+// number of analyses still flagged.
+static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms,
+                       dida_letkf_stats* stats, long long& need) {
+  const auto& c = h->cfg;
+  hipStream_t st = h->stream;
+  const long long nb = n_batches_of(h);
+  const bool queue = h->opt_schedule == 1 && !h->tiled;
+  std::vector<long long> tot(nb);
+  std::vector<int> counts(h->n_h), stt(h->n_an);
+  std::vector<long long> bst(queue ? nb * 2 * kBuckets : 0);
+  std::vector<unsigned long long> tk(queue && h->opt_timing ? nb * 5 * kBuckets : 0);
   int fl[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(tot.data(), h->batch_tot.p, sizeof(long long) * nb, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(counts.data(), h->nobsl_dev.p, sizeof(int) * h->n_h, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(stt.data(), h->status.p, sizeof(int) * h->n_an, hipMemcpyDeviceToHost, st));
-  if (c.transform_kernel != 1 || c.k > 64)
-    HIP_TRY(hipMemcpyAsync(fl, h->eig.flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(fl, h->eig.flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+  if (!bst.empty())
+    HIP_TRY(hipMemcpyAsync(bst.data(), h->b_stats.p, sizeof(long long) * bst.size(),
+                           hipMemcpyDeviceToHost, st));
+  if (!tk.empty())
+    HIP_TRY(hipMemcpyAsync(tk.data(), h->b_ticks.p, sizeof(unsigned long long) * tk.size(),
+                           hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  need = 0;
+  for (long long t : tot) need = std::max(need, t);
+  if (need > h->row_cap) return 0;
   int flagged = 0;
   for (int s : stt) flagged += (s != 0);
-  if (stats) {
-    stats->n_analyses = h->n_an;
-    stats->total_rows = total_rows;
-    stats->max_rows = max_rows;
-    stats->n_flagged = flagged;
-    stats->rows_over_10000 = over_cap;
-    stats->ms_select = ms_sel;
-    stats->ms_products = ms_prod;
-    stats->ms_transform = ms_xf;
-    stats->ms_total = ms_sel + ms_prod + ms_xf;
-    stats->ms_tridiag = ms_k[0];
-    stats->ms_ql = ms_k[1];
-    stats->ms_apply = ms_k[2];
-    stats->ms_update = ms_k[3];
-    stats->n_rerun = fl[1];
+  if (c.keep_transforms)
+    for (long long a = 0; a < h->n_an; ++a) h->nobsl_host[a] = counts[a / h->nzc];
+  if (!stats) return flagged;
+  long long total_rows = 0;
+  int max_rows = 0, over_cap = 0;
+  for (int r : counts) {
+    total_rows += (long long)r * h->nzc;
+    max_rows = std::max(max_rows, r);
+    over_cap += r > 10000;
   }
+  float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[4] = {0, 0, 0, 0};
+  if (timed) {
+    for (long long b = 0; b < nb; ++b) {
+      hipEvent_t* ev = h->ev.data() + 9 * b;
+      float t1 = 0, t2 = 0, t3 = 0;
+      HIP_TRY(hipEventElapsedTime(&t1, ev[0], ev[1]));
+      HIP_TRY(hipEventElapsedTime(&t2, ev[1], ev[2]));
+      HIP_TRY(hipEventElapsedTime(&t3, ev[2], ev[3]));
+      ms_sel += t1;
+      ms_prod += t2;
+      ms_xf += t3;
+      if (h->split_timed[b]) {
+        for (int i = 0; i < 4; ++i) {
+          float t = 0;
+          HIP_TRY(hipEventElapsedTime(&t, ev[4 + i], ev[5 + i]));
+          ms_k[i] += t;
+        }
+      }
+    }
+  }
+  *stats = dida_letkf_stats{};
+  stats->n_analyses = h->n_an;
+  stats->total_rows = total_rows;
+  stats->max_rows = max_rows;
+  stats->n_flagged = flagged;
+  stats->rows_over_10000 = over_cap;
+  stats->ms_select = ms_sel;
+  stats->ms_products = ms_prod;
+  stats->ms_transform = ms_xf;
+  stats->ms_total = timed ? ms_sel + ms_prod + ms_xf : graph_ms;
+  stats->ms_tridiag = ms_k[0];
+  stats->ms_ql = ms_k[1];
+  stats->ms_apply = ms_k[2];
+  stats->ms_update = ms_k[3];
+  stats->n_rerun = fl[1];
+  stats->schedule = queue ? 1 : 0;
+  if (queue) {
+    double busy = 0, cap = 0;
+    for (long long b = 0; b < nb; ++b) {
+      for (int i = 0; i < kBuckets; ++i) {
+        stats->bucket_analyses[i] += bst[b * 2 * kBuckets + i];
+        stats->bucket_rows[i] += bst[b * 2 * kBuckets + kBuckets + i];
+      }
+      if (tk.empty()) continue;
+      const unsigned long long* t = tk.data() + b * 5 * kBuckets;
+      for (int j = 0; j < kBuckets; ++j)
+        stats->bucket_wave_ms[kBuckets - 1 - j] += (double)t[j] / h->clock_khz;
+      for (int j = 0; j < kBuckets; ++j) {
+        const unsigned long long* l = t + kBuckets + 4 * j;
+        if (l[3] == 0 || l[2] == 0) continue;
+        busy += (double)l[2];
+        cap += (double)l[3] * (double)(l[1] - l[0]);
+      }
+    }
+    stats->products_balance = cap > 0 ? busy / cap : 0.0;
+  }
+  for (int j = 0; j < kBuckets; ++j) stats->variant[j] = h->variant[kBuckets - 1 - j];
   return flagged;
+}
+
+// Graph key: everything a captured call holds by value (pointers, sizes,
+// options), compared before each replay.
+static std::vector<uintptr_t> graph_key(const dida_letkf_handle* h, const StateArgs& sa) {
+  std::vector<uintptr_t> k;
+  auto put = [&](const void* p) { k.push_back((uintptr_t)p); };
+  auto putd = [&](double d) {
+    uintptr_t u = 0;
+    std::memcpy(&u, &d, sizeof(d));
+    k.push_back(u);
+  };
+  put(h->stream);
+  k.push_back((uintptr_t)h->obs_version);
+  k.push_back((uintptr_t)h->row_cap);
+  k.push_back((uintptr_t)h->opt_schedule);
+  k.push_back((uintptr_t)h->opt_timing);
+  for (int j = 0; j < kBuckets; ++j) k.push_back((uintptr_t)h->variant[j]);
+  put(sa.xb), put(sa.xb32), put(sa.axb), put(sa.xa), put(sa.xa32), put(sa.axa);
+  k.push_back((uintptr_t)sa.epi);
+  putd(sa.polar_lat), putd(sa.gamma1), putd(sa.gamma2);
+  put(sa.xa_out), put(sa.spread_b), put(sa.spread_a);
+  for (const DevBuf* b : {&h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc, &h->row_wd,
+                          &h->row_site, &h->nobsl_dev, &h->Abuf, &h->bbuf, &h->status, &h->trans,
+                          &h->wout, &h->batch_tot, &h->b_order, &h->b_blk, &h->b_seg, &h->b_q,
+                          &h->b_stats, &h->b_ticks, &h->eig.refl, &h->eig.tau, &h->eig.d,
+                          &h->eig.e, &h->eig.lam, &h->eig.rot, &h->eig.hdr, &h->eig.nchase,
+                          &h->eig.flags, &h->eig.rr_rot, &h->eig.rr_hdr, &h->eig.rr_nchase,
+                          &h->eig.rr_lam})
+    put(b->p);
+  return k;
+}
+
+static void drop_graph(dida_letkf_handle* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->gkey.clear();
+}
+
+// Captures the call's launches (single-stream transform, no events) unless
+// the graph of an identical earlier call exists, and launches the graph.
+static int graph_call(dida_letkf_handle* h, const StateArgs& sa, float& ms) {
+  hipStream_t st = h->stream;
+  if (!st) return set_error(DIDA_ERR_STATE, "graph mode needs a non-null stream");
+  std::vector<uintptr_t> key = graph_key(h, sa);
+  if (!h->gexec || key != h->gkey) {
+    drop_graph(h);
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_call(h, sa, false, 2, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc < 0 || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc < 0) return rc;
+      return set_error(DIDA_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    h->graph = g;
+    HIP_TRY(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    h->gkey = key;
+  }
+  for (auto& e : h->g_ev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(h->g_ev[0], st));
+  HIP_TRY(hipGraphLaunch(h->gexec, st));
+  HIP_TRY(hipEventRecord(h->g_ev[1], st));
+  HIP_TRY(hipEventSynchronize(h->g_ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms, h->g_ev[0], h->g_ev[1]));
+  return DIDA_OK;
+}
+
+static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_stats* stats) {
+  if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
+  if (!h->have_obs) return set_error(DIDA_ERR_STATE, "set_obs was not called");
+  RC_TRY(prepare_call(h));
+  h->split_timed.assign(n_batches_of(h), false);
+  bool graph = h->opt_graph != 0;
+  bool reran = false;
+  for (int attempt = 0;; ++attempt) {
+    float gms = 0;
+    const bool tune = !graph && h->opt_tune != 0 && h->opt_schedule == 1 && !h->tiled;
+    if (graph) {
+      RC_TRY(graph_call(h, sa, gms));
+    } else {
+      RC_TRY(enqueue_call(h, sa, true, h->cfg.transform_kernel, tune));
+      if (tune) h->opt_tune = 0;
+    }
+    long long need = 0;
+    const int flagged = finish_call(h, !graph, gms, stats, need);
+    if (flagged < 0) return flagged;
+    if (need <= h->row_cap) {
+      if (stats) {
+        stats->graph_replay = graph ? 1 : 0;
+        stats->reran_rows = reran ? 1 : 0;
+        stats->row_cap = h->row_cap;
+      }
+      return flagged;
+    }
+    // the lists were cut at the capacity: grow it to the measured need and
+    // run the call again (eagerly; a graph is captured again on a later call)
+    if (attempt > 0)
+      return set_error(DIDA_ERR_STATE, "row lists grew between two passes over the same obs");
+    h->row_cap = need + need / 8 + 4096;
+    drop_graph(h);
+    RC_TRY(prepare_call(h));
+    graph = false;
+    reran = true;
+  }
 }
 
 extern "C" {
@@ -1215,6 +1649,8 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
   pp.row_idx = cb.idx.as<int>();
   pp.row_gc = cb.g.as<double>();
   pp.row_off = cb.off.as<long long>();
+  pp.row_cap = nr;
+  pp.groups = 1;
   pp.yb64 = cb.y.as<double>();
   pp.yb32 = precision != DIDA_PREC_FP64 ? cb.y32.as<float>() : nullptr;
   pp.dep = cb.d.as<double>();

@@ -81,6 +81,7 @@ struct ProdParams {
   const double* row_wd;      // w * dep when pre_weighted
   int pre_weighted;
   const long long* row_off;  // per batch column, n_cols + 1 entries
+  long long row_cap;         // entries of row_idx / row_gc / row_wd
   const double* yb64;
   const float* yb32;
   const double* dep;
@@ -104,7 +105,49 @@ struct ProdParams {
   double local_dist;
   int gc_filter;
   int* nobsl;                // tile mode: local obs rows per analysis (batch-local)
+  // queue schedule: the launch's wavefront groups take analyses one at a
+  // time from positions [seg_off[seg_lo], seg_off[seg_hi]) of `order` (the
+  // length buckets, longest first) through the counter q_counter
+  const long long* seg_off;
+  int seg_lo, seg_hi;
+  int* q_counter;
+  int groups;                // row groups of 32 loaded per step (1 or 2; k <= 64 BF16)
+  // optional timing (queue schedule): wall-clock ticks summed per bucket
+  // position, and per launch the earliest start, latest end and summed busy
+  // ticks of its workers and the worker count (4 entries)
+  unsigned long long* bucket_ticks;
+  unsigned long long* launch_ticks;
 };
+
+// Length buckets over the local obs count p of a column: 0: p = 0,
+// 1: 1..256, 2: 257..512, ... 6: 4097..8192, 7: more.  The order list holds
+// the buckets longest first: bucket b at position kBuckets - 1 - b.
+constexpr int kBuckets = 8;
+__host__ __device__ inline int bucket_of(long long p) {
+  if (p <= 0) return 0;
+  if (p <= 256) return 1;
+  int b = 1;
+  long long e = 256;
+  while (b < kBuckets - 1 && p > e) {
+    e *= 2;
+    ++b;
+  }
+  return b;
+}
+
+struct BucketParams {
+  const int* counts;         // rows per column (batch-local), n_cols entries
+  int n_cols, nzc;
+  long long* blk;            // per block of kBucketBlock columns: kBuckets counts, then
+                             // kBuckets row sums, then kBuckets offsets
+  long long* seg_off;        // kBuckets + 1 positions (analyses) in the order list
+  long long* bucket_stats;   // kBuckets analyses, kBuckets rows
+  int* order;                // n_cols * nzc batch-local analysis offsets
+  int* q_counters;           // zeroed by the scan kernel (kBuckets entries)
+  unsigned long long* ticks; // optional: kBuckets bucket sums, then 4 per bucket
+                             // position for launch spans (reset by the scan kernel)
+};
+constexpr int kBucketBlock = 1024;
 
 struct XformParams {
   Geom g;
@@ -239,8 +282,23 @@ hipError_t launch_site_gather(hipStream_t st, int n_sites, const int* order, con
                               double* s_lonr, double* s_cosl, double* s_sinl, int* s_row0,
                               int* s_nrow);
 hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill);
+// exclusive prefix sum of n long longs (tmp: scan_tmp_elems(n) entries)
+long long scan_tmp_elems(long long n);
+hipError_t launch_exclusive_scan(hipStream_t st, const long long* in, long long* out, long long n,
+                                 long long* tmp);
+// after a batch's scan: per-list counts into counts[0..n_list) (off[i+1]-off[i]) and
+// the batch's total off[n_list] into *total
+hipError_t launch_list_counts(hipStream_t st, const long long* off, int n_list, int* counts,
+                              long long* total);
+hipError_t launch_zero_ll(hipStream_t st, long long* p, long long n);
 hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
+// queue schedule: a persistent launch over the bucket positions
+// [p.seg_lo, p.seg_hi) (p.order, p.seg_off and p.q_counter set)
+hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision);
+// length buckets of a batch's columns (BucketParams.blk: bucket_blk_elems)
+long long bucket_blk_elems(int n_cols);
+hipError_t launch_buckets(hipStream_t st, const BucketParams& p);
 hipError_t launch_products_tile(hipStream_t st, const ProdParams& p, int precision);
 hipError_t launch_transform(hipStream_t st, const XformParams& p);
 hipError_t launch_epilogue(hipStream_t st, const EpiParams& p);

@@ -17,6 +17,8 @@
 // with K0 = 1 (x4 shapes) or 8 consecutive k per lane (16x16x32 bf16).
 #include "letkf_kernels.h"
 
+#include <algorithm>
+#include <atomic>
 #include <type_traits>
 #include "letkf_device.h"
 
@@ -400,6 +402,69 @@ __device__ __forceinline__ long long analysis_at(const ProdParams& p, long long 
   return (b * B + rem % Bb) * L + rem / Bb;
 }
 
+// The analyses a products launch visits.  Plain schedule: each group of WPA
+// wavefronts takes the analysis at its logical position.  Queue schedule (a
+// persistent launch): the groups take positions of the order list from a
+// counter until the launch's bucket segment is used up, so a long row list
+// never leaves the rest of a workgroup idle while other groups wait to start.
+// WPA is 1 (a wavefront per analysis) or 4 (the whole 256-lane workgroup).
+// body(li) runs the products of batch-local analysis li; with timing on, the
+// wall-clock ticks of each analysis go to its bucket position, and each
+// worker adds its first start, last end and busy ticks, and itself, to the
+// launch's four totals.
+template <int WPA, class Body>
+__device__ __forceinline__ void for_analyses(const ProdParams& p, int lane, Body body) {
+  static_assert(WPA == 1 || WPA == 4, "a wavefront or a workgroup per analysis");
+  const int wave = threadIdx.x >> 6;
+  if (!p.q_counter) {
+    const int apb = (blockDim.x >> 6) / WPA;
+    const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
+    if (li >= p.n_an) return;
+    body(analysis_at(p, li));
+    return;
+  }
+  const long long lo = p.seg_off[p.seg_lo], hi = p.seg_off[p.seg_hi];
+  const bool timing = p.bucket_ticks != nullptr;
+  unsigned long long busy = 0, t_first = ~0ull, t_last = 0;
+  __shared__ int s_next;
+  for (;;) {
+    int v;
+    if constexpr (WPA == 1) {
+      v = 0;
+      if (lane == 0) v = atomicAdd(p.q_counter, 1);
+      v = __shfl(v, 0, 64);
+    } else {
+      if (threadIdx.x == 0) s_next = atomicAdd(p.q_counter, 1);
+      __syncthreads();
+      v = s_next;
+      __syncthreads();
+    }
+    const long long pos = lo + v;
+    if (pos >= hi) break;
+    const unsigned long long t0 = timing ? wall_clock64() : 0;
+    body((long long)p.order[pos]);
+    if (timing) {
+      const unsigned long long t1 = wall_clock64();
+      busy += t1 - t0;
+      t_first = t0 < t_first ? t0 : t_first;
+      t_last = t1;
+      if (lane == 0 && (WPA == 1 || wave == 0)) {
+        int j = p.seg_lo;
+        while (j + 1 < p.seg_hi && pos >= p.seg_off[j + 1]) ++j;
+        atomicAdd(&p.bucket_ticks[j], t1 - t0);
+      }
+    }
+  }
+  if (timing && lane == 0 && (WPA == 1 || wave == 0)) {
+    if (busy > 0) {
+      atomicMin(&p.launch_ticks[0], t_first);
+      atomicMax(&p.launch_ticks[1], t_last);
+      atomicAdd(&p.launch_ticks[2], busy);
+    }
+    atomicAdd(&p.launch_ticks[3], 1ull);
+  }
+}
+
 // Reduced-precision products: row lists longer than kFlushMinRows add their
 // FP32 partial sums to FP64 every kFlushRows rows.
 constexpr int kFlushMinRows = 3072;
@@ -519,20 +584,22 @@ __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   constexpr int WPA = (NPAIR + 9) / 10;
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int apb = (blockDim.x >> 6) / WPA;
-  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (li >= p.n_an) return;
-  const long long a = p.a0 + analysis_at(p, li);
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
-  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-  if (wsub == 0) products_f64_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
-  if constexpr (WPA > 1) {
-    if (wsub == 1) products_f64_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 2) products_f64_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 3) products_f64_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
-  }
+  for_analyses<WPA>(p, lane, [&](long long li) {
+    const long long a = p.a0 + li;
+    const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+    // row lists past the list buffer's capacity (the call then reruns with a
+    // larger one) are cut at it, so no read leaves the buffer
+    const long long r0 = min(p.row_off[ah - p.h0], p.row_cap);
+    const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
+    const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+    if (wsub == 0) products_f64_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+    if constexpr (WPA > 1) {
+      if (wsub == 1) products_f64_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 2) products_f64_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 3) products_f64_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+    }
+  });
 }
 
 template <int KP, int WS, int PPW>
@@ -664,20 +731,22 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   constexpr int WPA = (NPAIR + 9) / 10;
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int apb = (blockDim.x >> 6) / WPA;
-  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (li >= p.n_an) return;
-  const long long a = p.a0 + analysis_at(p, li);
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
-  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-  if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
-  if constexpr (WPA > 1) {
-    if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-    if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
-  }
+  for_analyses<WPA>(p, lane, [&](long long li) {
+    const long long a = p.a0 + li;
+    const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+    // row lists past the list buffer's capacity (the call then reruns with a
+    // larger one) are cut at it, so no read leaves the buffer
+    const long long r0 = min(p.row_off[ah - p.h0], p.row_cap);
+    const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
+    const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+    if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+    if constexpr (WPA > 1) {
+      if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+    }
+  });
 }
 
 // BF16 split of yhat = sqrt(w) y (FP32), FP32 accumulation (b = sum w d y
@@ -706,7 +775,7 @@ __device__ __forceinline__ void group_row(const ProdParams& p, long long r0, int
   wdf = in ? (float)wd : 0.f;
 }
 
-template <int KP, int WS, int PPW>
+template <int KP, int WS, int PPW, int G = 1>
 __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long long a, int ah,
                                                    double vc, long long r0, long long r1,
                                                    int lane) {
@@ -724,43 +793,58 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   for (int t = 0; t < T; ++t) bacc[t] = 0.f;
   const int n = (int)(r1 - r0);
   const int* __restrict__ ridx = p.row_idx + r0;
-  // the row indices of the next group are loaded one group ahead, so only
-  // the Yb gather of a group waits on memory; positions past the end read
-  // the list's last entry
-  int idx_n[8];
+  // G groups of 32 rows per step (more gathers in flight for G = 2; the
+  // MFMAs and the b sums run in the same order, so the sums are the same
+  // bits for every G).  The row indices of the next step are loaded one step
+  // ahead, so only the Yb gathers wait on memory; positions past the end
+  // read the list's last entry and enter with weight 0.
+  int idx_n[G][8];
   if (n > 0) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(8 * q + u, n - 1)];
+    for (int gg = 0; gg < G; ++gg)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) idx_n[gg][u] = ridx[min(32 * gg + 8 * q + u, n - 1)];
   }
-  for (int g0 = 0; g0 < n; g0 += 32) {
-    bf8 hi[T], lo[T];
-    int idx[8];
+  for (int g0 = 0; g0 < n; g0 += 32 * G) {
+    bf8 hi[G][T], lo[G][T];
+    int idx[G][8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) idx[u] = idx_n[u];
+    for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-    for (int u = 0; u < 8; ++u) idx_n[u] = ridx[min(g0 + 32 + 8 * q + u, n - 1)];
+      for (int u = 0; u < 8; ++u) idx[gg][u] = idx_n[gg][u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      float s, wdf;
-      group_row(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
-      const float* yr = p.yb32 + (long long)idx[u] * p.k;
+    for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const int m = 16 * t + i16;
-        const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
-        const float yh = s * yv;
-        const __bf16 h = (__bf16)yh;
-        hi[t][u] = h;
-        lo[t][u] = (__bf16)(yh - (float)h);
-        if (WS == 0) bacc[t] = fmaf(wdf, yv, bacc[t]);
+      for (int u = 0; u < 8; ++u)
+        idx_n[gg][u] = ridx[min(g0 + 32 * G + 32 * gg + 8 * q + u, n - 1)];
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        float s, wdf;
+        group_row(p, r0, n, g0 + 32 * gg + 8 * q + u, idx[gg][u], vc, s, wdf);
+        const float* yr = p.yb32 + (long long)idx[gg][u] * p.k;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const int m = 16 * t + i16;
+          const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
+          const float yh = s * yv;
+          const __bf16 h = (__bf16)yh;
+          hi[gg][t][u] = h;
+          lo[gg][t][u] = (__bf16)(yh - (float)h);
+          if (WS == 0) bacc[t] = fmaf(wdf, yv, bacc[t]);
+        }
       }
     }
 #pragma unroll
-    for (int x = 0; x < NP; ++x) {
-      const int I = PT::I(P0 + x), J = PT::J(P0 + x);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], hi[J], acc[x], 0, 0, 0);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], lo[J], acc[x], 0, 0, 0);
-      acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[I], hi[J], acc[x], 0, 0, 0);
+    for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+      for (int x = 0; x < NP; ++x) {
+        const int I = PT::I(P0 + x), J = PT::J(P0 + x);
+        acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[gg][I], hi[gg][J], acc[x], 0, 0, 0);
+        acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[gg][I], lo[gg][J], acc[x], 0, 0, 0);
+        acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[gg][I], hi[gg][J], acc[x], 0, 0, 0);
+      }
     }
   }
   double* A = p.A + (a - p.a0) * (long long)KP * KP;
@@ -927,7 +1011,7 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
   }
 }
 
-template <int KP>
+template <int KP, int G>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
@@ -935,24 +1019,29 @@ void k_products_bf16(ProdParams p) {
   constexpr int WPA = (NPAIR + 9) / 10;
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int apb = (blockDim.x >> 6) / WPA;
-  const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
   const int wsub = wave % WPA;
-  if (li >= p.n_an) return;
-  const long long a = p.a0 + analysis_at(p, li);
-  const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
-  const long long r0 = p.row_off[ah - p.h0], r1 = p.row_off[ah - p.h0 + 1];
-  const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-  if constexpr (WPA == 1) {
-    products_bf16_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
-  } else {
-    static_assert(WPA == 4 && KP % 64 == 0, "four wavefronts share an analysis");
-    __shared__ bf8 sh[2 * 3 * (KP / 16) * 64];
-    if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-    if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-    if (wsub == 2) products_bf16_shared<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-    if (wsub == 3) products_bf16_shared<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-  }
+  if constexpr (WPA == 4) static_assert(KP % 64 == 0, "four wavefronts share an analysis");
+  __shared__ bf8 sh[WPA == 1 ? 1 : 2 * 3 * (KP / 16) * 64];
+  for_analyses<WPA>(p, lane, [&](long long li) {
+    const long long a = p.a0 + li;
+    const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
+    // row lists past the list buffer's capacity (the call then reruns with a
+    // larger one) are cut at it, so no read leaves the buffer
+    const long long r0 = min(p.row_off[ah - p.h0], p.row_cap);
+    const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
+    const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
+    if constexpr (WPA == 1) {
+      products_bf16_wave<KP, 0, PPW, G>(p, a, ah, vc, r0, r1, lane);
+    } else {
+      if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 2) products_bf16_shared<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 3) products_bf16_shared<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+      // the next analysis' first stage must not overwrite operands another
+      // wavefront still reads
+      __syncthreads();
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1301,6 +1390,232 @@ hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill) {
 }
 
 // ---------------------------------------------------------------------------
+// exclusive prefix sum of the row counts (three kernels, no host round trip:
+// chunk scans, a scan of the chunk totals, the chunk offsets added)
+// ---------------------------------------------------------------------------
+constexpr int kScanChunk = 1024;        // elements per workgroup (256 threads x 4)
+
+__device__ __forceinline__ long long block_excl_scan_256(long long v, long long* sh,
+                                                         long long& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  long long before = 0;
+  for (int i = 0; i < w; ++i) before += sh[i];
+  total = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_chunks(const long long* __restrict__ in,
+                                                     long long* __restrict__ out, long long n,
+                                                     long long* __restrict__ chunk_tot) {
+  __shared__ long long sh[4];
+  const long long base = (long long)blockIdx.x * kScanChunk + 4 * threadIdx.x;
+  long long v[4], s = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    v[u] = base + u < n ? in[base + u] : 0;
+    s += v[u];
+  }
+  long long total;
+  long long ex = block_excl_scan_256(s, sh, total);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (base + u < n) out[base + u] = ex;
+    ex += v[u];
+  }
+  if (threadIdx.x == 0) chunk_tot[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of the chunk totals in place
+__global__ __launch_bounds__(256) void k_scan_totals(long long* __restrict__ tot, long long nb) {
+  __shared__ long long sh[4];
+  long long carry = 0;
+  for (long long c0 = 0; c0 < nb; c0 += 256) {
+    const long long i = c0 + threadIdx.x;
+    const long long v = i < nb ? tot[i] : 0;
+    long long total;
+    const long long ex = block_excl_scan_256(v, sh, total);
+    if (i < nb) tot[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(long long* __restrict__ out, long long n,
+                                                  const long long* __restrict__ chunk_off) {
+  const long long base = (long long)blockIdx.x * kScanChunk + 4 * threadIdx.x;
+  const long long add = chunk_off[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (base + u < n) out[base + u] += add;
+}
+
+long long scan_tmp_elems(long long n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+hipError_t launch_exclusive_scan(hipStream_t st, const long long* in, long long* out, long long n,
+                                 long long* tmp) {
+  if (n <= 0) return hipSuccess;
+  const long long nb = (n + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nb), dim3(256), 0, st, in, out, n, tmp);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(256), 0, st, tmp, nb);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, st, out, n,
+                     (const long long*)tmp);
+  return hipGetLastError();
+}
+
+__global__ void k_list_counts(const long long* __restrict__ off, int n_list, int* __restrict__ counts,
+                              long long* __restrict__ total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_list && counts) counts[i] = (int)(off[i + 1] - off[i]);
+  if (i == 0) *total = off[n_list];
+}
+
+hipError_t launch_list_counts(hipStream_t st, const long long* off, int n_list, int* counts,
+                              long long* total) {
+  hipLaunchKernelGGL(k_list_counts, dim3((unsigned)((n_list + 255) / 256 + (n_list == 0))),
+                     dim3(256), 0, st, off, n_list, counts, total);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_ll(long long* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_ll(hipStream_t st, long long* p, long long n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_ll, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Length buckets of a batch's columns over their local obs count p: a stable
+// counting sort (columns keep their order within a bucket, so neighbouring
+// columns, which share obs rows in L2, stay close in the queue), buckets
+// longest first.  Three launches: per-block counts, one workgroup scanning the
+// block counts per bucket, and the scatter.
+
+long long bucket_blk_elems(int n_cols) {
+  return (long long)((n_cols + kBucketBlock - 1) / kBucketBlock) * 3 * kBuckets;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_blocks(BucketParams p) {
+  __shared__ int s_cnt[kBuckets];
+  __shared__ unsigned long long s_rows[kBuckets];
+  if (threadIdx.x < kBuckets) {
+    s_cnt[threadIdx.x] = 0;
+    s_rows[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  const int base = blockIdx.x * kBucketBlock + 4 * threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int c = base + u;
+    if (c < p.n_cols) {
+      const int r = p.counts[c];
+      const int b = bucket_of(r);
+      atomicAdd(&s_cnt[b], 1);
+      atomicAdd(&s_rows[b], (unsigned long long)r);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets) {
+    long long* blk = p.blk + (long long)blockIdx.x * 3 * kBuckets;
+    blk[threadIdx.x] = s_cnt[threadIdx.x];
+    blk[kBuckets + threadIdx.x] = (long long)s_rows[threadIdx.x];
+  }
+}
+
+// one workgroup: bucket totals, the segment starts (analyses, longest bucket
+// first) and each block's first column position within its bucket; zeroes
+// the queue counters
+__global__ __launch_bounds__(64) void k_bucket_scan(BucketParams p, int n_blk) {
+  __shared__ long long s_tot[kBuckets];
+  __shared__ long long s_start[kBuckets];
+  const int b = threadIdx.x;
+  if (b < kBuckets) {
+    long long t = 0, r = 0;
+    for (int i = 0; i < n_blk; ++i) {
+      t += p.blk[(long long)i * 3 * kBuckets + b];
+      r += p.blk[(long long)i * 3 * kBuckets + kBuckets + b];
+    }
+    s_tot[b] = t;
+    p.bucket_stats[b] = t * p.nzc;
+    p.bucket_stats[kBuckets + b] = r * p.nzc;
+    p.q_counters[b] = 0;
+    if (p.ticks) {
+      p.ticks[b] = 0;
+      p.ticks[kBuckets + 4 * b] = ~0ull;
+      p.ticks[kBuckets + 4 * b + 1] = 0;
+      p.ticks[kBuckets + 4 * b + 2] = 0;
+      p.ticks[kBuckets + 4 * b + 3] = 0;
+    }
+  }
+  __syncthreads();
+  if (b == 0) {
+    long long at = 0;
+    for (int j = 0; j < kBuckets; ++j) {
+      const int bb = kBuckets - 1 - j;
+      s_start[bb] = at;
+      p.seg_off[j] = at * p.nzc;
+      at += s_tot[bb];
+    }
+    p.seg_off[kBuckets] = at * p.nzc;
+  }
+  __syncthreads();
+  if (b < kBuckets) {
+    long long at = s_start[b];
+    for (int i = 0; i < n_blk; ++i) {
+      long long* blk = p.blk + (long long)i * 3 * kBuckets;
+      blk[2 * kBuckets + b] = at;
+      at += blk[b];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(BucketParams p) {
+  __shared__ long long sh[4];
+  const int base = blockIdx.x * kBucketBlock + 4 * threadIdx.x;
+  int bk[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) bk[u] = base + u < p.n_cols ? bucket_of(p.counts[base + u]) : -1;
+  const long long* blk = p.blk + (long long)blockIdx.x * 3 * kBuckets;
+  for (int b = 0; b < kBuckets; ++b) {
+    long long mine = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) mine += bk[u] == b;
+    long long total;
+    long long rank = block_excl_scan_256(mine, sh, total);
+    if (total == 0) continue;
+    const long long first = blk[2 * kBuckets + b];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (bk[u] != b) continue;
+      const long long pos = first + rank++;
+      const int c = base + u;
+      for (int kc = 0; kc < p.nzc; ++kc) p.order[pos * p.nzc + kc] = c * p.nzc + kc;
+    }
+  }
+}
+
+hipError_t launch_buckets(hipStream_t st, const BucketParams& p) {
+  const int n_blk = (p.n_cols + kBucketBlock - 1) / kBucketBlock;
+  if (n_blk > 0)
+    hipLaunchKernelGGL(k_bucket_blocks, dim3((unsigned)n_blk), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, st, p, n_blk);
+  if (n_blk > 0)
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((unsigned)n_blk), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Tile-union products (horizontal loop, k <= 64).  A workgroup takes a tile
 // of NA consecutive analyses; the union of their candidate rows (k_select_tile)
 // is staged through LDS CH rows at a time, once for the whole tile, and every
@@ -1330,7 +1645,7 @@ __global__ __launch_bounds__(NA * 64) void k_products_tile(ProdParams p) {
   const long long ta0 = (long long)tile * p.tile;          // batch-local first analysis
   const long long a_loc = ta0 + wave;
   const bool act = wave < p.tile && a_loc < p.n_an;
-  const long long u0 = p.row_off[tile], u1 = p.row_off[tile + 1];
+  const long long u0 = min(p.row_off[tile], p.row_cap), u1 = min(p.row_off[tile + 1], p.row_cap);
   // this wavefront's analysis centre
   double c2 = 0.0, s2 = 0.0, l2 = 0.0;
   if (act) {
@@ -1508,6 +1823,39 @@ hipError_t launch_products_tile(hipStream_t st, const ProdParams& p, int precisi
   return hipGetLastError();
 }
 
+// Workgroups of kernel f resident on the whole device at once (the grid of a
+// queue-schedule launch), computed once per kernel.
+static int resident_blocks(const void* f, std::atomic<int>& cache) {
+  int v = cache.load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  v = std::max(1, per_cu) * std::max(1, cus);
+  cache.store(v, std::memory_order_relaxed);
+  return v;
+}
+
+template <class K>
+static hipError_t launch_prod(hipStream_t st, const ProdParams& p, K* kern, long long blocks,
+                              std::atomic<int>& cache) {
+  ProdParams q = p;
+  unsigned grid;
+  if (q.q_counter) {
+    const int r = resident_blocks(reinterpret_cast<const void*>(kern), cache);
+    if (r <= 0) return hipErrorInvalidDeviceFunction;
+    grid = (unsigned)std::min<long long>(r, blocks);
+    q.n_blocks = (int)grid;
+  } else {
+    q.n_blocks = (int)blocks;
+    grid = (unsigned)(((blocks + 7) / 8) * 8);
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, q);
+  return hipGetLastError();
+}
+
 template <int KP>
 static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int precision) {
   constexpr int T = KP / 16;
@@ -1516,13 +1864,16 @@ static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int pr
   constexpr int APB = 4 / WPA;
   const long long blocks = (p.n_an + APB - 1) / APB;
   if (blocks <= 0) return hipSuccess;
-  ProdParams q = p;
-  q.n_blocks = (int)blocks;
-  const unsigned grid = (unsigned)(((blocks + 7) / 8) * 8);
-  if (precision == 0) hipLaunchKernelGGL(k_products_f64<KP>, dim3(grid), dim3(256), 0, st, q);
-  else if (precision == 1) hipLaunchKernelGGL(k_products_f32<KP>, dim3(grid), dim3(256), 0, st, q);
-  else hipLaunchKernelGGL(k_products_bf16<KP>, dim3(grid), dim3(256), 0, st, q);
-  return hipGetLastError();
+  static std::atomic<int> c64{0}, c32{0}, c16a{0}, c16b{0};
+  if (precision == 0) return launch_prod(st, p, k_products_f64<KP>, blocks, c64);
+  if (precision == 1) return launch_prod(st, p, k_products_f32<KP>, blocks, c32);
+  if (WPA == 1 && p.groups == 2) return launch_prod(st, p, k_products_bf16<KP, 2>, blocks, c16b);
+  return launch_prod(st, p, k_products_bf16<KP, 1>, blocks, c16a);
+}
+
+hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision) {
+  if (!p.q_counter || !p.order || !p.seg_off || p.seg_lo >= p.seg_hi) return hipErrorInvalidValue;
+  return launch_products(st, p, precision);
 }
 
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision) {

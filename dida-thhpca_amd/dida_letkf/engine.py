@@ -77,15 +77,30 @@ class DidaStats(C.Structure):
                 ("ms_select", C.c_double), ("ms_products", C.c_double),
                 ("ms_transform", C.c_double), ("ms_total", C.c_double),
                 ("ms_tridiag", C.c_double), ("ms_ql", C.c_double), ("ms_apply", C.c_double),
-                ("ms_update", C.c_double), ("n_rerun", C.c_int)]
+                ("ms_update", C.c_double), ("n_rerun", C.c_int),
+                ("schedule", C.c_int), ("bucket_analyses", C.c_longlong * 8),
+                ("bucket_rows", C.c_longlong * 8), ("bucket_wave_ms", C.c_double * 8),
+                ("products_balance", C.c_double), ("variant", C.c_int * 8),
+                ("graph_replay", C.c_int), ("reran_rows", C.c_int), ("row_cap", C.c_longlong)]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        out = {}
+        for f, _ in self._fields_:
+            v = getattr(self, f)
+            out[f] = list(v) if isinstance(v, C.Array) else v
+        return out
+
+
+# dida_letkf_set_option
+OPT_SCHEDULE, OPT_TIMING, OPT_TUNE, OPT_GRAPH, OPT_ROW_CAP = 1, 2, 3, 4, 5
+# length buckets of the local obs count p (upper bounds; the last is open)
+BUCKET_EDGES = (0, 256, 512, 1024, 2048, 4096, 8192)
 
 
 EXPORTS = [
     "dida_letkf_config_default", "dida_letkf_create", "dida_letkf_destroy",
-    "dida_letkf_set_stream", "dida_letkf_set_grid", "dida_letkf_set_obs",
+    "dida_letkf_set_stream", "dida_letkf_set_option", "dida_letkf_set_grid",
+    "dida_letkf_set_obs",
     "dida_letkf_set_obs_device", "dida_letkf_analyze", "dida_letkf_analyze_device",
     "dida_letkf_analyze_device_f32", "dida_letkf_analyze_epilogue_device",
     "dida_letkf_analyze_epilogue_device_f32",
@@ -118,6 +133,7 @@ def lib(path: Optional[str] = None):
     L.dida_letkf_create.argtypes = [C.POINTER(DidaConfig), C.POINTER(_vp)]
     L.dida_letkf_destroy.argtypes = [_vp]
     L.dida_letkf_set_stream.argtypes = [_vp, _vp]
+    L.dida_letkf_set_option.argtypes = [_vp, C.c_int, C.c_longlong]
     L.dida_letkf_set_grid.argtypes = [_vp, _fp, _fp]
     L.dida_letkf_set_obs.argtypes = [_vp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp]
     L.dida_letkf_set_obs_device.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
@@ -285,6 +301,11 @@ class LetkfEngine:
 
     def set_stream(self, stream_ptr: int):
         _check(lib().dida_letkf_set_stream(self.h, _vp(stream_ptr)), "set_stream")
+
+    def set_option(self, option: int, value: int):
+        """dida_letkf_set_option (OPT_SCHEDULE, OPT_TIMING, OPT_TUNE, OPT_GRAPH,
+        OPT_ROW_CAP)."""
+        _check(lib().dida_letkf_set_option(self.h, int(option), int(value)), "set_option")
 
     def set_grid(self, grid_lat: np.ndarray, grid_lon: np.ndarray):
         la = np.ascontiguousarray(grid_lat, np.float32)

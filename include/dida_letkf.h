@@ -123,6 +123,21 @@ typedef struct dida_letkf_stats {
     int n_rerun;            /* analyses the eigensolver's first pass flagged (QL not
                                converged in 30 iterations, or its rotation log full)
                                and its list pass redid */
+    /* Products schedule.  Length buckets over the local obs count p of a
+     * column: [0] p = 0, [1] 1..256, [2] 257..512, ... [6] 4097..8192, [7] more. */
+    int schedule;                    /* 1: buckets + persistent work queue; 0: column order */
+    long long bucket_analyses[8];    /* analyses per bucket */
+    long long bucket_rows[8];        /* their local obs rows */
+    double bucket_wave_ms[8];        /* DIDA_OPT_TIMING: products time per bucket summed
+                                        over the queue's workers (worker-ms) */
+    double products_balance;         /* DIDA_OPT_TIMING: busy fraction of the products
+                                        workers over their launch spans (1: none idle) */
+    int variant[8];                  /* products variant per bucket (32-row groups per step) */
+    int graph_replay;                /* 1: the call ran as a hipGraph launch (ms_total is the
+                                        graph's time; the per-phase times are 0) */
+    int reran_rows;                  /* 1: the row lists outgrew their capacity and the call
+                                        ran again with a larger one */
+    long long row_cap;               /* row-list capacity per batch after the call */
 } dida_letkf_stats;
 
 typedef struct dida_letkf_handle dida_letkf_handle;
@@ -139,6 +154,39 @@ int dida_letkf_destroy(dida_letkf_handle *h);
 /* Runs all later device work of the handle on this hipStream_t (NULL: the
  * handle's own stream). */
 int dida_letkf_set_stream(dida_letkf_handle *h, void *hip_stream);
+
+/* Scheduling options of a handle.  No counterpart in the reference, whose
+ * analyses run one after another per MPI rank (letkf_compute.F90:438-552).
+ *   DIDA_OPT_SCHEDULE  1 (default): the products of each batch run over
+ *                      length buckets of p, longest first, as persistent
+ *                      launches that take analyses from a work queue (one
+ *                      launch per run of buckets sharing a variant); 0: one
+ *                      launch in column order.  Results are the same bits.
+ *   DIDA_OPT_TIMING    1: the queue's kernels time each bucket and their
+ *                      workers' busy spans (stats bucket_wave_ms,
+ *                      products_balance); 0 (default): off.
+ *   DIDA_OPT_TUNE      1: the next eager call measures every products
+ *                      variant on each non-empty bucket, keeps the fastest and
+ *                      writes them to the tuning cache ($DIDA_TUNE_FILE, else
+ *                      $XDG_CACHE_HOME or $HOME/.cache, dida_letkf/tune.tsv),
+ *                      keyed by (device, k, precision, bucket); handles read
+ *                      the cache when created.  Variants give the same bits.
+ *   DIDA_OPT_GRAPH     1: a call captures its launches in a hipGraph (one
+ *                      stream, no events) and launches it; later calls with
+ *                      the same buffers, sizes, options and obs replay it.
+ *                      Needs a non-null stream.  0 (default): eager launches.
+ *   DIDA_OPT_ROW_CAP   row-list capacity per batch (0: estimated from the obs
+ *                      density).  A call whose lists need more runs again with
+ *                      the measured need; the capacity never shrinks by itself.
+ * Returns DIDA_ERR_ARG for an unknown option or value. */
+enum {
+    DIDA_OPT_SCHEDULE = 1,
+    DIDA_OPT_TIMING = 2,
+    DIDA_OPT_TUNE = 3,
+    DIDA_OPT_GRAPH = 4,
+    DIDA_OPT_ROW_CAP = 5
+};
+int dida_letkf_set_option(dida_letkf_handle *h, int option, long long value);
 
 /* Block grid as real(4) degrees, like pro_info's grid_lat/grid_lon
  * (pro_info.F90:11-12); grid_lat has nlat entries, grid_lon nlon. */

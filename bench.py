@@ -181,23 +181,37 @@ def serving_leg(torch, dev, args, precision):
     return out
 
 
-def obs_buckets(nob):
-    """Histogram of the local obs count p per analysis, with each bucket's share
-    of the rows (the products' work).  Analyses are not bucketed on the
-    device: each is one workgroup of a batch-wide launch, dispatched as CUs
-    free up, in a banded visiting order."""
-    nob = np.asarray(nob, np.int64)
-    edges = [0, 1, 256, 512, 1024, 2048, 4096, 1 << 40]
-    names = ["0", "1-255", "256-511", "512-1023", "1024-2047", "2048-4095", ">=4096"]
-    tot = max(int(nob.sum()), 1)
+BUCKET_NAMES = ["0", "1-256", "257-512", "513-1024", "1025-2048", "2049-4096", "4097-8192",
+                ">8192"]
+
+
+def products_buckets(st, nob, products_ms):
+    """The products schedule's length buckets of p, from the device: analyses,
+    share of the rows (the products' work), the variant the tuner chose, and,
+    from a step with in-kernel timing, each bucket's share of the workers'
+    time and the workers' busy fraction over their launch spans."""
+    rows = max(sum(st["bucket_rows"]), 1)
+    wave = sum(st["bucket_wave_ms"])
     out = []
-    for lo, hi, nm in zip(edges[:-1], edges[1:], names):
-        m = (nob >= lo) & (nob < hi)
-        if m.any():
-            out.append({"p": nm, "analyses": int(m.sum()),
-                        "rows_share": round(float(nob[m].sum()) / tot, 4)})
-    return {"p_mean": round(float(nob.mean()), 1), "p_max": int(nob.max()),
-            "buckets": out}
+    for b, nm in enumerate(BUCKET_NAMES):
+        if not st["bucket_analyses"][b]:
+            continue
+        out.append({"p": nm, "analyses": st["bucket_analyses"][b],
+                    "rows_share": round(st["bucket_rows"][b] / rows, 4),
+                    "row_groups_per_step": st["variant"][b],
+                    "worker_ms": round(st["bucket_wave_ms"][b], 3),
+                    "time_share": round(st["bucket_wave_ms"][b] / wave, 4) if wave else None})
+    nob = np.asarray(nob, np.int64) if nob is not None else None
+    return {"schedule": ("length buckets, persistent work queue per run of buckets "
+                         "sharing a variant" if st["schedule"] else "column order"),
+            "p_mean": round(float(nob.mean()), 1) if nob is not None else None,
+            "p_max": int(nob.max()) if nob is not None else None,
+            "buckets": out,
+            "load_balance": round(st["products_balance"], 4),
+            "load_balance_note": ("summed busy time of the products' queue workers over "
+                                  "workers x launch span (1: no worker idle while another "
+                                  "still works)"),
+            "products_ms": round(products_ms, 3)}
 
 
 def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
@@ -232,14 +246,20 @@ def make_line(args, cfg, n_a, n_obs_rows, res, acc, cpu):
                                  "square root, " + ("FP32 perturbation state, update on f32 MFMA"
                                                     if args.mixed_state == "f32" else
                                                     "FP64 state and update")),
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "products_schedule": args.schedule,
+                   "timed_steps": ("hipGraph replay of the call" if args.graph
+                                   else "eager launches")},
         "kernel_ms": head["kernel_ms"],
+        "kernel_ms_timed_steps": head.get("kernel_ms_overlapped"),
         "kernel_ms_note": ("HIP-event time of each kernel over the whole batch, from two extra "
                            "steps with the eigensolver on one stream"
                            if "kernel_ms_overlapped" in head else
                            "HIP-event time of each kernel over the whole batch"),
         "roofline": roof,
-        "local_obs": obs_buckets(head["nobsl"]) if head.get("nobsl") is not None else None,
+        "products_buckets": (products_buckets(head["bucket_stats"], head.get("nobsl"),
+                                              head["kernel_ms"]["products"])
+                             if head.get("bucket_stats") else None),
         "cpu_baseline": cpu,
         "accuracy": acc,
     }
@@ -278,6 +298,13 @@ def parse():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy checks and the local-obs histogram")
+    ap.add_argument("--schedule", choices=("queue", "column"), default="queue",
+                    help="products over length buckets on persistent work queues, or one "
+                         "launch in column order")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="1: tune the products variant per length bucket before the timed steps")
+    ap.add_argument("--graph", action="store_true",
+                    help="timed steps replay a captured hipGraph of the call")
     ap.add_argument("--eig-one-stream", action="store_true",
                     help="time the steps with the eigensolver on one stream (transform_kernel "
                          "2; the default overlaps two half-batches on two streams)")
@@ -336,6 +363,7 @@ def main():
         eng = engine.LetkfEngine(c, transform_kernel=transform_kernel, keep_transforms=keep)
         eng.set_stream(stream.cuda_stream)
         eng.set_grid(inst.grid_lat, inst.grid_lon)
+        eng.set_option(engine.OPT_SCHEDULE, 1 if args.schedule == "queue" else 0)
         return eng
 
     def step(eng, xa, axa):
@@ -348,13 +376,21 @@ def main():
         return eng.analyze_device(d["xb"].data_ptr(), d["axb"].data_ptr(), xa.data_ptr(),
                                   axa.data_ptr())
 
-    def run_mode(prec, transform_kernel=0, steps=None, warmup=None, counts=False):
+    def run_mode(prec, transform_kernel=0, steps=None, warmup=None, counts=False, graph=False,
+                 bucket_timing=False):
         steps = args.steps if steps is None else steps
         warmup = args.warmup if warmup is None else warmup
         eng = make_engine(prec, transform_kernel, keep=counts)
         f32 = prec != PREC_FP64 and args.mixed_state == "f32"
         xa = torch.zeros_like(d["xb32"] if f32 else d["xb"])
         axa = torch.zeros_like(d["axb"])
+        if args.tune and args.schedule == "queue":
+            # the tuner measures the products variants per length bucket on
+            # the first call (untimed) and caches its choice
+            eng.set_option(engine.OPT_TUNE, 1)
+            step(eng, xa, axa)
+        if graph:
+            eng.set_option(engine.OPT_GRAPH, 1)
         for _ in range(warmup):
             step(eng, xa, axa)
         torch.cuda.synchronize()
@@ -376,11 +412,19 @@ def main():
         el = time.perf_counter() - ts
         stats = eng.stats.as_dict()
         nob = eng.local_obs_counts() if counts else None
+        bstats = None
+        if bucket_timing and args.schedule == "queue":
+            # one more (untimed) step with the products timed per bucket in the kernels
+            eng.set_option(engine.OPT_GRAPH, 0)
+            eng.set_option(engine.OPT_TIMING, 1)
+            step(eng, xa, axa)
+            bstats = eng.stats.as_dict()
         eng.close()
         for kk in k_ms:
             k_ms[kk] /= max(steps, 1)
         return dict(elapsed=el, ms_step=1e3 * el / steps, value=N_A * steps / el,
-                    kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa, nobsl=nob)
+                    kernel_ms=k_ms, stats=stats, flagged=flagged, xa=xa, axa=axa, nobsl=nob,
+                    bucket_stats=bstats)
 
     # timed steps: the eigensolver overlaps two half-batches on two streams
     # (transform_kernel 0), so its kernels are timed one at a time in a short
@@ -389,10 +433,12 @@ def main():
     res = {}
     modes = [args.mixed] + ([] if args.no_fp64 else [PREC_FP64])
     for prec in modes:
-        r = run_mode(prec, tk)
-        if tk == 0:
+        r = run_mode(prec, tk, graph=args.graph)
+        if tk == 0 or args.graph:
             r["kernel_ms_overlapped"] = r["kernel_ms"]
-            r["kernel_ms"] = run_mode(prec, 2, steps=2, warmup=1)["kernel_ms"]
+            rk = run_mode(prec, 2, steps=2, warmup=1, bucket_timing=prec == args.mixed)
+            r["kernel_ms"] = rk["kernel_ms"]
+            r["bucket_stats"] = rk["bucket_stats"]
         if prec == args.mixed and not args.no_accuracy:
             # local obs counts of the cycle (keep_transforms also writes every
             # trans, so this step is not timed)

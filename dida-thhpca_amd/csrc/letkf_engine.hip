@@ -333,6 +333,16 @@ int rerun_flagged(hipStream_t st, EigScratch& s, const EigParams& ep) {
   return DIDA_OK;
 }
 
+// The half-batch pipeline's second stream and its fork / join events (a
+// captured call records them as graph dependencies).
+int eig_second_stream(EigScratch& s) {
+  if (s.st2) return DIDA_OK;
+  HIP_TRY(hipStreamCreateWithFlags(&s.st2, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&s.e_first, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&s.e_join, hipEventDisableTiming));
+  return DIDA_OK;
+}
+
 // Transform + update of analyses [xp.a0, xp.a0 + xp.n_an): the split
 // eigensolver, then its list pass over the analyses it flagged; mode 1 runs
 // only the single kernel (k <= 64).  Mode 0 runs batches of 2 kEigPipeMinHalf
@@ -351,11 +361,7 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   RC_TRY(prepare_eig(s, xp, ep));
   HIP_TRY(launch_zero_ints(st, ep.flag_n, 1));
   if (mode == 0 && xp.n_an >= 2 * kEigPipeMinHalf) {
-    if (!s.st2) {
-      HIP_TRY(hipStreamCreateWithFlags(&s.st2, hipStreamNonBlocking));
-      HIP_TRY(hipEventCreateWithFlags(&s.e_first, hipEventDisableTiming));
-      HIP_TRY(hipEventCreateWithFlags(&s.e_join, hipEventDisableTiming));
-    }
+    RC_TRY(eig_second_stream(s));
     HIP_TRY(launch_eig_pipelined(st, s.st2, s.e_first, s.e_join, ep, kev));
   } else {
     HIP_TRY(launch_eig(st, ep, kev));
@@ -429,8 +435,6 @@ struct dida_letkf_handle {
   hipEvent_t g_ev[2] = {nullptr, nullptr};
   // options (dida_letkf_set_option)
   int opt_schedule = 1, opt_timing = 0, opt_tune = 0, opt_graph = 0;
-  // bumped by set_grid / set_obs: a captured graph holds their buffers and sizes
-  long long obs_version = 0;
   // length buckets: order list, block counts, segment starts, queue counters,
   // per-batch bucket statistics and timing ticks
   DevBuf b_order, b_blk, b_seg, b_q, b_stats, b_ticks;
@@ -646,7 +650,6 @@ int dida_letkf_set_grid(dida_letkf_handle* h, const float* lat, const float* lon
   }
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->have_grid = true;
-  ++h->obs_version;
   return DIDA_OK;
 }
 
@@ -713,7 +716,6 @@ static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
   }
   HIP_TRY(hipGetLastError());
   h->have_obs = true;
-  ++h->obs_version;
   return DIDA_OK;
 }
 
@@ -924,6 +926,8 @@ static int prepare_call(dida_letkf_handle* h) {
     h->nobsl_host.assign(h->n_an, 0);
   }
   RC_TRY(eig_prepare(h->eig, c.k, mb * h->nzc));
+  if (c.transform_kernel == 0 && mb * h->nzc >= 2 * kEigPipeMinHalf)
+    RC_TRY(eig_second_stream(h->eig));
   if (h->opt_schedule == 1 && !h->tiled) {
     RC_TRY(h->b_order.ensure(sizeof(int) * mb * h->nzc));
     RC_TRY(h->b_blk.ensure(sizeof(long long) * std::max(bucket_blk_elems((int)mb), 1LL)));
@@ -1330,7 +1334,12 @@ static std::vector<uintptr_t> graph_key(const dida_letkf_handle* h, const StateA
     k.push_back(u);
   };
   put(h->stream);
-  k.push_back((uintptr_t)h->obs_version);
+  // grid and obs buffers (set_obs refills them in place while they are large
+  // enough, so a new cycle's obs replay the same graph)
+  for (const DevBuf* b : {&h->glat, &h->glon, &h->o_yb64, &h->o_dep, &h->o_vco, &h->bin_start,
+                          &h->s_cos, &h->s_sin, &h->s_lonr, &h->s_cosl, &h->s_sinl, &h->s_row0,
+                          &h->s_nrow, &h->yb32, &h->rinv32, &h->d_hyai, &h->d_hybi})
+    put(b->p);
   k.push_back((uintptr_t)h->row_cap);
   k.push_back((uintptr_t)h->opt_schedule);
   k.push_back((uintptr_t)h->opt_timing);
@@ -1358,7 +1367,8 @@ static void drop_graph(dida_letkf_handle* h) {
   h->gkey.clear();
 }
 
-// Captures the call's launches (single-stream transform, no events) unless
+// Captures the call's launches (no timing events; the half-batch pipeline's
+// second stream joins the capture through its fork / join events) unless
 // the graph of an identical earlier call exists, and launches the graph.
 static int graph_call(dida_letkf_handle* h, const StateArgs& sa, float& ms) {
   hipStream_t st = h->stream;
@@ -1367,7 +1377,7 @@ static int graph_call(dida_letkf_handle* h, const StateArgs& sa, float& ms) {
   if (!h->gexec || key != h->gkey) {
     drop_graph(h);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    const int rc = enqueue_call(h, sa, false, 2, false);
+    const int rc = enqueue_call(h, sa, false, h->cfg.transform_kernel, false);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &g);
     if (rc < 0 || e != hipSuccess) {

@@ -13,10 +13,16 @@ def test_line_has_contract_keys():
     import bench
     from dida_letkf import synth
     cfg = synth.make_config("c2")
-    args = argparse.Namespace(mixed=2, mixed_state="f32", steps=5, warmup=1, config="c2")
+    args = argparse.Namespace(mixed=2, mixed_state="f32", steps=5, warmup=1, config="c2",
+                              schedule="queue", graph=False)
     km = dict(select=2.8, products=9.6, transform=41.9, tridiag=8.5, ql=9.0, apply=24.0, update=1.5)
     st = dict(total_rows=72_613_192, max_rows=2656, ms_update=1.5, rows_over_10000=0)
-    res = {2: dict(value=1.18e6, ms_step=54.9, kernel_ms=km, stats=st),
+    bst = dict(schedule=1, bucket_analyses=[0, 10, 20, 30, 0, 0, 0, 0],
+               bucket_rows=[0, 1000, 6000, 21000, 0, 0, 0, 0],
+               bucket_wave_ms=[0.0, 1.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0],
+               variant=[1, 1, 2, 2, 1, 1, 1, 1], products_balance=0.9)
+    res = {2: dict(value=1.18e6, ms_step=54.9, kernel_ms=km, stats=st, bucket_stats=bst,
+                   nobsl=[100, 300, 700]),
            0: dict(value=1.0e6, ms_step=64.8, kernel_ms=dict(km, products=19.9), stats=st)}
     line = bench.make_line(args, cfg, cfg.n_analyses(), 500_000, res, {}, None)
     for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
@@ -43,6 +49,12 @@ def test_line_has_contract_keys():
     assert r2["bound"] == "hbm" and r2["unit"] == "TB/s"
     assert abs(r2["achieved"] - n_a * (8 * k * m + 16 * m) / 3.0e-3 / 1e12) < 1e-3
     assert line["mixed_over_fp64"] == round(1.18e6 / 1.0e6, 3)
+    pb = line["products_buckets"]
+    assert [b["p"] for b in pb["buckets"]] == ["1-256", "257-512", "513-1024"]
+    assert [b["rows_share"] for b in pb["buckets"]] == [round(1 / 28, 4), round(6 / 28, 4),
+                                                          0.75]
+    assert [b["time_share"] for b in pb["buckets"]] == [0.1, 0.3, 0.6]
+    assert pb["load_balance"] == 0.9 and pb["p_max"] == 700
     json.dumps(line)
 
 

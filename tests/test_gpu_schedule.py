@@ -39,6 +39,11 @@ def _instances():
         "c1_k32_vertical": synth.generate("c1", seed=42, nlat=12, nlon=24, nz=6, n_obs=3000),
         "c3_k128": synth.generate("c3", seed=43, k=128, nlat=10, nlon=20, nz=3, n_obs=12000,
                                   local_dist=900.0),
+        # >= 2048 analyses: the eigensolver's half-batch pipeline on two streams
+        "c0_k64_pipe": synth.generate("c0", seed=44, k=64, nlat=48, nlon=48, n_obs=400,
+                                      local_dist=600.0),
+        "c0_k128_pipe": synth.generate("c0", seed=45, k=128, nlat=48, nlon=48, n_obs=400,
+                                       local_dist=600.0),
     }
 
 
@@ -157,7 +162,8 @@ def test_row_capacity_overflow_reruns(schedule, max_batch):
 
 @pytest.mark.parametrize("name,prec,f32", [("c2_k64", 2, True), ("c2_k64", 0, False),
                                            ("c1_k32_vertical", 2, False),
-                                           ("c3_k128", 2, False)])
+                                           ("c3_k128", 2, False), ("c0_k64_pipe", 2, True),
+                                           ("c0_k128_pipe", 0, False)])
 def test_graph_replays_equal_eager(name, prec, f32):
     import copy
     import torch

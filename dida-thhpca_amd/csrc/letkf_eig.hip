@@ -70,10 +70,8 @@ __device__ __forceinline__ AnRows an_rows(const EigParams& p, long long a) {
   AnRows r;
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  r.sj = ic * p.g.cube_n + 1;
-  r.ej = min(r.sj + p.g.cube_n - 1, p.g.nlat);
-  r.si = jc * p.g.cube_n + 1;
-  r.ei = min(r.si + p.g.cube_n - 1, p.g.nlon);
+  cube_span(p.g, ic, p.g.nlat, &r.sj, &r.ej);
+  cube_span(p.g, jc, p.g.nlon, &r.si, &r.ei);
   r.lat_ext = p.g.nlat + 2 * p.halo;
   r.sk = 1;
   r.ek = p.nz;

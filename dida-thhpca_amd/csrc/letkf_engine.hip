@@ -370,6 +370,16 @@ int run_transform(hipStream_t st, EigScratch& s, const XformParams& xp, int mode
   return rerun_flagged(st, s, ep);
 }
 
+// cubes along n grid points with starts every cube_n - overlap (cube_n
+// without overlap): letkf_compute.F90:53,61 and :197,205
+static long long cubes_along(int n, int cube_n, int overlap) {
+  const int s = overlap > 0 ? cube_n - overlap : cube_n;
+  return n >= 1 && s >= 1 ? (n - 1) / s + 1 : 0;
+}
+static long long cubes_h(const dida_letkf_config& c) {
+  return cubes_along(c.nlat, c.cube_n, c.overlap) * cubes_along(c.nlon, c.cube_n, c.overlap);
+}
+
 namespace dida {
 // columns whose per-batch scratch fits in `bytes`
 long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes) {
@@ -387,8 +397,7 @@ long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes) {
 long long max_batch_columns(const dida_letkf_config& c) {
   long long mb = batch_columns_for_bytes(c, 16ull << 30);
   if (c.max_batch_columns > 0) mb = std::min<long long>(mb, c.max_batch_columns);
-  const long long nh = (long long)((c.nlat + c.cube_n - 1) / c.cube_n) *
-                       ((c.nlon + c.cube_n - 1) / c.cube_n);
+  const long long nh = cubes_h(c);
   return std::min<long long>(mb, nh);
 }
 
@@ -479,8 +488,8 @@ void dida_letkf_config_default(dida_letkf_config* c) {
 
 long long dida_letkf_n_analyses(const dida_letkf_config* c) {
   if (!c || c->cube_n < 1 || c->nlat < 1 || c->nlon < 1) return -1;
-  long long nh = (long long)((c->nlat + c->cube_n - 1) / c->cube_n) *
-                 ((c->nlon + c->cube_n - 1) / c->cube_n);
+  if (c->overlap < 0 || c->overlap >= c->cube_n) return -1;
+  long long nh = cubes_h(*c);
   if (c->vert_dist > 0.0) {
     int cz = c->cube_nz > 0 ? c->cube_nz : c->nz;
     nh *= (c->nz + cz - 1) / cz;
@@ -497,6 +506,11 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
     return set_error(DIDA_ERR_ARG, "bad grid/variable sizes");
   if (c.n2d + c.n3d > kMaxVars) return set_error(DIDA_ERR_ARG, "more than %d variables", kMaxVars);
   if (c.halo < 0 || c.cube_n < 1) return set_error(DIDA_ERR_ARG, "bad halo/cube_n");
+  if (c.overlap < 0 || c.overlap >= c.cube_n)
+    return set_error(DIDA_ERR_ARG, "overlap=%d outside 0..cube_n-1", c.overlap);
+  if (c.overlap > 0 && c.vert_dist > 0.0)
+    return set_error(DIDA_ERR_UNSUPPORTED,
+                     "overlapped cubes run the horizontal loop only (letkf.F90:227-235)");
   if (!(c.local_dist > 0.0)) return set_error(DIDA_ERR_ARG, "local_dist must be > 0");
   if (!(c.inflation_factor > 0.0)) return set_error(DIDA_ERR_ARG, "inflation_factor must be > 0");
   if (c.precision < DIDA_PREC_FP64 || c.precision > DIDA_PREC_BF16X3)
@@ -522,8 +536,8 @@ int dida_letkf_create(const dida_letkf_config* cfg, dida_letkf_handle** out) {
   }
   if (h->cfg.cube_nz < 1) h->cfg.cube_nz = c.nz;
   h->n = c.n2d + c.n3d * c.nz;
-  h->nlat_c = (c.nlat + c.cube_n - 1) / c.cube_n;
-  h->nlon_c = (c.nlon + c.cube_n - 1) / c.cube_n;
+  h->nlat_c = (int)cubes_along(c.nlat, c.cube_n, c.overlap);
+  h->nlon_c = (int)cubes_along(c.nlon, c.cube_n, c.overlap);
   h->nzc = c.vert_dist > 0.0 ? (c.nz + h->cfg.cube_nz - 1) / h->cfg.cube_nz : 1;
   h->n_h = (long long)h->nlat_c * h->nlon_c;
   h->n_an = h->n_h * h->nzc;
@@ -920,8 +934,9 @@ static int prepare_call(dida_letkf_handle* h) {
     RC_TRY(h->row_gc.ensure(sizeof(double) * rc));
     if (h->nzc == 1) RC_TRY(h->row_wd.ensure(sizeof(double) * rc));
   }
-  if (c.keep_transforms) {
+  if (c.keep_transforms || c.overlap > 0)
     RC_TRY(h->trans.ensure(sizeof(double) * h->n_an * c.k * c.k));
+  if (c.keep_transforms) {
     RC_TRY(h->wout.ensure(sizeof(double) * h->n_an * c.k));
     h->nobsl_host.assign(h->n_an, 0);
   }
@@ -1012,6 +1027,7 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
   g.nlat = c.nlat;
   g.nlon = c.nlon;
   g.cube_n = c.cube_n;
+  g.cube_s = c.overlap > 0 ? c.cube_n - c.overlap : c.cube_n;
   g.nlat_c = h->nlat_c;
   g.nlon_c = h->nlon_c;
   const long long mb = h->max_batch;
@@ -1196,10 +1212,13 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
         cur += c.nz;
       }
     }
-    xp.xb = sa.xb;
+    // overlapped cubes: the eigensolver writes each cube's trans only, and
+    // k_overlap_update forms every column from its cubes after the last batch
+    const bool ovl = c.overlap > 0;
+    xp.xb = ovl ? nullptr : sa.xb;
     xp.axb = axb;
-    xp.xa = sa.xa;
-    xp.axa = sa.axa;
+    xp.xa = ovl ? nullptr : sa.xa;
+    xp.axa = ovl ? nullptr : sa.axa;
     xp.state_f32 = sa.xb32 != nullptr;
     xp.xb32 = sa.xb32;
     xp.xa32 = sa.xa32;
@@ -1210,13 +1229,27 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
     xp.xa_out = sa.xa_out;
     xp.spread_b = sa.spread_b;
     xp.spread_a = sa.spread_a;
-    xp.trans_out = c.keep_transforms ? h->trans.as<double>() : nullptr;
+    xp.trans_out = (c.keep_transforms || ovl) ? h->trans.as<double>() : nullptr;
     xp.w_out = c.keep_transforms ? h->wout.as<double>() : nullptr;
     xp.status = h->status.as<int>();
     xp.list = nullptr;
     bool split = false;
     RC_TRY(run_transform(st, h->eig, xp, tmode, timed ? ev + 4 : nullptr, &split));
     h->split_timed[bi] = timed && split;
+    if (ovl && h1 == h->n_h) {
+      OverlapParams op{};
+      op.g = g;
+      op.k = c.k;
+      op.n = h->n;
+      op.halo = c.halo;
+      op.xb = sa.xb;
+      op.axb = axb;
+      op.xa = sa.xa;
+      op.axa = sa.axa;
+      op.trans = h->trans.as<double>();
+      op.status = h->status.as<int>();
+      HIP_TRY(launch_overlap_update(st, op));
+    }
     if (timed) HIP_TRY(hipEventRecord(ev[3], st));
   }
   return DIDA_OK;
@@ -1402,6 +1435,9 @@ static int graph_call(dida_letkf_handle* h, const StateArgs& sa, float& ms) {
 static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_stats* stats) {
   if (!h->have_grid) return set_error(DIDA_ERR_STATE, "set_grid was not called");
   if (!h->have_obs) return set_error(DIDA_ERR_STATE, "set_obs was not called");
+  if (h->cfg.overlap > 0 && (sa.xb32 || sa.epi))
+    return set_error(DIDA_ERR_UNSUPPORTED,
+                     "overlapped cubes update an FP64 state without the fused epilogue");
   RC_TRY(prepare_call(h));
   h->split_timed.assign(n_batches_of(h), false);
   bool graph = h->opt_graph != 0;
@@ -1649,7 +1685,7 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
     HIP_TRY(hipGetLastError());
   }
   Geom g{};
-  g.nlat = 1; g.nlon = 1; g.cube_n = 1; g.nlat_c = 1; g.nlon_c = 1;
+  g.nlat = 1; g.nlon = 1; g.cube_n = 1; g.cube_s = 1; g.nlat_c = 1; g.nlon_c = 1;
   ProdParams pp{};
   pp.g = g;
   pp.k = k;
@@ -1847,9 +1883,9 @@ size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows)
   const int cz = c->cube_nz > 0 ? c->cube_nz : c->nz;
   const size_t nzc = c->vert_dist > 0.0 ? (c->nz + cz - 1) / cz : 1;
   const size_t mb = (size_t)max_batch_columns(*c);
-  const size_t nh = (size_t)((c->nlat + c->cube_n - 1) / c->cube_n) *
-                    ((c->nlon + c->cube_n - 1) / c->cube_n);
+  const size_t nh = (size_t)cubes_h(*c);
   const size_t n_an = nh * nzc;
+  const size_t ovl_trans = c->overlap > 0 ? n_an * k * k * 8 : 0;
   const size_t n_state = (size_t)(c->nlat + 2 * c->halo) * (c->nlon + 2 * c->halo) *
                          (c->n2d + c->n3d * c->nz);
   size_t b = 0;
@@ -1867,6 +1903,7 @@ size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows)
   b += rerun_cap_of(c->k, (long long)(mb * nzc)) * rerun_bytes_per_slot(c->k);
   b += 4 * n_an;
   if (c->keep_transforms) b += n_an * (8 * k * k + 8 * k);
+  else b += ovl_trans;                                              // overlapped cubes' trans
   b += n_state * (k + 1) * 8;                                       // serving ping-pong state
   b += nr * (k + 1) * 8;                                            // serving operator Yb, d
   return b + (16 << 20);

@@ -28,8 +28,15 @@ struct Geom {
   const float* glon;   // nlon
   int nlat, nlon;
   int cube_n;
+  int cube_s;          // cube start stride: cube_n - overlap
   int nlat_c, nlon_c;  // cubes per direction
 };
+
+// 1-based clipped bounds [s, e] of cube c along a direction of n points
+__host__ __device__ inline void cube_span(const Geom& g, int c, int n, int* s, int* e) {
+  *s = c * g.cube_s + 1;
+  *e = *s + g.cube_n - 1 < n ? *s + g.cube_n - 1 : n;
+}
 
 struct SelectParams {
   Geom g;
@@ -260,6 +267,20 @@ struct ObsOpParams {
   double* yb;                // n_rows x k
   double* dep;
 };
+
+// Overlapped cubes (letkf_compute.F90:211-255): each column's analysis is the
+// mean of the analyses of the cubes that cover it.
+struct OverlapParams {
+  Geom g;
+  int k, n, halo;
+  const double* xb;
+  const double* axb;
+  double* xa;
+  double* axa;
+  const double* trans;   // per cube, Fortran (k, k): trans(m, e) at [c k^2 + e k + m]
+  const int* status;     // per cube: nonzero keeps the background (trans = I)
+};
+hipError_t launch_overlap_update(hipStream_t st, const OverlapParams& p);
 
 struct EpiParams {
   int k, n, nlat, nlon, halo;

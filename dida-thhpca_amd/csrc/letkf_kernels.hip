@@ -79,10 +79,8 @@ __global__ void k_site_gather(int n_sites, const int* __restrict__ order,
 __device__ __forceinline__ void column_center(const Geom& g, int ah, int* sj, int* ej, int* si,
                                               int* ei) {
   const int ic = ah % g.nlat_c, jc = ah / g.nlat_c;
-  *sj = ic * g.cube_n + 1;
-  *ej = min(*sj + g.cube_n - 1, g.nlat);
-  *si = jc * g.cube_n + 1;
-  *ei = min(*si + g.cube_n - 1, g.nlon);
+  cube_span(g, ic, g.nlat, sj, ej);
+  cube_span(g, jc, g.nlon, si, ei);
 }
 
 // Bins a disc of half-angle th (degrees) around (clat, clon) touches: latitude
@@ -1261,8 +1259,9 @@ __global__ __launch_bounds__(64) void k_transform(XformParams p) {
   // ---- 6. update ----
   const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
   const int ic = ah % p.g.nlat_c, jc = ah / p.g.nlat_c;
-  const int sj = ic * p.g.cube_n + 1, ej = min(sj + p.g.cube_n - 1, p.g.nlat);
-  const int si = jc * p.g.cube_n + 1, ei = min(si + p.g.cube_n - 1, p.g.nlon);
+  int sj, ej, si, ei;
+  cube_span(p.g, ic, p.g.nlat, &sj, &ej);
+  cube_span(p.g, jc, p.g.nlon, &si, &ei);
   const long long lat_ext = p.g.nlat + 2 * p.halo;
   int sk = 1, ek = p.nz;
   if (p.vertical) {
@@ -1889,6 +1888,100 @@ hipError_t launch_transform(hipStream_t st, const XformParams& p) {
   if (p.k <= 32) hipLaunchKernelGGL(k_transform<32>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
   else if (p.k <= 64) hipLaunchKernelGGL(k_transform<64>, dim3((unsigned)p.n_an), dim3(64), 0, st, p);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// One workgroup per column (halo columns included: they are zeroed, as
+// letkf_compute.F90:191-193 zeroes xa and axa before the loop); wavefront w
+// takes the state elements v = w, w + 4, ...  For each covering cube in loop
+// order (lon cubes outer, lat cubes inner), lane e forms
+// dist(e) = axb(v) + sum_m xb(m, v) trans(m, e) (xb(m, v) broadcast from lane
+// m), the wavefront sums dist for the mean, and the lane adds dist(e) - mean
+// to xa(e, v) and the mean to axa(v) (xb_add2_xa, letkf_compute.F90:332-382);
+// the column is then divided by its cube count (:248-255).
+template <int KM>
+__global__ __launch_bounds__(256) void k_overlap_update(OverlapParams p) {
+  constexpr int R = KM / 64;            // members per lane
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lat_ext = p.g.nlat + 2 * p.halo;
+  const int la_e = blockIdx.x % lat_ext, lo_e = blockIdx.x / lat_ext;
+  const int k = p.k, n = p.n;
+  const long long off = ((long long)lo_e * lat_ext + la_e) * n;
+  const int la = la_e - p.halo, lo = lo_e - p.halo;
+  if (la < 0 || la >= p.g.nlat || lo < 0 || lo >= p.g.nlon) {
+    for (int i = threadIdx.x; i < n * k; i += blockDim.x) p.xa[off * k + i] = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p.axa[off + i] = 0.0;
+    return;
+  }
+  const int cn = p.g.cube_n, cs = p.g.cube_s;
+  // cubes c with c cs <= x <= c cs + cn - 1 (0-based x)
+  const int ic0 = la - cn + 1 <= 0 ? 0 : (la - cn + cs) / cs;
+  const int ic1 = min(la / cs, p.g.nlat_c - 1);
+  const int jc0 = lo - cn + 1 <= 0 ? 0 : (lo - cn + cs) / cs;
+  const int jc1 = min(lo / cs, p.g.nlon_c - 1);
+  const double count = (double)((ic1 - ic0 + 1) * (jc1 - jc0 + 1));
+  const long long kk = (long long)k * k;
+  for (int v = wave; v < n; v += 4) {
+    const double* xbc = p.xb + (off + v) * k;
+    const double ab = p.axb[off + v];
+    double xv[R], acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = lane + 64 * r;
+      xv[r] = e < k ? xbc[e] : 0.0;
+      acc[r] = 0.0;
+    }
+    double accm = 0.0;
+    for (int jc = jc0; jc <= jc1; ++jc) {
+      for (int ic = ic0; ic <= ic1; ++ic) {
+        const long long c = (long long)jc * p.g.nlat_c + ic;
+        const bool keep = p.status[c] != 0;
+        double dist[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int e = lane + 64 * r;
+          const double* tr = p.trans + c * kk + (long long)(e < k ? e : 0) * k;
+          double s = 0.0;
+          if (keep) {
+            s = xv[r];
+          } else {
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr)
+              for (int m = 0; m < min(64, k - 64 * rr); ++m)
+                s = fma(__shfl(xv[rr], m, 64), tr[64 * rr + m], s);
+          }
+          dist[r] = e < k ? ab + s : 0.0;
+        }
+        double sum = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) sum += dist[r];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        const double mean = sum / k;
+        accm += mean;
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += dist[r] - mean;
+      }
+    }
+    double* xac = p.xa + (off + v) * k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = lane + 64 * r;
+      if (e < k) xac[e] = acc[r] / count;
+    }
+    if (lane == 0) p.axa[off + v] = accm / count;
+  }
+}
+
+hipError_t launch_overlap_update(hipStream_t st, const OverlapParams& p) {
+  const long long cols = (long long)(p.g.nlat + 2 * p.halo) * (p.g.nlon + 2 * p.halo);
+  if (cols <= 0) return hipSuccess;
+  if (p.k <= 64)
+    hipLaunchKernelGGL(k_overlap_update<64>, dim3((unsigned)cols), dim3(256), 0, st, p);
+  else if (p.k <= 128)
+    hipLaunchKernelGGL(k_overlap_update<128>, dim3((unsigned)cols), dim3(256), 0, st, p);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

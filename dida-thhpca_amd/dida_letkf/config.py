@@ -22,8 +22,9 @@ precision               computing_precision         53
 =====================  ==========================  =========================
 
 ``obs_polar`` is ``da_polar_letkf_lat`` when ``polar_discard`` is set and 91
-otherwise (letkf_init.F90:50-54).  The engine runs the no-overlap loops
-(``letkf_compute_nooverlap`` and its vertical variant); ``overlap`` must be 0.
+otherwise (letkf_init.F90:50-54).  ``overlap`` = 0 runs the no-overlap loops
+(``letkf_compute_nooverlap`` and its vertical variant); 1 .. cube_n - 1 runs
+the overlapped-cube ``letkf_compute`` (horizontal loop, FP64 state).
 """
 from __future__ import annotations
 

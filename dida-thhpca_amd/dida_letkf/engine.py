@@ -39,7 +39,7 @@ class DidaConfig(C.Structure):
                 ("vcoord_mode", C.c_int), ("ps_index", C.c_int), ("hyai", _dp), ("hybi", _dp),
                 ("p0", C.c_double), ("ptop", C.c_double), ("keep_transforms", C.c_int),
                 ("max_batch_columns", C.c_int), ("transform_kernel", C.c_int),
-                ("products_kernel", C.c_int)]
+                ("products_kernel", C.c_int), ("overlap", C.c_int)]
 
 
 class ServeConfig(C.Structure):
@@ -198,6 +198,7 @@ def to_c_config(cfg: LetkfConfig, keep_transforms: bool = False,
     c.max_batch_columns = int(max_batch_columns)
     c.transform_kernel = int(transform_kernel)
     c.products_kernel = int(products_kernel)
+    c.overlap = int(getattr(cfg, "overlap", 0))
     return c
 
 

@@ -102,6 +102,12 @@ typedef struct dida_letkf_config {
                                analysis; 2: tiles of 8 analyses sharing their
                                LDS-staged union of obs rows (horizontal loop,
                                k <= 64; other loops use 1) */
+    int overlap;            /* da_block_overlap: 0 runs the no-overlap loops;
+                               1 .. cube_n-1 runs letkf_compute
+                               (letkf_compute.F90:156-256): cubes of cube_n
+                               columns every cube_n - overlap columns, each
+                               column the mean of its cubes' analyses
+                               (horizontal loop, FP64 state) */
 } dida_letkf_config;
 
 typedef struct dida_letkf_stats {
@@ -142,8 +148,8 @@ typedef struct dida_letkf_stats {
 
 typedef struct dida_letkf_handle dida_letkf_handle;
 
-/* Defaults of namelist_da where they exist (cube_n = 1 instead of 4, overlap
- * is not supported). */
+/* Defaults of namelist_da where they exist (cube_n = 1 instead of 4,
+ * overlap = 0 instead of 2). */
 void dida_letkf_config_default(dida_letkf_config *cfg);
 
 /* Replaces letkf_init (letkf_init.F90:2): validates the configuration and

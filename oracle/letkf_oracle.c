@@ -19,6 +19,8 @@
  *   - letkf_core_mpal eigenvalue guard, x86/DA/LETKF/src/letkf_core_mpal.F90:73-76,113-118
  *   - xb_2_xa         x86/DA/LETKF/src/letkf_compute.F90:259-329
  *   - letkf_compute_nooverlap          letkf_compute.F90:12-154
+ *   - letkf_compute (overlapped cubes) letkf_compute.F90:156-256, with
+ *     xb_add2_xa :332-382 and the cube_count division :248-255
  *   - letkf_compute_nooverlap_vertical letkf_compute.F90:386-554
  *   - vertical_localization            letkf_yb.F90:240-260
  *   - run_da_parallel epilogue         x86/DA/EnDNN/src/module_da_parallel.F90:186-261
@@ -590,15 +592,26 @@ typedef struct {
     int *status_out;
     int a_begin, a_end;          /* analysis index range [begin, end) */
     const int *a_list;           /* optional explicit analysis list */
+    double *cube_trans;          /* overlapped cubes: every cube's trans */
+    int *cube_status;
     int rc;
 } orc_job;
 
-static int orc_n_analyses_h(const orc_problem *pb)
+/* Cube start stride: cube_n - overlap (letkf_compute.F90:197,205), or
+   cube_n without overlap (:53,61). */
+static int orc_cube_stride(const orc_problem *pb)
 {
     int cn = pb->cube_n > 0 ? pb->cube_n : 1;
-    int nlat_c = (pb->nlat + cn - 1) / cn;
-    int nlon_c = (pb->nlon + cn - 1) / cn;
-    return nlat_c * nlon_c;
+    return pb->overlap > 0 ? cn - pb->overlap : cn;
+}
+
+/* cubes along n grid points: starts 1, 1 + s, ... <= n */
+static int orc_cubes_along(int n, int s) { return (n - 1) / s + 1; }
+
+static int orc_n_analyses_h(const orc_problem *pb)
+{
+    int s = orc_cube_stride(pb);
+    return orc_cubes_along(pb->nlat, s) * orc_cubes_along(pb->nlon, s);
 }
 
 int orc_n_analyses(const orc_problem *pb)
@@ -629,11 +642,13 @@ static int orc_run_horizontal(orc_job *jb, int ah, double *yb, double *dis, floa
 {
     const orc_problem *pb = jb->pb;
     const int k = pb->k, cn = pb->cube_n > 0 ? pb->cube_n : 1;
-    const int nlat_c = (pb->nlat + cn - 1) / cn;
+    const int cs = orc_cube_stride(pb);
+    const int nlat_c = orc_cubes_along(pb->nlat, cs);
     int ic = ah % nlat_c, jc = ah / nlat_c;
-    /* letkf_compute.F90:53-65 (1-based, clipped), centre (sj+ej)/2, (si+ei)/2 */
-    int sj = ic * cn + 1, ej = sj + cn - 1; if (ej > pb->nlat) ej = pb->nlat;
-    int si = jc * cn + 1, ei = si + cn - 1; if (ei > pb->nlon) ei = pb->nlon;
+    /* letkf_compute.F90:53-65 and :195-209 (1-based, clipped), centre
+       (sj+ej)/2, (si+ei)/2 */
+    int sj = ic * cs + 1, ej = sj + cn - 1; if (ej > pb->nlat) ej = pb->nlat;
+    int si = jc * cs + 1, ei = si + cn - 1; if (ei > pb->nlon) ei = pb->nlon;
     double clat = (double)pb->grid_lat[(sj + ej) / 2 - 1];
     double clon = (double)pb->grid_lon[(si + ei) / 2 - 1];
     int nobsl = orc_gen_yb_dis(pb->obs, k, clat, clon, pb->local_dist, pb->gc_filter,
@@ -646,6 +661,13 @@ static int orc_run_horizontal(orc_job *jb, int ah, double *yb, double *dis, floa
         if (jb->w_out) memcpy(jb->w_out + (size_t)ah * k, w, sizeof(double) * k);
         if (jb->nobsl_out) jb->nobsl_out[ah] = nobsl;
         if (jb->status_out) jb->status_out[ah] = st;
+        if (pb->overlap > 0) {
+            /* letkf_compute: the cube's trans is added to its columns by
+               xb_add2_xa after every cube is solved (orc_overlap_update) */
+            memcpy(jb->cube_trans + (size_t)ah * k * k, trans, sizeof(double) * k * k);
+            jb->cube_status[ah] = st;
+            return 0;
+        }
         /* xb_2_xa over the cube: columns are not contiguous across lon, so
            update one (lat-run, lon) strip at a time */
         for (int lo = si; lo <= ei; ++lo) {
@@ -754,6 +776,98 @@ static void *orc_worker(void *arg)
 }
 
 /*
+ * letkf_compute's accumulation (letkf_compute.F90:211-255): xa = 0 and axa = 0
+ * everywhere (halo included, :191-193), then for each cube in loop order
+ * (lon cubes outer, lat cubes inner) xb_add2_xa (:332-382) adds to each of
+ * its columns, per state element v,
+ *     dist(e) = axb(v) + dot_product(xb(:, v), trans(:, e)),
+ *     mean = sum_e dist(e) / ens,  axa(v) += mean,  xa(e, v) += dist(e) - mean,
+ * and cube_count of the column grows by one; finally interior columns are
+ * divided by cube_count (:248-255).  A column receives its cubes' terms in loop
+ * order whether the cubes are visited once (the reference) or gathered per
+ * column (here, lon strips over threads), so the sums are the same.
+ */
+typedef struct {
+    const orc_problem *pb;
+    const double *xb, *axb, *trans;
+    double *xa, *axa;
+    int lon_begin, lon_end;
+} orc_ovl_job;
+
+static void *orc_overlap_worker(void *arg)
+{
+    orc_ovl_job *jb = (orc_ovl_job *)arg;
+    const orc_problem *pb = jb->pb;
+    const int k = pb->k, n = pb->n, cn = pb->cube_n, cs = orc_cube_stride(pb);
+    const int nlat_c = orc_cubes_along(pb->nlat, cs);
+    const int nlon_c = orc_cubes_along(pb->nlon, cs);
+    double *dist = (double *)malloc(sizeof(double) * k);
+    for (int lo = jb->lon_begin; lo < jb->lon_end; ++lo) {
+        /* lon cubes jc with jc*cs <= lo <= jc*cs + cn - 1 (0-based) */
+        int jc0 = lo - cn + 1 <= 0 ? 0 : (lo - cn + 1 + cs - 1) / cs, jc1 = lo / cs;
+        if (jc1 > nlon_c - 1) jc1 = nlon_c - 1;
+        for (int la = 0; la < pb->nlat; ++la) {
+            int ic0 = la - cn + 1 <= 0 ? 0 : (la - cn + 1 + cs - 1) / cs, ic1 = la / cs;
+            if (ic1 > nlat_c - 1) ic1 = nlat_c - 1;
+            size_t off = col_off(pb, la, lo);
+            double count = 0.0;
+            for (int jc = jc0; jc <= jc1; ++jc)
+                for (int ic = ic0; ic <= ic1; ++ic) {
+                    const double *tr = jb->trans + (size_t)(jc * nlat_c + ic) * k * k;
+                    for (int v = 0; v < n; ++v) {
+                        const double *xbc = jb->xb + (off + v) * k;
+                        double *xac = jb->xa + (off + v) * k;
+                        double ab = jb->axb[off + v];
+                        double mean = 0.0;
+                        for (int e = 0; e < k; ++e) {
+                            double s = 0.0;
+                            for (int f = 0; f < k; ++f) s += xbc[f] * tr[IDX(f, e, k)];
+                            dist[e] = ab + s;
+                            mean += dist[e];
+                        }
+                        mean = mean / k;
+                        jb->axa[off + v] += mean;
+                        for (int e = 0; e < k; ++e) xac[e] += dist[e] - mean;
+                    }
+                    count += 1.0;
+                }
+            for (int v = 0; v < n; ++v) {
+                double *xac = jb->xa + (off + v) * k;
+                for (int e = 0; e < k; ++e) xac[e] = xac[e] / count;
+                jb->axa[off + v] = jb->axa[off + v] / count;
+            }
+        }
+    }
+    free(dist);
+    return NULL;
+}
+
+static void orc_overlap_update(const orc_problem *pb, const double *xb, const double *axb,
+                               const double *trans, double *xa, double *axa, int nthreads)
+{
+    const size_t cols = (size_t)(pb->nlat + 2 * pb->halo) * (pb->nlon + 2 * pb->halo);
+    memset(xa, 0, sizeof(double) * cols * pb->n * pb->k);
+    memset(axa, 0, sizeof(double) * cols * pb->n);
+    if (nthreads > pb->nlon) nthreads = pb->nlon;
+    if (nthreads < 1) nthreads = 1;
+    orc_ovl_job *jobs = (orc_ovl_job *)calloc(nthreads, sizeof(orc_ovl_job));
+    pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) {
+        orc_ovl_job *jb = &jobs[t];
+        jb->pb = pb; jb->xb = xb; jb->axb = axb; jb->trans = trans; jb->xa = xa; jb->axa = axa;
+        jb->lon_begin = (int)((long long)pb->nlon * t / nthreads);
+        jb->lon_end = (int)((long long)pb->nlon * (t + 1) / nthreads);
+    }
+    if (nthreads == 1) {
+        orc_overlap_worker(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, orc_overlap_worker, &jobs[t]);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    }
+    free(jobs); free(th);
+}
+
+/*
  * Runs letkf_compute_nooverlap (vert_dist <= 0) or
  * letkf_compute_nooverlap_vertical (vert_dist > 0) over horizontal analyses
  * [h_begin, h_end) (or the explicit list h_list of n_list entries when
@@ -764,7 +878,12 @@ int orc_letkf_compute(const orc_problem *pb, const double *xb, const double *axb
                       double *axa, double *trans_out, double *w_out, int *nobsl_out,
                       int *status_out, const int *h_list, int n_list, int nthreads)
 {
+    const int ovl = pb->overlap > 0;
+    if (ovl && (h_list || pb->vert_dist > 0.0 || pb->overlap >= pb->cube_n)) return -3;
     int nh = h_list ? n_list : orc_n_analyses_h(pb);
+    double *cube_trans = ovl ? (double *)malloc(sizeof(double) * (size_t)nh * pb->k * pb->k) : NULL;
+    int *cube_status = ovl ? (int *)calloc(nh > 0 ? nh : 1, sizeof(int)) : NULL;
+    const int nthreads_all = nthreads < 1 ? 1 : nthreads;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > nh) nthreads = nh > 0 ? nh : 1;
     orc_job *jobs = (orc_job *)calloc(nthreads, sizeof(orc_job));
@@ -775,6 +894,8 @@ int orc_letkf_compute(const orc_problem *pb, const double *xb, const double *axb
         jb->trans_out = trans_out; jb->w_out = w_out; jb->nobsl_out = nobsl_out;
         jb->status_out = status_out;
         jb->a_list = h_list;
+        jb->cube_trans = cube_trans;
+        jb->cube_status = cube_status;
         jb->a_begin = (int)((long long)nh * t / nthreads);
         jb->a_end = (int)((long long)nh * (t + 1) / nthreads);
     }
@@ -787,6 +908,8 @@ int orc_letkf_compute(const orc_problem *pb, const double *xb, const double *axb
     int rc = 0;
     for (int t = 0; t < nthreads; ++t) if (jobs[t].rc) rc = jobs[t].rc;
     free(jobs); free(th);
+    if (ovl && rc == 0) orc_overlap_update(pb, xb, axb, cube_trans, xa, axa, nthreads_all);
+    free(cube_trans); free(cube_status);
     return rc;
 }
 

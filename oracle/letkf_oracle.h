@@ -53,6 +53,8 @@ typedef struct {
     const double *hyai, *hybi; /* nz+1 hybrid coefficients (LNP mode) */
     double p0, ptop;
     const orc_obs *obs;
+    int overlap;           /* da_block_overlap: cubes every cube_n - overlap columns
+                              (letkf_compute, horizontal loop only); 0: no overlap */
 } orc_problem;
 
 double orc_schr2(double ri);

@@ -39,7 +39,7 @@ class OrcProblem(C.Structure):
                 ("core_flags", C.c_int), ("vcoord_mode", C.c_int), ("ps_index", C.c_int),
                 ("n_var", C.c_int), ("var_start", _ip), ("var_lev", _ip),
                 ("hyai", _dp), ("hybi", _dp), ("p0", C.c_double), ("ptop", C.c_double),
-                ("obs", C.POINTER(OrcObs))]
+                ("obs", C.POINTER(OrcObs)), ("overlap", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -202,12 +202,14 @@ def _problem(inst, core_flags: int = 0):
         pb.hybi = _p(keep(np.asarray(cfg.hybi, np.float64)), _dp)
     pb.p0, pb.ptop = cfg.p0, cfg.ptop
     pb.obs = C.pointer(obs)
+    pb.overlap = int(getattr(cfg, "overlap", 0))
     return pb, keep
 
 
 def letkf_compute(inst, nthreads: Optional[int] = None, columns: Optional[np.ndarray] = None,
                   want_trans: bool = False, jacobi: bool = False, guard: bool = False):
-    """letkf_compute_nooverlap / _vertical on an Instance.
+    """letkf_compute_nooverlap / _vertical on an Instance, or letkf_compute
+    (overlapped cubes, every column; no column subset) when cfg.overlap > 0.
 
     Returns dict with xa, axa (same shapes as inst.xb / inst.axb; only the
     analysed columns are written, the rest are zero), and optionally
@@ -236,6 +238,9 @@ def letkf_compute(inst, nthreads: Optional[int] = None, columns: Optional[np.nda
     rc = L.orc_letkf_compute(C.byref(pb), _p(xb, _dp), _p(axb, _dp), _p(xa, _dp), _p(axa, _dp),
                              _p(trans, _dp), _p(w, _dp), _p(nobsl, _ip), _p(status, _ip),
                              _p(cols, _ip), 0 if cols is None else int(cols.size), int(nthreads))
+    if rc == -3:
+        raise ValueError("overlapped cubes: horizontal loop only, every column, "
+                         "overlap < cube_n")
     if rc:
         raise RuntimeError(f"oracle letkf_compute failed rc={rc}")
     out = dict(xa=xa, axa=axa, nobsl=nobsl, status=status)

@@ -5,7 +5,9 @@ are checked here against
 
 1. a second restatement in numpy, written from the reference text rather than
    from the oracle's C: ``letkf_compute_nooverlap`` (letkf_compute.F90:12-154),
-   ``letkf_compute_nooverlap_vertical`` (letkf_compute.F90:386-554),
+   ``letkf_compute_nooverlap_vertical`` (letkf_compute.F90:386-554), the
+   overlapped-cube ``letkf_compute`` (letkf_compute.F90:156-256) with
+   ``xb_add2_xa`` (:332-382) and the ``cube_count`` division (:248-255),
    ``gen_yb_dis[_vertical]`` (letkf_yb.F90:3-238), ``vertical_localization``
    (letkf_yb.F90:240-260), ``schr2`` (letkf_yb.F90:262-286),
    ``latlon_dist_r8`` (pro_info.F90:148-161), ``letkf_core``
@@ -16,8 +18,9 @@ are checked here against
    a mutant of the restatement (the cube centre one row off, ps updated with
    the first vertical chunk instead of the last, polar obs kept, obs counted
    twice, rdiaginv kept in double, the ln p of a level instead of the layer
-   midpoint) that the oracle must *disagree* with, so a slip of that kind in
-   the oracle could not pass;
+   midpoint, overlapped cubes not divided by their count, stepped by cube_n,
+   or added without subtracting their mean) that the oracle must *disagree*
+   with, so a slip of that kind in the oracle could not pass;
 2. closed forms of the whole analysis: R^-1 = 0 leaves the state unchanged;
    the mean increment is Xb overline_w; one observation at a grid point with
    H = identity and rho = 1 gives the scalar ETKF update
@@ -125,21 +128,26 @@ def ps2alnp(ps, cfg):
 
 
 def loop(inst, mut=()):
-    """letkf_compute_nooverlap[_vertical] over the instance.  Returns xa, axa
-    (the analysed interior columns; halo entries 0), trans, overline_w and
-    nobsl per analysis in the engine's order (lon cube major, then lat cube,
-    then vertical chunk)."""
+    """letkf_compute_nooverlap[_vertical], or letkf_compute when
+    cfg.overlap > 0, over the instance.  Returns xa, axa (the analysed
+    interior columns; halo entries 0), trans, overline_w and nobsl per
+    analysis in the engine's order (lon cube major, then lat cube, then
+    vertical chunk)."""
     cfg = inst.cfg
     k, h, cn = cfg.k, cfg.halo, cfg.cube_n
+    ovl = getattr(cfg, "overlap", 0)
+    # letkf_compute.F90:53,61 step cube_n; :197,205 step cube_n - overlap
+    step = cn - ovl if ovl > 0 and "stride_cn" not in mut else cn
     rdi = inst.obs.rinv if "rinv_f64" in mut else inst.obs.rinv.astype(np.float32)
     xa = np.zeros_like(inst.xb)
     axa = np.zeros_like(inst.axb)
+    count = np.zeros(inst.axb.shape[:2])
     out_t, out_w, out_n = [], [], []
     vs = cfg.var_start() - 1
     vl = cfg.var_lev()
-    for i in range(1, cfg.nlon + 1, cn):                # letkf_compute.F90:53
+    for i in range(1, cfg.nlon + 1, step):              # letkf_compute.F90:53 / :197
         si, ei = i, min(i + cn - 1, cfg.nlon)
-        for j in range(1, cfg.nlat + 1, cn):
+        for j in range(1, cfg.nlat + 1, step):
             sj, ej = j, min(j + cn - 1, cfg.nlat)
             cj = (sj + ej) // 2 + (1 if "centre_off" in mut else 0)
             cj = min(cj, cfg.nlat)
@@ -153,6 +161,13 @@ def loop(inst, mut=()):
                 trans, w = letkf_core(Y, rdi[rows], inst.obs.dep[rows], gc, cfg.inflation_factor)
                 out_t.append(trans), out_w.append(w), out_n.append(rows.size)
                 xe = update(inst.xb[lonr, latr], inst.axb[lonr, latr], trans)
+                if ovl > 0:
+                    # xb_add2_xa (:332-382): add dist - mean and mean, count the cube
+                    m = xe.sum(axis=-1) / k
+                    xa[lonr, latr] += xe - (0.0 if "no_recentre" in mut else m[..., None])
+                    axa[lonr, latr] += m
+                    count[lonr, latr] += 1.0
+                    continue
                 m = xe.sum(axis=-1) / k
                 xa[lonr, latr] = xe - m[..., None]
                 axa[lonr, latr] = m
@@ -192,6 +207,11 @@ def loop(inst, mut=()):
             m = xe_col.sum(axis=-1) / k                       # :539-552
             xa[lonr, latr] = xe_col - m[..., None]
             axa[lonr, latr] = m
+    if ovl > 0 and "no_count" not in mut:
+        # :248-255, interior columns only (the halo stays 0)
+        inner = (slice(h, h + cfg.nlon), slice(h, h + cfg.nlat))
+        xa[inner] /= count[inner][..., None, None]
+        axa[inner] /= count[inner][..., None]
     return dict(xa=xa, axa=axa, trans=np.array(out_t), overline_w=np.array(out_w),
                 nobsl=np.array(out_n))
 
@@ -259,6 +279,11 @@ CASES = {
     "cube2_clipped_halo_polar_inflation": lambda: _with(
         _horizontal(4, nlat=7, nlon=13, halo=1), cube_n=2, polar_discard=True, polar_lat=50.0,
         inflation_factor=1.3),
+    "overlap_c4o2_halo": lambda: _with(_horizontal(9, nlat=9, nlon=14, halo=1), cube_n=4,
+                                       overlap=2),
+    "overlap_c3o1_clipped": lambda: _with(_horizontal(10, nlat=8, nlon=11), cube_n=3, overlap=1),
+    "overlap_c5o3_inflation": lambda: _with(_horizontal(11, nlat=7, nlon=12, n_obs=300), cube_n=5,
+                                            overlap=3, inflation_factor=1.2),
     "vertical_levels": lambda: _vertical(5),
     "vertical_lnp": lambda: _vertical(6, lnp=True),
 }
@@ -303,6 +328,10 @@ MUTANTS = [
     ("vertical_levels", "ps_first_chunk"),
     ("vertical_lnp", "ps_first_chunk"),
     ("vertical_lnp", "lnp_level"),
+    ("overlap_c4o2_halo", "no_count"),
+    ("overlap_c4o2_halo", "stride_cn"),
+    ("overlap_c3o1_clipped", "no_recentre"),
+    ("overlap_c5o3_inflation", "centre_off"),
 ]
 
 
@@ -335,6 +364,20 @@ def test_no_obs_weight_leaves_state(orc):
     assert np.abs(ref["xa"] - inst.xb).max() < 1e-13
     assert np.abs(ref["axa"] - inst.axb).max() < 1e-13
     assert np.abs(ref["trans"] - np.eye(inst.cfg.k)).max() < 1e-13
+
+
+def test_overlap_no_obs_weight_leaves_state(orc):
+    """Overlapped cubes with R^-1 = 0: every cube's trans is I, each adds
+    dist = axb + xb (mean axb) once per covering cube, and the division by the
+    cube count returns xb and axb."""
+    inst = _with(_horizontal(12, nlat=9, nlon=13, halo=1), cube_n=4, overlap=2)
+    inst.obs.rinv[:] = 0.0
+    ref = orc.letkf_compute(inst, want_trans=True)
+    h = inst.cfg.halo
+    inner = (slice(h, h + inst.cfg.nlon), slice(h, h + inst.cfg.nlat))
+    assert np.abs(ref["xa"][inner] - inst.xb[inner]).max() < 1e-13
+    assert np.abs(ref["axa"][inner] - inst.axb[inner]).max() < 1e-13
+    assert np.all(ref["xa"][:h] == 0) and np.all(ref["axa"][:, :h] == 0)
 
 
 def test_mean_increment_is_xb_wbar(orc):

@@ -447,6 +447,7 @@ struct dida_letkf_handle {
   // length buckets: order list, block counts, segment starts, queue counters,
   // per-batch bucket statistics and timing ticks
   DevBuf b_order, b_blk, b_seg, b_q, b_stats, b_ticks;
+  DevBuf sp_xb, sp_xb32, sp_axb, sp_nan;  // device split of the full state
   int variant[kBuckets] = {0};        // products variant per bucket position
   std::vector<bool> split_timed;      // per batch: split eigensolver events recorded
   std::string arch;                   // device architecture name (tuning cache key)
@@ -600,7 +601,8 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->eig.rr_lam};
   for (DevBuf* b : bufs) b->release();
   h->batch_tot.release();
-  for (DevBuf* b : {&h->b_order, &h->b_blk, &h->b_seg, &h->b_q, &h->b_stats, &h->b_ticks})
+  for (DevBuf* b : {&h->b_order, &h->b_blk, &h->b_seg, &h->b_q, &h->b_stats, &h->b_ticks,
+                    &h->sp_xb, &h->sp_xb32, &h->sp_axb, &h->sp_nan})
     b->release();
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->g_ev)
@@ -820,6 +822,9 @@ struct StateArgs {
   double* xa_out = nullptr;
   double* spread_b = nullptr;
   double* spread_a = nullptr;
+  // the caller's full ensemble state, split on the device into xb / xb32 and
+  // axb (handle buffers) before the analysis
+  const double* x_in = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -1034,6 +1039,17 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
   const bool tiled = h->tiled;
   const bool queue = h->opt_schedule == 1 && !tiled;
   HIP_TRY(launch_zero_ints(st, h->eig.flags.as<int>(), 2));
+  if (sa.x_in) {
+    SplitParams spp{};
+    spp.x_in = sa.x_in;
+    spp.xb = const_cast<double*>(sa.xb);
+    spp.xb32 = const_cast<float*>(sa.xb32);
+    spp.axb = const_cast<double*>(sa.axb);
+    spp.n_elems = (long long)(c.nlat + 2 * c.halo) * (c.nlon + 2 * c.halo) * h->n;
+    spp.k = c.k;
+    spp.nan = h->sp_nan.as<unsigned long long>();
+    HIP_TRY(launch_split(st, spp));
+  }
 
   long long bi = 0;
   for (long long h0 = 0; h0 < h->n_h; h0 += mb, ++bi) {
@@ -1255,11 +1271,25 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
   return DIDA_OK;
 }
 
+// The reference stops on a NaN background member (module_da_parallel.F90:
+// 189-193); here the call fails with the first such element, in the
+// reference's (k, j, i) indices.
+static int nan_error(const dida_letkf_handle* h, const unsigned long long* nan) {
+  const auto& c = h->cfg;
+  const long long n_elems = (long long)(c.nlat + 2 * c.halo) * (c.nlon + 2 * c.halo) * h->n;
+  const long long el = n_elems - (long long)nan[1];
+  const long long col = el / h->n, lat_ext = c.nlat + 2 * c.halo;
+  return set_error(DIDA_ERR_ARG,
+                   "background NaN in %llu state elements; first at k=%lld j=%lld i=%lld",
+                   nan[0], el % h->n + 1, col % lat_ext + 1 - c.halo,
+                   col / lat_ext + 1 - c.halo);
+}
+
 // After the call's launches: one readback (row totals and counts, status,
 // flags, bucket statistics), the stats, and the largest row total of a batch
 // in need (the call's lists were cut if it exceeds row_cap).  Returns the
 // number of analyses still flagged.
-static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms,
+static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms, bool split,
                        dida_letkf_stats* stats, long long& need) {
   const auto& c = h->cfg;
   hipStream_t st = h->stream;
@@ -1270,6 +1300,9 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms,
   std::vector<long long> bst(queue ? nb * 2 * kBuckets : 0);
   std::vector<unsigned long long> tk(queue && h->opt_timing ? nb * 5 * kBuckets : 0);
   int fl[2] = {0, 0};
+  unsigned long long nan[2] = {0, 0};
+  if (split)
+    HIP_TRY(hipMemcpyAsync(nan, h->sp_nan.p, sizeof(nan), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(tot.data(), h->batch_tot.p, sizeof(long long) * nb, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(counts.data(), h->nobsl_dev.p, sizeof(int) * h->n_h, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(stt.data(), h->status.p, sizeof(int) * h->n_an, hipMemcpyDeviceToHost, st));
@@ -1281,6 +1314,7 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms,
     HIP_TRY(hipMemcpyAsync(tk.data(), h->b_ticks.p, sizeof(unsigned long long) * tk.size(),
                            hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  if (nan[0]) return nan_error(h, nan);
   need = 0;
   for (long long t : tot) need = std::max(need, t);
   if (need > h->row_cap) return 0;
@@ -1380,7 +1414,7 @@ static std::vector<uintptr_t> graph_key(const dida_letkf_handle* h, const StateA
   put(sa.xb), put(sa.xb32), put(sa.axb), put(sa.xa), put(sa.xa32), put(sa.axa);
   k.push_back((uintptr_t)sa.epi);
   putd(sa.polar_lat), putd(sa.gamma1), putd(sa.gamma2);
-  put(sa.xa_out), put(sa.spread_b), put(sa.spread_a);
+  put(sa.xa_out), put(sa.spread_b), put(sa.spread_a), put(sa.x_in);
   for (const DevBuf* b : {&h->cnt, &h->off, &h->scan_tmp, &h->row_idx, &h->row_gc, &h->row_wd,
                           &h->row_site, &h->nobsl_dev, &h->Abuf, &h->bbuf, &h->status, &h->trans,
                           &h->wout, &h->batch_tot, &h->b_order, &h->b_blk, &h->b_seg, &h->b_q,
@@ -1452,7 +1486,7 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
       if (tune) h->opt_tune = 0;
     }
     long long need = 0;
-    const int flagged = finish_call(h, !graph, gms, stats, need);
+    const int flagged = finish_call(h, !graph, gms, sa.x_in != nullptr, stats, need);
     if (flagged < 0) return flagged;
     if (need <= h->row_cap) {
       if (stats) {
@@ -1538,6 +1572,59 @@ int dida_letkf_analyze_epilogue_device_f32(dida_letkf_handle* h, const float* xb
   if (!xb) return set_error(DIDA_ERR_ARG, "null argument");
   return analyze_epilogue(h, nullptr, xb, axb, polar_lat, gamma1, gamma2, xa_out, spread_b,
                           spread_a, stats);
+}
+
+int dida_letkf_split_device(dida_letkf_handle* h, const double* x_in, double* xb, float* xb32,
+                            double* axb) {
+  if (!h || !x_in || !axb || (!xb == !xb32))
+    return set_error(DIDA_ERR_ARG, "null argument (give exactly one of xb, xb32)");
+  const auto& c = h->cfg;
+  hipStream_t st = h->stream;
+  RC_TRY(h->sp_nan.ensure(2 * sizeof(unsigned long long)));
+  SplitParams spp{};
+  spp.x_in = x_in;
+  spp.xb = xb;
+  spp.xb32 = xb32;
+  spp.axb = axb;
+  spp.n_elems = (long long)(c.nlat + 2 * c.halo) * (c.nlon + 2 * c.halo) * h->n;
+  spp.k = c.k;
+  spp.nan = h->sp_nan.as<unsigned long long>();
+  HIP_TRY(launch_split(st, spp));
+  unsigned long long nan[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(nan, h->sp_nan.p, sizeof(nan), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (nan[0]) return nan_error(h, nan);
+  return DIDA_OK;
+}
+
+int dida_letkf_run_da_device(dida_letkf_handle* h, const double* x_in, int state_f32,
+                             double polar_lat, double gamma1, double gamma2, double* xa_out,
+                             double* spread_b, double* spread_a, dida_letkf_stats* stats) {
+  if (!h || !x_in || !xa_out || !spread_b || !spread_a)
+    return set_error(DIDA_ERR_ARG, "null argument");
+  const auto& c = h->cfg;
+  const size_t ne = (size_t)(c.nlat + 2 * c.halo) * (c.nlon + 2 * c.halo) * h->n;
+  RC_TRY(h->sp_nan.ensure(2 * sizeof(unsigned long long)));
+  RC_TRY(h->sp_axb.ensure(ne * sizeof(double)));
+  StateArgs sa;
+  if (state_f32) {
+    RC_TRY(h->sp_xb32.ensure(ne * c.k * sizeof(float)));
+    sa.xb32 = h->sp_xb32.as<float>();
+  } else {
+    RC_TRY(h->sp_xb.ensure(ne * c.k * sizeof(double)));
+    sa.xb = h->sp_xb.as<double>();
+  }
+  sa.axb = h->sp_axb.as<double>();
+  sa.x_in = x_in;
+  sa.epi = 1;
+  sa.polar_lat = polar_lat;
+  sa.gamma1 = gamma1;
+  sa.gamma2 = gamma2;
+  sa.xa_out = xa_out;
+  sa.spread_b = spread_b;
+  sa.spread_a = spread_a;
+  HIP_TRY(launch_zero_halo(h->stream, xa_out, c.nlat, c.nlon, c.halo, (long long)h->n * c.k));
+  return run_analysis(h, sa, stats);
 }
 
 int dida_letkf_analyze(dida_letkf_handle* h, const double* xb, const double* axb, double* xa,

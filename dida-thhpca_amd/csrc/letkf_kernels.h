@@ -268,6 +268,21 @@ struct ObsOpParams {
   double* dep;
 };
 
+// The caller's split of the full ensemble state into mean and perturbations
+// (module_da_parallel.F90:186-211): per state element, axb = sum_e x / ens and
+// xb = x - axb (FP64, or rounded to FP32); elements with a NaN member are
+// counted, and nan[1] keeps n_elems - (first such element).
+struct SplitParams {
+  const double* x_in;    // (k, n_elems), member fastest
+  double* xb;            // FP64 perturbations, or null
+  float* xb32;           // FP32 perturbations, or null
+  double* axb;
+  long long n_elems;
+  int k;
+  unsigned long long* nan;  // [count, n_elems - first]; zeroed by the launcher
+};
+hipError_t launch_split(hipStream_t st, const SplitParams& p);
+
 // Overlapped cubes (letkf_compute.F90:211-255): each column's analysis is the
 // mean of the analyses of the cubes that cover it.
 struct OverlapParams {

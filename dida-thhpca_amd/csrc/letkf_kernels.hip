@@ -1891,6 +1891,53 @@ hipError_t launch_transform(hipStream_t st, const XformParams& p) {
   return hipGetLastError();
 }
 
+// A wavefront per state element: lane e holds members e and e + 64.
+template <int R>
+__global__ __launch_bounds__(256) void k_split(SplitParams p) {
+  const int lane = threadIdx.x & 63;
+  const long long el = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (el >= p.n_elems) return;
+  const double* x = p.x_in + el * p.k;
+  double v[R], sum = 0.0;
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = lane + 64 * r;
+    v[r] = e < p.k ? x[e] : 0.0;
+    bad = bad || (v[r] != v[r]);
+    sum += v[r];
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  const double mean = sum / p.k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = lane + 64 * r;
+    if (e >= p.k) continue;
+    if (p.xb) p.xb[el * p.k + e] = v[r] - mean;
+    else p.xb32[el * p.k + e] = (float)(v[r] - mean);
+  }
+  if (lane == 0) p.axb[el] = mean;
+  if (__any(bad) && lane == 0) {
+    atomicAdd(&p.nan[0], 1ull);
+    atomicMax(&p.nan[1], (unsigned long long)(p.n_elems - el));
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+hipError_t launch_split(hipStream_t st, const SplitParams& p) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p.nan, 2);
+  if (p.n_elems <= 0) return hipGetLastError();
+  const dim3 grid((unsigned)((p.n_elems + 3) / 4));
+  if (p.k <= 64) hipLaunchKernelGGL(k_split<1>, grid, dim3(256), 0, st, p);
+  else if (p.k <= 128) hipLaunchKernelGGL(k_split<2>, grid, dim3(256), 0, st, p);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 // One workgroup per column (halo columns included: they are zeroed, as
 // letkf_compute.F90:191-193 zeroes xa and axa before the loop); wavefront w
 // takes the state elements v = w, w + 4, ...  For each covering cube in loop

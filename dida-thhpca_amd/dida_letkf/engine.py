@@ -103,7 +103,8 @@ EXPORTS = [
     "dida_letkf_set_obs",
     "dida_letkf_set_obs_device", "dida_letkf_analyze", "dida_letkf_analyze_device",
     "dida_letkf_analyze_device_f32", "dida_letkf_analyze_epilogue_device",
-    "dida_letkf_analyze_epilogue_device_f32",
+    "dida_letkf_analyze_epilogue_device_f32", "dida_letkf_split_device",
+    "dida_letkf_run_da_device",
     "dida_letkf_get_transforms", "dida_letkf_n_analyses", "dida_letkf_core_batched",
     "dida_letkf_core_batched_noalloc",
     "letkf_core_", "dida_letkf_epilogue_device", "dida_letkf_last_error",
@@ -146,6 +147,9 @@ def lib(path: Optional[str] = None):
                                                      C.POINTER(DidaStats)]
     L.dida_letkf_analyze_epilogue_device_f32.argtypes = \
         L.dida_letkf_analyze_epilogue_device.argtypes
+    L.dida_letkf_split_device.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.dida_letkf_run_da_device.argtypes = [_vp, _vp, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, _vp, _vp, _vp, C.POINTER(DidaStats)]
     L.dida_letkf_get_transforms.argtypes = [_vp, C.c_longlong, _dp, _dp, _ip]
     L.dida_letkf_n_analyses.argtypes = [C.POINTER(DidaConfig)]
     L.dida_letkf_n_analyses.restype = C.c_longlong
@@ -367,6 +371,24 @@ class LetkfEngine:
         rc = fn(self.h, _vp(xb_ptr), _vp(axb_ptr), float(polar_lat), float(gamma1),
                 float(gamma2), _vp(xa_out_ptr), _vp(sb_ptr), _vp(sa_ptr), C.byref(self.stats))
         return _check(rc, "dida_letkf_analyze_epilogue_device" + ("_f32" if f32 else ""))
+
+    def split_device(self, x_in_ptr: int, xb_ptr: int = 0, xb32_ptr: int = 0,
+                     axb_ptr: int = 0) -> int:
+        """dnn2letkf on the device (module_da_parallel.F90:186-211): mean and
+        perturbations (FP64 into xb, or FP32 into xb32) of the full state."""
+        return _check(lib().dida_letkf_split_device(self.h, _vp(x_in_ptr), _vp(xb_ptr or None),
+                                                    _vp(xb32_ptr or None), _vp(axb_ptr)),
+                      "dida_letkf_split_device")
+
+    def run_da_device(self, x_in_ptr: int, polar_lat: float, gamma1: float, gamma2: float,
+                      xa_out_ptr: int, sb_ptr: int, sa_ptr: int, state_f32: bool = False) -> int:
+        """run_da_parallel's analysis step (module_da_parallel.F90:186-261) on
+        the device: split, analysis, fused letkf2dnn epilogue."""
+        rc = lib().dida_letkf_run_da_device(self.h, _vp(x_in_ptr), int(bool(state_f32)),
+                                            float(polar_lat), float(gamma1), float(gamma2),
+                                            _vp(xa_out_ptr), _vp(sb_ptr), _vp(sa_ptr),
+                                            C.byref(self.stats))
+        return _check(rc, "dida_letkf_run_da_device")
 
     def get_transforms(self):
         k = self.cfg.k

@@ -249,6 +249,25 @@ int dida_letkf_analyze_epilogue_device_f32(dida_letkf_handle *h, const float *xb
                                            double gamma2, double *xa_out, double *spread_b,
                                            double *spread_a, dida_letkf_stats *stats);
 
+/* Replaces run_da_parallel's dnn2letkf block (module_da_parallel.F90:
+ * 186-211) on the device: x_in is the full ensemble state xb_in(ens, n,
+ * 1-halo:nlat+halo, 1-halo:nlon+halo); axb = sum_e x_in / ens and
+ * xb = x_in - axb for every column, halo included, into xb (FP64) or xb32
+ * (FP32; give exactly one).  The reference stops on a NaN member; this call
+ * returns DIDA_ERR_ARG naming the first such element (k, j, i). */
+int dida_letkf_split_device(dida_letkf_handle *h, const double *x_in, double *xb, float *xb32,
+                            double *axb);
+
+/* run_da_parallel's analysis step on the device (module_da_parallel.F90:
+ * 186-261): the split above (into buffers of the handle; FP32 perturbations
+ * when state_f32), the analysis, and the letkf2dnn epilogue fused into the
+ * update as dida_letkf_analyze_epilogue_device does.  One readback at the
+ * end; a NaN background member fails the call as above (outputs are then
+ * undefined). */
+int dida_letkf_run_da_device(dida_letkf_handle *h, const double *x_in, int state_f32,
+                             double polar_lat, double gamma1, double gamma2, double *xa_out,
+                             double *spread_b, double *spread_a, dida_letkf_stats *stats);
+
 /* Per-analysis outputs of the last analyze call when keep_transforms was
  * set: trans (k*k per analysis, column-major), overline_w (k), nobsl.
  * Any pointer may be NULL.  n_analyses must equal the analysis count. */

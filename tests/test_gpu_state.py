@@ -178,3 +178,134 @@ def test_fused_epilogue_failed_analyses_keep_background():
     same = np.all(xo == full_b, axis=(2, 3))
     assert same.sum() == rc                       # one column per failed analysis
     assert np.array_equal(sa[same], sb[same])
+
+
+# ---------------------------------------------------------------------------
+# the caller's split on the device (module_da_parallel.F90:186-211) and the
+# whole run_da_parallel analysis step (:186-261)
+# ---------------------------------------------------------------------------
+
+def _split_np(x_in):
+    k = x_in.shape[-1]
+    axb = x_in.sum(axis=-1) / k
+    return x_in - axb[..., None], axb
+
+
+def test_split_device_fp64_and_f32():
+    import torch
+    inst = _instances()["c0_k20_halo"]
+    x_in = inst.xb + inst.axb[..., None] + 3.0
+    xb_r, axb_r = _split_np(x_in)
+    with _engine(inst) as eng:
+        xi = _dev(x_in)
+        xb = torch.zeros_like(xi)
+        xb32 = torch.zeros(xi.shape, dtype=torch.float32, device=xi.device)
+        axb = torch.zeros(x_in.shape[:-1], dtype=torch.float64, device=xi.device)
+        assert eng.split_device(xi.data_ptr(), xb_ptr=xb.data_ptr(), axb_ptr=axb.data_ptr()) == 0
+        torch.cuda.synchronize()
+        assert rel(axb.cpu().numpy(), axb_r) < 1e-14
+        assert np.abs(xb.cpu().numpy() - xb_r).max() < 1e-13 * np.abs(x_in).max()
+        # the perturbations are centred
+        assert np.abs(xb.cpu().numpy().sum(axis=-1)).max() < 1e-12 * np.abs(x_in).max()
+        axb.zero_()
+        assert eng.split_device(xi.data_ptr(), xb32_ptr=xb32.data_ptr(),
+                                axb_ptr=axb.data_ptr()) == 0
+        torch.cuda.synchronize()
+        assert rel(axb.cpu().numpy(), axb_r) < 1e-14
+        x32 = xb32.cpu().numpy().astype(np.float64)
+        assert np.abs(x32 - xb_r).max() <= 2 ** -23 * np.abs(xb_r).max() * 1.01
+
+
+def test_split_device_nan_is_an_error():
+    import torch
+    from dida_letkf import engine
+    inst = _instances()["c0_k20_halo"]
+    h = inst.cfg.halo
+    x_in = inst.xb + inst.axb[..., None]
+    # C order (lon, lat, n, k): Fortran xb_in(e, k, j, i) with j = lat, i = lon
+    x_in[5, 0, 0, 3] = np.nan
+    x_in[9, 7, 0, 0] = np.nan
+    with _engine(inst) as eng:
+        xi = _dev(x_in)
+        xb = torch.zeros_like(xi)
+        axb = torch.zeros(x_in.shape[:-1], dtype=torch.float64, device=xi.device)
+        with pytest.raises(engine.DidaError) as ei:
+            eng.split_device(xi.data_ptr(), xb_ptr=xb.data_ptr(), axb_ptr=axb.data_ptr())
+    msg = str(ei.value)
+    print(msg)
+    assert "in 2 state elements" in msg
+    assert f"k=1 j={0 + 1 - h} i={5 + 1 - h}" in msg
+
+
+@pytest.mark.parametrize("name", ["c2_k64", "c0_k20_halo", "c3_k128"])
+def test_run_da_device_fp64(name):
+    import copy
+    import orc
+    import torch
+    inst = _instances()[name]
+    x_in = inst.xb + inst.axb[..., None]
+    xb_r, axb_r = _split_np(x_in)
+    ref_inst = copy.copy(inst)
+    ref_inst.xb, ref_inst.axb = xb_r, axb_r
+    ref = orc.letkf_compute(ref_inst)
+    xo_r, sb_r, sa_r = orc.epilogue(ref_inst, ref["xa"], ref["axa"], polar_lat=60.0, gamma1=0.25,
+                                    gamma2=0.75)
+    cfg = inst.cfg
+    with _engine(inst) as eng:
+        xi = _dev(x_in)
+        xo = torch.full(x_in.shape, 7.0, dtype=torch.float64, device=xi.device)
+        sb = torch.zeros((cfg.nlon, cfg.nlat, cfg.n), dtype=torch.float64, device=xi.device)
+        sa = torch.zeros_like(sb)
+        assert eng.run_da_device(xi.data_ptr(), 60.0, 0.25, 0.75, xo.data_ptr(), sb.data_ptr(),
+                                 sa.data_ptr()) == 0
+        torch.cuda.synchronize()
+        e = dict(xa_out=rel(xo.cpu().numpy(), xo_r), spread_b=rel(sb.cpu().numpy(), sb_r),
+                 spread_a=rel(sa.cpu().numpy(), sa_r))
+    print(name, e)
+    assert max(e.values()) < TOL64, e
+
+
+@pytest.mark.parametrize("name", ["c2_k64", "c3_k128"])
+def test_run_da_device_f32_state(name):
+    import copy
+    import orc
+    import torch
+    inst = _instances()[name]
+    x_in = inst.xb + inst.axb[..., None]
+    xb_r, axb_r = _split_np(x_in)
+    ref_inst = copy.copy(inst)
+    ref_inst.xb, ref_inst.axb = xb_r, axb_r
+    ref = orc.letkf_compute(ref_inst)
+    xo_r, sb_r, sa_r = orc.epilogue(ref_inst, ref["xa"], ref["axa"], polar_lat=60.0, gamma1=0.2,
+                                    gamma2=0.8)
+    cfg = inst.cfg
+    with _engine(inst, precision=2) as eng:
+        xi = _dev(x_in)
+        xo = torch.zeros(x_in.shape, dtype=torch.float64, device=xi.device)
+        sb = torch.zeros((cfg.nlon, cfg.nlat, cfg.n), dtype=torch.float64, device=xi.device)
+        sa = torch.zeros_like(sb)
+        assert eng.run_da_device(xi.data_ptr(), 60.0, 0.2, 0.8, xo.data_ptr(), sb.data_ptr(),
+                                 sa.data_ptr(), state_f32=True) == 0
+        torch.cuda.synchronize()
+        xo, sb, sa = xo.cpu().numpy(), sb.cpu().numpy(), sa.cpu().numpy()
+    err = increment_error(ref_inst, xo, xo_r)
+    print(name, "run_da FP32 state + BF16x3 increment err", err)
+    assert max(err.values()) < 1e-5
+    assert rel(sb, sb_r) < 1e-6 and rel(sa, sa_r) < 1e-5
+
+
+def test_run_da_device_nan_fails():
+    import torch
+    from dida_letkf import engine
+    inst = _instances()["c0_k20_halo"]
+    cfg = inst.cfg
+    x_in = inst.xb + inst.axb[..., None]
+    x_in[3, 4, 0, 2] = np.inf - np.inf
+    with _engine(inst) as eng:
+        xi = _dev(x_in)
+        xo = torch.zeros(x_in.shape, dtype=torch.float64, device=xi.device)
+        sb = torch.zeros((cfg.nlon, cfg.nlat, cfg.n), dtype=torch.float64, device=xi.device)
+        sa = torch.zeros_like(sb)
+        with pytest.raises(engine.DidaError, match="background NaN in 1 state elements"):
+            eng.run_da_device(xi.data_ptr(), 90.0, 0.0, 1.0, xo.data_ptr(), sb.data_ptr(),
+                              sa.data_ptr())

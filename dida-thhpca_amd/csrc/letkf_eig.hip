@@ -631,6 +631,12 @@ void k_vec(EigParams p) {
           for (int r = 0; r < 4; ++r) refl[tile_off<KMAX>(pass * IPP + I, J, r, lane)] = acc[I][J][r];
     }
   }
+  if (__builtin_amdgcn_ballot_w64(in_r && !__builtin_isfinite(wbar))) {
+    __syncthreads();                               // the trans writes above are done
+    if (lane == 0) p.status[a] = kStatusNonFinite;
+    nan_transform(p, a);
+    return;
+  }
   if (p.w_out && in_r) p.w_out[a * (long long)k + lane] = wbar;
 }
 
@@ -751,7 +757,7 @@ void k_update(EigParams p) {
   long long a, ai, ar;
   if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
   if (p.status[a] != 0) {                // redone by the list pass, or failed
-    if (p.list || p.status[a] == kStatusNotRedone) copy_background_t<TS, EPI>(p, a);
+    if (p.list || keeps_background(p.status[a])) copy_background_t<TS, EPI>(p, a);
     return;
   }
   const int k = p.k;
@@ -1406,6 +1412,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             refl[tile_off<KMAX>(4 * pass + I, 2 * wv + J, rr, lane)] = acc[I][J][rr];
     }
   }
+  // every thread scans overline_w in LDS, so the decision is uniform
+  bool bad = false;
+  for (int f = 0; f < k; ++f) bad = bad || !__builtin_isfinite(sw[f]);
+  if (bad) {
+    __syncthreads();                               // the trans writes above are done
+    if (threadIdx.x == 0) p.status[a] = kStatusNonFinite;
+    nan_transform(p, a);
+    return;
+  }
   if (p.w_out && h == 0 && in_r) p.w_out[a * (long long)k + r] = sw[r];
 }
 
@@ -1424,7 +1439,7 @@ __global__ __launch_bounds__(256) void k_update128(EigParams p) {
   long long a, ai, ar;
   if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
   if (p.status[a] != 0) {                // redone by the list pass, or failed
-    if (p.list || p.status[a] == kStatusNotRedone) copy_background_t<TS, EPI>(p, a);
+    if (p.list || keeps_background(p.status[a])) copy_background_t<TS, EPI>(p, a);
     return;
   }
   const int k = p.k, kp = p.kp;

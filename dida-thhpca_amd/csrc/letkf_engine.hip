@@ -27,6 +27,7 @@
 #include "dida_letkf.h"
 #include "letkf_device.h"
 #include "letkf_kernels.h"
+#include "letkf_pool.h"
 
 using namespace dida;
 
@@ -63,19 +64,54 @@ namespace {
     if (rc_ < 0) return rc_;    \
   } while (0)
 
+thread_local DevicePool* tl_pool = nullptr;
+
+// Test hook: with DIDA_FAIL_ALLOC_AFTER=N set, the device allocations of a
+// host thread after its first N since the value was set fail (NOMEM).
+bool alloc_refused() {
+  thread_local std::string seen;
+  thread_local long long count = 0;
+  const char* v = std::getenv("DIDA_FAIL_ALLOC_AFTER");
+  if (!v || !*v) {
+    seen.clear();
+    return false;
+  }
+  if (seen != v) {
+    seen = v;
+    count = 0;
+  }
+  return count++ >= std::atoll(v);
+}
+
 // Growable device buffer, or (wrap) a fixed piece of memory it does not own:
-// ensure() then succeeds only within the piece and never allocates.
+// ensure() then succeeds only within the piece and never allocates.  While a
+// DevicePool is bound to the thread, growth takes blocks of that pool
+// instead of hipMalloc (and gives them back instead of hipFree).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   bool owned = true;
+  DevicePool* pool = nullptr;     // the pool p came from, if any
   int ensure(size_t n) {
     if (n <= bytes && p) return DIDA_OK;
     if (!owned)
       return set_error(DIDA_ERR_ARG, "fixed scratch piece of %zu bytes is too small (%zu needed)",
                        bytes, n);
-    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    release();
     if (n == 0) n = 16;
+    if (alloc_refused())
+      return set_error(DIDA_ERR_NOMEM, "device allocation of %zu bytes refused "
+                       "(DIDA_FAIL_ALLOC_AFTER)", n);
+    if (DevicePool* pl = tl_pool) {
+      const size_t off = pl->fit.alloc(n);
+      if (off == FitAlloc::npos)
+        return set_error(DIDA_ERR_NOMEM, "job memory region exhausted: %zu bytes needed, "
+                         "largest free block %zu of %zu", n, pl->fit.largest(), pl->fit.size());
+      p = pl->base + off;
+      pool = pl;
+      bytes = n;
+      return DIDA_OK;
+    }
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -85,18 +121,36 @@ struct DevBuf {
     return DIDA_OK;
   }
   void wrap(void* q, size_t n) {
+    release();
     p = q;
     bytes = n;
     owned = false;
   }
   void release() {
-    if (p && owned) (void)hipFree(p);
+    if (p && owned) {
+      if (pool) (void)pool->fit.release((size_t)((char*)p - pool->base), bytes);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
     owned = true;
+    pool = nullptr;
   }
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
+
+}  // namespace
+
+namespace dida {
+DevicePool* bind_thread_pool(DevicePool* pool) {
+  DevicePool* old = tl_pool;
+  tl_pool = pool;
+  return old;
+}
+DevicePool* thread_pool() { return tl_pool; }
+}  // namespace dida
+
+namespace {
 
 __global__ void k_f64_to_f32(long long n, const double* __restrict__ in, float* __restrict__ out) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,13 +436,24 @@ static long long cubes_h(const dida_letkf_config& c) {
 
 namespace dida {
 // columns whose per-batch scratch fits in `bytes`
-long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes) {
+long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes, int n_rows) {
   const int kp = kp_of(c.k);
   const int cz = c.cube_nz > 0 ? c.cube_nz : c.nz;
   const long long nzc = c.vert_dist > 0.0 ? (c.nz + cz - 1) / cz : 1;
   const long long per_col =
-      nzc * ((long long)kp * (kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k));
+      nzc * ((long long)kp * (kp + 1) * 8 + (long long)eig_bytes_per_analysis(c.k)) +
+      20 * rows_per_column_bound(c, n_rows);
   return std::max<long long>(1, (long long)(bytes / (size_t)per_col));
+}
+
+// Local obs rows of a column to provision for when the obs count is known
+// beforehand (serving): six times the rows of uniformly spread obs within the
+// support radius 2c, plus 64; 0 for n_rows = 0.
+long long rows_per_column_bound(const dida_letkf_config& c, int n_rows) {
+  if (n_rows <= 0) return 0;
+  const double th = std::min(M_PI, 2.0 * c.local_dist / kEarthF32);
+  const double frac = std::min(1.0, 6.0 * 0.5 * (1.0 - std::cos(th)));
+  return std::min<long long>(n_rows, (long long)(frac * n_rows) + 64);
 }
 }  // namespace dida
 
@@ -1980,11 +2045,10 @@ size_t dida_letkf_job_bytes(const dida_letkf_config* c, int n_sites, int n_rows)
   b += nr * (8 * k + 8 * 3 + 4);                                    // rows
   if (c->precision != DIDA_PREC_FP64) b += nr * 4 * k;              // FP32 copy of yb
   b += 2 * 4 * 200000;                                              // bins (upper bound)
-  // local obs rows of one batch: support cap fraction of the sphere, x2
-  const double th = 2.0 * c->local_dist / 6371.39306640625;
-  const double frac = std::min(1.0, (1.0 - std::cos(std::min(th, M_PI))) / 2.0 * 2.0);
-  const size_t rows_col = std::min<size_t>(nr, (size_t)(frac * nr) + 64);
-  b += 20 * mb * rows_col;                                          // row_idx, row_gc, row_wd
+  // local obs rows of one batch (row_idx, row_gc, row_wd), provisioned at
+  // six times the uniform density: a call whose lists need more runs again
+  // with the measured need, within this
+  b += 20 * mb * (size_t)rows_per_column_bound(*c, n_rows);
   b += 16 * (mb + 1) * 2;
   b += mb * nzc * (8 * kp * kp + 8 * kp + eig_bytes_per_analysis(c->k));
   b += rerun_cap_of(c->k, (long long)(mb * nzc)) * rerun_bytes_per_slot(c->k);

@@ -10,8 +10,10 @@ int set_error(int code, const char* fmt, ...);
 }  // namespace dida
 struct dida_letkf_config;
 namespace dida {
-// analysis columns per batch whose scratch fits in `bytes`
-long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes);
+// analysis columns per batch whose scratch (with the row lists of n_rows
+// obs rows, provisioned as rows_per_column_bound) fits in `bytes`
+long long batch_columns_for_bytes(const dida_letkf_config& c, size_t bytes, int n_rows = 0);
+long long rows_per_column_bound(const dida_letkf_config& c, int n_rows);
 constexpr int kRotCapPerK2 = 2;   // rotation log: 2 k^2 Givens per analysis, plus
 // the identity entries that align each chase to blocks of 8 positions
 inline int rot_cap_of(int k) { return kRotCapPerK2 * k * k + 16 * k + 64; }
@@ -19,9 +21,16 @@ constexpr int kHdrCapPerK = 8;    // chase log: 8 k chases per analysis
 // eigensolver status per analysis: 0 converged; l + 1 > 0: the QL of
 // eigenvalue l did not converge; kStatusLogFull: the rotation or chase log
 // filled up; kStatusNotRedone: flagged by the first pass when the list pass
-// was full (the analysis keeps its background)
+// was full (the analysis keeps its background); kStatusNonFinite: overline_w
+// is not finite (a NaN or Inf in d or R^-1 with a finite A; the analysis
+// keeps its background)
 constexpr int kStatusLogFull = -1;
 constexpr int kStatusNotRedone = -2;
+constexpr int kStatusNonFinite = -3;
+// the update keeps the background of an analysis with this first-pass status
+__host__ __device__ inline bool keeps_background(int status) {
+  return status == kStatusNotRedone || status == kStatusNonFinite;
+}
 
 struct Geom {
   const float* glat;   // nlat, real(4) degrees

@@ -1245,6 +1245,14 @@ __global__ __launch_bounds__(64) void k_transform(XformParams p) {
       }
     }
   }
+  // a non-finite overline_w (NaN or Inf in d or R^-1): trans = I, so the
+  // update returns the background up to its re-centring
+  if (__builtin_amdgcn_ballot_w64(act_r && !__builtin_isfinite(wbar))) {
+    status = kStatusNonFinite;
+#pragma unroll
+    for (int f = 0; f < KMAX; ++f) tcol[f] = (f == lane) ? 1.0 : 0.0;
+    wbar = __longlong_as_double(0x7ff8000000000000LL);
+  }
   if (p.status) p.status[a] = status;
   if (p.trans_out && act_r) {
     // trans(f, e) column-major: column e is tcol of lane e

@@ -2,7 +2,12 @@
 //
 // FIFO admission by HBM budget and an active-job limit; every admitted job
 // runs on its own host thread, engine handle and HIP stream, so kernels of
-// different jobs overlap on the device.  A job's ensemble stays in device
+// different jobs overlap on the device.  The budget is allocated once, when
+// the server is created: an admitted job gets a contiguous region of it
+// (first fit), and every device buffer of the job -- its engine handle's,
+// its state and operator buffers -- is carved from that region, so no
+// hipMalloc or hipFree (which synchronises the device) runs while other
+// jobs' kernels are in flight.  A job's ensemble stays in device
 // memory between its cycles: the analysis of cycle c (mean-zero perturbations
 // and mean) is the background of cycle c+1, ping-ponging between the caller's
 // output arrays and one server-owned state buffer.
@@ -21,7 +26,10 @@
 
 #include "dida_letkf.h"
 #include "letkf_kernels.h"
+#include "letkf_pool.h"
 
+using dida::DevicePool;
+using dida::FitAlloc;
 using dida::set_error;
 
 namespace {
@@ -42,6 +50,26 @@ struct Job {
   dida_serve_status st{};
   std::atomic<bool> cancel{false};
   int max_sites = 0, max_rows = 0;
+  size_t region = FitAlloc::npos;      // offset of the job's region in the arena
+  DevicePool pool;                     // the region and its allocator
+};
+
+// A block of a job's region, given back when it goes out of scope.
+struct PoolBlock {
+  DevicePool* pool = nullptr;
+  size_t off = FitAlloc::npos, n = 0;
+  double* get(DevicePool* pl, size_t bytes) {
+    pool = pl;
+    n = bytes;
+    off = pl->fit.alloc(bytes);
+    return off == FitAlloc::npos ? nullptr : reinterpret_cast<double*>(pl->base + off);
+  }
+  void drop() {
+    if (pool && off != FitAlloc::npos) (void)pool->fit.release(off, n);
+    pool = nullptr;
+    off = FitAlloc::npos;
+  }
+  ~PoolBlock() { drop(); }
 };
 
 }  // namespace
@@ -56,6 +84,8 @@ struct dida_serve {
   std::map<long long, std::thread> workers;   // running (or just finished) jobs
   std::vector<long long> finished;            // workers to join
   size_t used = 0;
+  char* arena = nullptr;               // the whole budget, allocated at create
+  FitAlloc fit;                        // regions of admitted jobs
   int active = 0;
   long long next_id = 1;
   bool stopping = false;
@@ -64,9 +94,12 @@ struct dida_serve {
 
 static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
   (void)hipSetDevice(s->device);
+  // every device buffer below comes from the job's region
+  DevicePool* prev_pool = dida::bind_thread_pool(&j->pool);
   dida_serve_job& sp = j->spec;
   dida_letkf_handle* h = nullptr;
   int rc = dida_letkf_create(&sp.cfg, &h);
+  PoolBlock b_xt, b_at, b_yb, b_dep;
   double* xt = nullptr;
   double* at = nullptr;
   double* yb_op = nullptr;
@@ -77,13 +110,15 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
   const size_t k = sp.cfg.k;
   if (rc == DIDA_OK) rc = dida_letkf_set_grid(h, j->glat.data(), j->glon.data());
   if (rc == DIDA_OK && sp.n_cycles > 1) {
-    if (hipMalloc(&xt, n_state * k * 8) != hipSuccess || hipMalloc(&at, n_state * 8) != hipSuccess)
-      rc = DIDA_ERR_NOMEM;
+    xt = b_xt.get(&j->pool, n_state * k * 8);
+    at = b_at.get(&j->pool, n_state * 8);
+    if (!xt || !at) rc = set_error(DIDA_ERR_NOMEM, "job region exhausted (state buffers)");
   }
   if (rc == DIDA_OK && j->op_mode) {
     const size_t nr = (size_t)std::max(j->max_rows, 1);
-    if (hipMalloc(&yb_op, nr * k * 8) != hipSuccess || hipMalloc(&dep_op, nr * 8) != hipSuccess)
-      rc = DIDA_ERR_NOMEM;
+    yb_op = b_yb.get(&j->pool, nr * k * 8);
+    dep_op = b_dep.get(&j->pool, nr * 8);
+    if (!yb_op || !dep_op) rc = set_error(DIDA_ERR_NOMEM, "job region exhausted (operator)");
   }
   // ping-pong: cycle c reads (bx, ba) and writes (ox, oa); the last cycle
   // writes the caller's (xa, axa)
@@ -126,12 +161,12 @@ static void run_job(dida_serve* s, std::shared_ptr<Job> j) {
     ba = oa;
   }
   if (h) dida_letkf_destroy(h);
-  if (xt) (void)hipFree(xt);
-  if (at) (void)hipFree(at);
-  if (yb_op) (void)hipFree(yb_op);
-  if (dep_op) (void)hipFree(dep_op);
+  for (PoolBlock* b : {&b_xt, &b_at, &b_yb, &b_dep}) b->drop();
+  dida::bind_thread_pool(prev_pool);
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    // every block is back in the job's region; the region goes back to the arena
+    s->fit.release(j->region, j->st.bytes);
     j->st.t_end = now_s();
     j->st.rc = rc;
     j->st.n_analyses = n_an;
@@ -160,15 +195,17 @@ static void dispatch_loop(dida_serve* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
+    // the head job fits when the arena has a free block of its size
+    auto fits = [&] {
+      return FitAlloc::round(s->queue.front()->st.bytes) <= s->fit.largest();
+    };
     s->cv.wait(lk, [&] {
       if (s->stopping || !s->finished.empty()) return true;
       if (s->queue.empty() || s->active >= s->cfg.max_active) return false;
-      return s->used + s->queue.front()->st.bytes <= s->cfg.hbm_budget_bytes;
+      return fits();
     });
     reap_workers(s);
-    if (!s->stopping &&
-        (s->queue.empty() || s->active >= s->cfg.max_active ||
-         s->used + s->queue.front()->st.bytes > s->cfg.hbm_budget_bytes))
+    if (!s->stopping && (s->queue.empty() || s->active >= s->cfg.max_active || !fits()))
       continue;
     if (s->stopping && s->queue.empty()) break;
     if (s->stopping) {
@@ -189,6 +226,9 @@ static void dispatch_loop(dida_serve* s) {
       s->cv.notify_all();
       continue;
     }
+    j->region = s->fit.alloc(j->st.bytes);
+    j->pool.base = s->arena + j->region;
+    j->pool.fit.reset(FitAlloc::round(j->st.bytes));
     s->used += j->st.bytes;
     s->active += 1;
     j->st.state = DIDA_JOB_RUNNING;
@@ -211,6 +251,12 @@ int dida_serve_create(const dida_serve_config* cfg, dida_serve** out) {
     delete s;
     return DIDA_ERR_HIP;
   }
+  s->fit.reset(cfg->hbm_budget_bytes);
+  if (hipMalloc(&s->arena, s->fit.size()) != hipSuccess) {
+    delete s;
+    return set_error(DIDA_ERR_NOMEM, "serve: the HBM budget of %zu bytes cannot be allocated",
+                     (size_t)cfg->hbm_budget_bytes);
+  }
   now_s();
   s->dispatcher = std::thread(dispatch_loop, s);
   *out = s;
@@ -232,10 +278,6 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
     j->spec.cfg.hyai = nullptr;
     j->spec.cfg.hybi = nullptr;
   }
-  // a serving job's batches are bounded to 4 GiB of scratch unless set
-  if (j->spec.cfg.max_batch_columns <= 0)
-    j->spec.cfg.max_batch_columns = (int)std::min<long long>(
-        dida::batch_columns_for_bytes(job->cfg, 4ull << 30), 1 << 30);
   j->glat.assign(job->grid_lat, job->grid_lat + job->cfg.nlat);
   j->glon.assign(job->grid_lon, job->grid_lon + job->cfg.nlon);
   j->obs.assign(job->obs, job->obs + job->n_cycles);
@@ -253,9 +295,14 @@ int dida_serve_submit(dida_serve* s, const dida_serve_job* job, long long* job_i
     j->max_sites = std::max(j->max_sites, o.n_sites);
     j->max_rows = std::max(j->max_rows, o.n_rows);
   }
+  // a serving job's batches are bounded to 4 GiB of scratch, row lists
+  // included, unless set
+  if (j->spec.cfg.max_batch_columns <= 0)
+    j->spec.cfg.max_batch_columns = (int)std::min<long long>(
+        dida::batch_columns_for_bytes(job->cfg, 4ull << 30, j->max_rows), 1 << 30);
   j->st.bytes = dida_letkf_job_bytes(&j->spec.cfg, j->max_sites, j->max_rows);
   if (j->st.bytes == 0) return set_error(DIDA_ERR_ARG, "submit: bad job configuration");
-  if (j->st.bytes > s->cfg.hbm_budget_bytes)
+  if (FitAlloc::round(j->st.bytes) > s->fit.size())
     return set_error(DIDA_ERR_ADMISSION, "submit: job needs %zu bytes, budget %zu", j->st.bytes,
                      (size_t)s->cfg.hbm_budget_bytes);
   {
@@ -322,6 +369,7 @@ int dida_serve_destroy(dida_serve* s) {
   // running jobs finish on their own threads
   for (auto& t : s->workers)
     if (t.second.joinable()) t.second.join();
+  if (s->arena) (void)hipFree(s->arena);
   delete s;
   return DIDA_OK;
 }

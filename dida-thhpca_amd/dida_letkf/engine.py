@@ -487,12 +487,13 @@ class Server:
         self._keep = {}
 
     def submit(self, cfg: LetkfConfig, grid_lat, grid_lon, xb_ptr, axb_ptr, obs_cycles,
-               xa_ptr, axa_ptr) -> int:
+               xa_ptr, axa_ptr, max_batch_columns: int = 0) -> int:
         """obs_cycles: list of dicts with device pointers n_sites, site_lat, site_lon,
         site_row_ptr, n_rows, rinv, vcoord and either yb, dep or (operator mode)
-        row_var, row_ph, yo."""
+        row_var, row_ph, yo.  max_batch_columns: 0 lets the server bound each
+        batch's scratch to 4 GiB."""
         job = ServeJob()
-        job.cfg = to_c_config(cfg)
+        job.cfg = to_c_config(cfg, max_batch_columns=max_batch_columns)
         if cfg.hyai is not None:          # copied by the server at submit
             hy = np.ascontiguousarray(cfg.hyai, np.float64)
             hb = np.ascontiguousarray(cfg.hybi, np.float64)

@@ -363,7 +363,14 @@ int dida_letkf_epilogue_device(dida_letkf_handle *h, const double *xb, const dou
 typedef struct dida_serve dida_serve;
 
 typedef struct dida_serve_config {
-    size_t hbm_budget_bytes;  /* device memory admitted jobs may hold in total */
+    size_t hbm_budget_bytes;  /* device memory admitted jobs may hold in total:
+                                 allocated once by dida_serve_create (rounded
+                                 down to 256 bytes); each admitted job gets a
+                                 contiguous region of it (its size rounded up
+                                 to 256 bytes, first fit, FIFO) from which
+                                 every device buffer of the job is carved, so
+                                 no device allocation or free runs while other
+                                 jobs are in flight */
     int max_active;           /* jobs running at once (one stream each), >= 1 */
 } dida_serve_config;
 

@@ -32,12 +32,16 @@ def _dev_obs(torch, dev, obs):
 
 def _jobs(torch, dev):
     from dida_letkf import synth
-    specs = [("c0", dict(seed=31, nlat=18, nlon=36, n_obs=700)),
-             ("c2", dict(seed=32, nlat=16, nlon=32, nz=3, n_obs=9000, local_dist=800.0)),
-             ("c1", dict(seed=33, nlat=10, nlon=20, nz=4, n_obs=2500)),
-             ("c0", dict(seed=34, nlat=12, nlon=24, n_obs=400))]
+    # (config, generator arguments, max_batch_columns): a k = 128 job, and a
+    # job whose cycles run in several batches
+    specs = [("c0", dict(seed=31, nlat=18, nlon=36, n_obs=700), 0),
+             ("c2", dict(seed=32, nlat=16, nlon=32, nz=3, n_obs=9000, local_dist=800.0), 0),
+             ("c1", dict(seed=33, nlat=10, nlon=20, nz=4, n_obs=2500), 0),
+             ("c0", dict(seed=34, nlat=12, nlon=24, n_obs=400), 0),
+             ("c3", dict(seed=35, k=128, nlat=10, nlon=20, nz=2, n_obs=8000, local_dist=900.0), 0),
+             ("c2", dict(seed=36, nlat=20, nlon=40, nz=2, n_obs=6000, local_dist=700.0), 96)]
     jobs = []
-    for name, kw in specs:
+    for name, kw, mb in specs:
         a = synth.generate(name, **kw)
         kw2 = dict(kw)
         kw2["seed"] = kw["seed"] + 100          # second cycle's obs
@@ -45,7 +49,7 @@ def _jobs(torch, dev):
         t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
         o0, p0 = _dev_obs(torch, dev, a.obs)
         o1, p1 = _dev_obs(torch, dev, b.obs)
-        jobs.append(dict(inst=a, xb=t(a.xb), axb=t(a.axb), obs=[p0, p1], keep=[o0, o1]))
+        jobs.append(dict(inst=a, xb=t(a.xb), axb=t(a.axb), obs=[p0, p1], keep=[o0, o1], mb=mb))
     return jobs
 
 
@@ -58,7 +62,7 @@ def _run(server_cls, torch, jobs, max_active, budget=64 << 30):
             axa = torch.zeros_like(j["axb"])
             jid = srv.submit(j["inst"].cfg, j["inst"].grid_lat, j["inst"].grid_lon,
                              j["xb"].data_ptr(), j["axb"].data_ptr(), j["obs"], xa.data_ptr(),
-                             axa.data_ptr())
+                             axa.data_ptr(), max_batch_columns=j.get("mb", 0))
             ids.append((jid, xa, axa))
             if max_active == 1:
                 st = srv.wait(jid)
@@ -72,12 +76,28 @@ def _run(server_cls, torch, jobs, max_active, budget=64 << 30):
 
 def test_alone_equals_mixed(torch_dev):
     torch, dev = torch_dev
+    from dida_letkf import engine
     from dida_letkf.engine import Server
     jobs = _jobs(torch, dev)
     alone = _run(Server, torch, jobs, max_active=1)
     mixed = _run(Server, torch, jobs, max_active=4)
     for (xa1, axa1, s1), (xa2, axa2, s2) in zip(alone, mixed):
         assert np.array_equal(xa1, xa2) and np.array_equal(axa1, axa2)
+    # the same jobs through an engine handle, cycle by cycle: same bits
+    for j, (xa1, axa1, _) in zip(jobs, alone):
+        with engine.LetkfEngine(j["inst"].cfg, max_batch_columns=j["mb"]) as eng:
+            eng.set_grid(j["inst"].grid_lat, j["inst"].grid_lon)
+            eng.set_stream(torch.cuda.current_stream().cuda_stream)
+            bx, ba = j["xb"], j["axb"]
+            for o in j["obs"]:
+                ox, oa = torch.zeros_like(bx), torch.zeros_like(ba)
+                eng.set_obs_device(o["n_sites"], o["site_lat"], o["site_lon"],
+                                   o["site_row_ptr"], o["n_rows"], o["yb"], o["dep"], o["rinv"],
+                                   o["vcoord"])
+                eng.analyze_device(bx.data_ptr(), ba.data_ptr(), ox.data_ptr(), oa.data_ptr())
+                bx, ba = ox, oa
+            torch.cuda.synchronize()
+        assert np.array_equal(bx.cpu().numpy(), xa1) and np.array_equal(ba.cpu().numpy(), axa1)
     # jobs ran concurrently in the mixed run: some pair of [start, end] overlaps
     iv = [(s[2]["t_start"], s[2]["t_end"]) for s in mixed]
     assert any(a[0] < b[1] and b[0] < a[1] for i, a in enumerate(iv) for b in iv[i + 1:])
@@ -123,7 +143,8 @@ def test_admission_refusal_and_cancel(torch_dev):
                        [p0], torch.zeros_like(xb).data_ptr(), torch.zeros_like(axb).data_ptr())
         assert "(-5)" in str(ei.value)
     outs = [(torch.zeros_like(xb), torch.zeros_like(axb)) for _ in range(4)]
-    with engine.Server(need + 1, 1) as srv:     # room for exactly one job at a time
+    # room for exactly one job at a time (regions are whole 256-byte granules)
+    with engine.Server(need + 512, 1) as srv:
         ids = [srv.submit(inst.cfg, inst.grid_lat, inst.grid_lon, xb.data_ptr(), axb.data_ptr(),
                           [p0, p0, p0], xa.data_ptr(), axa.data_ptr()) for xa, axa in outs]
         srv.cancel(ids[-1])

@@ -338,8 +338,11 @@ def main():
         print(json.dumps(dict(copy_probe_bytes=8 * n, launches=3)), flush=True)
         return
     if args.serve_only:
-        out = serving_leg(torch, dev, args, args.mixed)
-        print(json.dumps(dict(metric="LETKF serving job latency (p50/p99)", serving=out)), flush=True)
+        out = dict(metric="LETKF serving job latency (p50/p99)",
+                   serving=serving_leg(torch, dev, args, args.mixed))
+        if not args.no_fp64:
+            out["serving_fp64"] = serving_leg(torch, dev, args, PREC_FP64)
+        print(json.dumps(out), flush=True)
         return
     t0 = time.time()
     inst = synth.generate(args.config, seed=args.seed, k=args.k or None)
@@ -503,6 +506,8 @@ def main():
         del d
         torch.cuda.empty_cache()
         line["serving"] = serving_leg(torch, dev, args, args.mixed)
+        if not args.no_fp64:
+            line["serving_fp64"] = serving_leg(torch, dev, args, PREC_FP64)
     print(json.dumps(line), flush=True)
 
 

@@ -1899,6 +1899,39 @@ void release_core(CoreBufs& cb, EigScratch& es) {
   for (int i = 0; i < nb; ++i) bufs[i]->release();
 }
 
+// The drop-in letkf_core_ is called once per local analysis (64,800 times a
+// cycle at 360x180): each calling host thread keeps, per device, a stream and
+// the buffers of its largest call so far, until the process ends, so that a
+// call costs its copies and launches only.
+struct CoreCtx {
+  int device = -1;
+  hipStream_t st = nullptr;
+  CoreBufs cb;
+  EigScratch es;
+};
+
+int core_ctx(CoreCtx** out) {
+  thread_local std::vector<CoreCtx*> ctxs;   // never freed: see above
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  for (CoreCtx* c : ctxs)
+    if (c->device == dev) {
+      *out = c;
+      return DIDA_OK;
+    }
+  CoreCtx* c = new (std::nothrow) CoreCtx();
+  if (!c) return set_error(DIDA_ERR_NOMEM, "host allocation failed");
+  c->device = dev;
+  const hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_error(DIDA_ERR_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+  }
+  ctxs.push_back(c);
+  *out = c;
+  return DIDA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2017,8 +2050,17 @@ void letkf_core_(const int* nanals, const int* nobsl, const double* hxens_in,
   (void)pflag;
   long long rp[2] = {0, (long long)*nobsl};
   int st = 0;
-  int rc = dida_letkf_core_batched(*nanals, 1, rp, hxens_in, rdiaginv, dep, rloc, trans,
-                                   overline_w, *inflation_factor, DIDA_PREC_FP64, &st);
+  CoreCtx* cx = nullptr;
+  int rc = core_check(*nanals, 1, rp, hxens_in, rdiaginv, dep, rloc, trans, overline_w,
+                      *inflation_factor, DIDA_PREC_FP64);
+  if (rc == DIDA_OK) rc = core_ctx(&cx);
+  if (rc == DIDA_OK) {
+    rc = core_run(cx->st, cx->cb, cx->es, 0, *nanals, 1, rp, hxens_in, rdiaginv, dep, rloc,
+                  trans, overline_w, *inflation_factor, DIDA_PREC_FP64, &st);
+    const hipError_t e = hipStreamSynchronize(cx->st);
+    if (rc >= 0 && e != hipSuccess)
+      rc = set_error(DIDA_ERR_HIP, "letkf_core_: %s", hipGetErrorString(e));
+  }
   if (rc < 0 && *nanals > 0) {
     // no result: the outputs are NaN, so the caller's update cannot pass for one
     const double nan = std::nan("");

@@ -56,7 +56,8 @@ class ServeResult:
     analyses: int = 0
     makespan: float = 0.0
     failed: int = 0
-    client_stalls: int = 0
+    client_stalls: int = 0          # jobs whose submit waited for output memory
+    client_stall_s: float = 0.0     # the waits' total
 
     def summary(self) -> dict:
         lat = np.asarray(self.latency)
@@ -72,6 +73,7 @@ class ServeResult:
             analyses_per_s=self.analyses / self.makespan if self.makespan > 0 else None,
             failed=self.failed,
             client_stalls=self.client_stalls,
+            client_stall_s=round(self.client_stall_s, 3),
         )
 
 
@@ -229,10 +231,13 @@ def run(torch, jobs: List[JobSpec], pools: Pools, precision: int = PREC_BF16X3,
             grid = pools.grids[j.grid]
             xb, axb, glat, glon = pools.background(j.grid, j.k)
             need = xb.numel() * 8 + axb.numel() * 8
-            while out_bytes + need > out_cap and outstanding:
+            if out_bytes + need > out_cap and outstanding:
                 res.client_stalls += 1
-                if not reap(srv):
-                    time.sleep(0.002)
+                ts = srv.now()
+                while out_bytes + need > out_cap and outstanding:
+                    if not reap(srv):
+                        time.sleep(0.002)
+                res.client_stall_s += srv.now() - ts
             xa = torch.empty_like(xb)
             axa = torch.empty_like(axb)
             cyc, keep = [], [xa, axa]

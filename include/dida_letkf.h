@@ -309,7 +309,11 @@ int dida_letkf_core_batched_noalloc(int nanals, long long n_analyses, const long
 /* Drop-in for the external letkf_core (letkf_core.F90:14-33), the same
  * by-reference signature the Sunway C kernel uses
  * (sw/DA/LETKF/src/letkf_half/letkf_core_half_master.c:24).  Runs one
- * analysis on the GPU in FP64; pflag is accepted and ignored. */
+ * analysis on the GPU in FP64; pflag is accepted and ignored.  Each calling
+ * thread keeps, per device, a stream and the device buffers of its largest
+ * call until the process ends, so a call costs its copies and launches.  On
+ * an error, trans and overline_w are NaN and the error is printed to stderr
+ * (the reference's interface has no status). */
 void letkf_core_(const int *nanals, const int *nobsl, const double *hxens_in,
                  const float *rdiaginv, const double *dep, const double *rloc, double *trans,
                  double *overline_w, const double *inflation_factor, const int *pflag);

@@ -72,6 +72,31 @@ def test_letkf_core_dropin(k, p):
     assert np.abs(w_g - w_o).max() < TOL64 * max(1.0, np.abs(w_o).max())
 
 
+def test_letkf_core_repeated_calls_reuse_context():
+    """The drop-in is called once per analysis: calls that grow and shrink
+    reuse the calling thread's stream and buffers and keep matching the
+    oracle; an unsupported k gives NaN outputs."""
+    import time
+    import orc
+    from dida_letkf import engine
+    sizes = [(40, 300), (40, 1200), (64, 50), (40, 10), (128, 400), (20, 0), (64, 2000),
+             (40, 300)] * 4
+    times = []
+    for i, (k, p) in enumerate(sizes):
+        Y, rinv, dep, rloc = _case(k, p, 7000 + i)
+        t0 = time.perf_counter()
+        t_g, w_g = engine.letkf_core(Y, rinv, dep, rloc, inflation_factor=1.1)
+        times.append(time.perf_counter() - t0)
+        t_o, w_o, _ = orc.letkf_core(Y, rinv, dep, rloc, inflation_factor=1.1)
+        assert rel(t_g, t_o) < TOL64, (k, p)
+        assert np.abs(w_g - w_o).max() < TOL64 * max(1.0, np.abs(w_o).max())
+    print("letkf_core_ per call: median %.3f ms, first %.3f ms" %
+          (1e3 * float(np.median(times)), 1e3 * times[0]))
+    Y, rinv, dep, rloc = _case(130, 20, 1)
+    t_g, w_g = engine.letkf_core(Y, rinv, dep, rloc)
+    assert np.all(np.isnan(t_g)) and np.all(np.isnan(w_g))
+
+
 @pytest.mark.parametrize("k,cuts", [(100, (30, 70)), (128, (40, 90)), (128, (60, 64, 66)),
                                     (128, (8, 16, 24, 32, 40, 48, 56))])
 def test_letkf_core_block_structured(k, cuts):

@@ -11,6 +11,8 @@
 // (letkf_obs.F90:158-170).
 #include <algorithm>
 
+#include "dida_letkf.h"
+
 #include "letkf_device.h"
 #include "letkf_kernels.h"
 
@@ -195,3 +197,75 @@ hipError_t launch_obs_operator(hipStream_t st, const ObsOpParams& p) {
 }
 
 }  // namespace dida
+
+// ---------------------------------------------------------------------------
+// Observation records as read_obs receives them (read_obs_mod.F90:338-449):
+// host-side conversion into the sites and rows of the observation operator.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// record i starts a new site: every record of a single-level type; on a
+// multi-level type (ADPUPA, GPSRO) the records with lev == 1 (:411-416)
+bool starts_site(int multi_level, float lev) { return !multi_level || lev == 1.0f; }
+
+}  // namespace
+
+extern "C" {
+
+int dida_obs_records_count(int n_records, int nvar, int multi_level, const float* lev,
+                           int* n_sites, int* n_rows) {
+  if (n_records < 0 || nvar < 1 || !n_sites || !n_rows || (n_records > 0 && multi_level && !lev))
+    return dida::set_error(DIDA_ERR_ARG, "obs records: bad arguments");
+  int s = 0;
+  for (int i = 0; i < n_records; ++i) {
+    const float l = multi_level ? lev[i] : 1.0f;
+    if (starts_site(multi_level, l)) ++s;
+    else if (s == 0)
+      return dida::set_error(DIDA_ERR_ARG, "obs records: record %d continues a profile that "
+                             "never started (lev = %g before any lev = 1)", i, (double)l);
+    if (!(l >= 1.0f) || l != (float)(int)l)
+      return dida::set_error(DIDA_ERR_ARG, "obs records: record %d has level %g", i, (double)l);
+  }
+  *n_sites = s;
+  *n_rows = n_records * nvar;
+  return DIDA_OK;
+}
+
+int dida_obs_records_convert(int n_records, int nvar, int multi_level, const float* lon,
+                             const float* lat, const float* lev, const float* val,
+                             const float* noise, const int* var_index, double* site_lat,
+                             double* site_lon, int* site_row_ptr, int* row_var, double* row_ph,
+                             double* yo, double* rinv) {
+  int ns = 0, nr = 0;
+  const int rc = dida_obs_records_count(n_records, nvar, multi_level, lev, &ns, &nr);
+  if (rc < 0) return rc;
+  if (n_records > 0 && (!lon || !lat || !val || !var_index || !site_lat || !site_lon ||
+                        !site_row_ptr || !row_var || !row_ph || !yo || !rinv))
+    return dida::set_error(DIDA_ERR_ARG, "obs records: null argument");
+  const int w = 2 * nvar + 1;     // val_buf(2 nvar + 1, obs_num): value, oe per var, then ph
+  int s = -1;
+  for (int i = 0; i < n_records; ++i) {
+    if (starts_site(multi_level, multi_level ? lev[i] : 1.0f)) {
+      ++s;
+      site_lat[s] = lat[i];
+      site_lon[s] = lon[i];
+      site_row_ptr[s] = i * nvar;
+    }
+    const float* v = val + (size_t)i * w;
+    for (int n = 0; n < nvar; ++n) {
+      const int r = i * nvar + n;
+      // :400-405 value + randn * oe, and :430-442 rinv = 1 / oe**2, in real(4)
+      const float oe = v[2 * n + 1];
+      const float z = noise ? noise[(size_t)i * nvar + n] : 0.0f;
+      yo[r] = (double)(v[2 * n] + z * oe);
+      rinv[r] = (double)(1.0f / (oe * oe));
+      row_var[r] = var_index[n];
+      row_ph[r] = (double)v[2 * nvar];
+    }
+  }
+  if (ns > 0) site_row_ptr[ns] = n_records * nvar;
+  return DIDA_OK;
+}
+
+}  // extern "C"

@@ -111,6 +111,7 @@ EXPORTS = [
     "dida_letkf_abi_version", "dida_serve_create", "dida_serve_submit", "dida_serve_poll",
     "dida_serve_wait", "dida_serve_cancel", "dida_serve_destroy", "dida_serve_now",
     "dida_letkf_job_bytes", "dida_letkf_obs_operator_device", "dida_letkf_fill_halo_device",
+    "dida_obs_records_count", "dida_obs_records_convert",
 ]
 
 _lib = None
@@ -168,6 +169,10 @@ def lib(path: Optional[str] = None):
     L.dida_serve_cancel.argtypes = [_vp, C.c_longlong]
     L.dida_serve_destroy.argtypes = [_vp]
     L.dida_serve_now.restype = C.c_double
+    L.dida_obs_records_count.argtypes = [C.c_int, C.c_int, C.c_int, _fp, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+    L.dida_obs_records_convert.argtypes = [C.c_int, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp,
+                                           _ip, _dp, _dp, _ip, _ip, _dp, _dp, _dp]
     L.dida_letkf_job_bytes.argtypes = [C.POINTER(DidaConfig), C.c_int, C.c_int]
     L.dida_letkf_job_bytes.restype = C.c_size_t
     _lib = L

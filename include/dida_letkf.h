@@ -318,6 +318,32 @@ void letkf_core_(const int *nanals, const int *nobsl, const double *hxens_in,
                  const float *rdiaginv, const double *dep, const double *rloc, double *trans,
                  double *overline_w, const double *inflation_factor, const int *pflag);
 
+/* Observation records as read_obs receives them from its store
+ * (read_obs_mod.F90:338-449; RedisGetobserveF's buffers), host arrays:
+ * record i has lon[i], lat[i], lev[i] (real(4); lev is the 1-based level of a
+ * multi-level type) and val[(2*nvar+1)*i ...] = value_1, oe_1, ..., value_nvar,
+ * oe_nvar, ph.  multi_level != 0 (ADPUPA, GPSRO): a record with lev == 1 starts
+ * a site and the records after it are its other levels (:411-428); otherwise
+ * every record is a site of one level (:429-437).
+ * dida_obs_records_count gives the sites and rows (nvar per record);
+ * dida_obs_records_convert fills the inputs of dida_letkf_obs_operator_device
+ * and dida_letkf_set_obs[_device]: per site lat/lon and site_row_ptr
+ * (n_sites + 1), per row (level-major, the record's variables in order)
+ * row_var = var_index[n] (the DA variable of obs variable n, which read_obs
+ * matches by name), row_ph = ph, yo = value + noise * oe and
+ * rinv = 1 / oe**2, both in real(4) as the reference computes them.  noise
+ * (nvar per record, standard normal draws; NULL: none) replaces the
+ * reference's randn.  Both return DIDA_ERR_ARG for a profile record before
+ * any lev == 1 (which the reference would store at site 0) or a level that
+ * is not a positive whole number.  No device work. */
+int dida_obs_records_count(int n_records, int nvar, int multi_level, const float *lev,
+                           int *n_sites, int *n_rows);
+int dida_obs_records_convert(int n_records, int nvar, int multi_level, const float *lon,
+                             const float *lat, const float *lev, const float *val,
+                             const float *noise, const int *var_index, double *site_lat,
+                             double *site_lon, int *site_row_ptr, int *row_var, double *row_ph,
+                             double *yo, double *rinv);
+
 /* Observation operator and Yb/d centring of gen_obs_struct
  * (letkf_obs.F90:16-187: find_obs_grid :192-248, ps2ph_3d_interp and
  * obs_interpolation_3d_2 obs_make.F90:233-381, centring :158-170) on device

@@ -110,3 +110,69 @@ def test_fill_halo_matches_host_rule():
         torch.cuda.synchronize()
     assert np.array_equal(xd.cpu().numpy(), synth.fill_halo(x.copy(), 2))
     assert np.array_equal(ad.cpu().numpy(), synth.fill_halo(ax.copy(), 2))
+
+
+def test_obs_records_through_operator_and_analysis():
+    """Obs ingestion end to end: the committed record fixtures (profiles of
+    u, v, T and surface ps, read_obs_mod.F90:338-449) become rows, the
+    observation operator runs on the device and the analysis follows; both
+    agree with the oracle's operator and analysis on the same rows."""
+    import dataclasses
+    import os
+    import torch
+    import orc
+    from dida_letkf import synth, engine, obs_records
+    here = os.path.dirname(os.path.abspath(__file__))
+    parts = []
+    for name, var_index in (("profiles", [1, 2, 3]), ("surface", [0])):
+        rec = obs_records.read_records(os.path.join(here, "golden", f"obs_records_{name}.bin"))
+        parts.append(obs_records.to_rows(rec, var_index))
+    # one obs set: the surface sites after the profile sites
+    off = parts[0]["site_row_ptr"][-1]
+    rows = {key: np.concatenate([parts[0][key], parts[1][key]]) for key in
+            ("site_lat", "site_lon", "row_var", "row_ph", "yo", "rinv")}
+    rows["site_row_ptr"] = np.concatenate([parts[0]["site_row_ptr"],
+                                           parts[1]["site_row_ptr"][1:] + off]).astype(np.int32)
+    c = synth.obs_operator_case("c2", seed=11, k=24, nlat=18, nlon=36, nz=5, n_obs=10,
+                                n_stations=2, n_ps=2)
+    c = dataclasses.replace(c, site_lat=rows["site_lat"], site_lon=rows["site_lon"],
+                            site_row_ptr=rows["site_row_ptr"], row_var=rows["row_var"],
+                            row_ph=rows["row_ph"], yo=rows["yo"])
+    inst = c.inst
+    cfg = inst.cfg
+    cfg.local_dist = 2500.0
+    yb_o, dep_o = orc.obs_operator(c)
+    vco = np.log(c.row_ph)
+    inst.obs = synth.Obs(site_lat=c.site_lat, site_lon=c.site_lon, site_row_ptr=c.site_row_ptr,
+                         yb=yb_o, dep=dep_o, rinv=rows["rinv"], vcoord=vco, row_var=c.row_var)
+    ref = orc.letkf_compute(inst, want_trans=True)
+    assert ref["nobsl"].max() > cfg.k
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=None: torch.from_numpy(np.ascontiguousarray(a if dt is None else a.astype(dt))).to(dev)
+    xb, axb = t(inst.xb), t(inst.axb)
+    la, lo, rp = t(c.site_lat), t(c.site_lon), t(c.site_row_ptr, np.int32)
+    rv, rph, yo = t(c.row_var, np.int32), t(c.row_ph), t(c.yo)
+    rinv, vc = t(rows["rinv"]), t(vco)
+    nr = int(c.site_row_ptr[-1])
+    yb = torch.empty((nr, cfg.k), dtype=torch.float64, device=dev)
+    dep = torch.empty(nr, dtype=torch.float64, device=dev)
+    xa, axa = torch.zeros_like(xb), torch.zeros_like(axb)
+    with engine.LetkfEngine(cfg) as eng:
+        eng.set_grid(inst.grid_lat, inst.grid_lon)
+        eng.set_stream(torch.cuda.current_stream().cuda_stream)
+        eng.obs_operator_device(xb.data_ptr(), axb.data_ptr(), c.site_lat.size, la.data_ptr(),
+                                lo.data_ptr(), rp.data_ptr(), nr, rv.data_ptr(), rph.data_ptr(),
+                                yo.data_ptr(), yb.data_ptr(), dep.data_ptr())
+        torch.cuda.synchronize()
+        H_o = yb_o + (c.yo - dep_o)[:, None]
+        H_g = yb.cpu().numpy() + (c.yo - dep.cpu().numpy())[:, None]
+        assert _rel(H_g, H_o) < TOL
+        eng.set_obs_device(c.site_lat.size, la.data_ptr(), lo.data_ptr(), rp.data_ptr(), nr,
+                           yb.data_ptr(), dep.data_ptr(), rinv.data_ptr(), vc.data_ptr())
+        eng.analyze_device(xb.data_ptr(), axb.data_ptr(), xa.data_ptr(), axa.data_ptr())
+        torch.cuda.synchronize()
+    h = cfg.halo
+    g = (xa + axa[..., None]).cpu().numpy()[h:-h, h:-h]
+    r = (ref["xa"] + ref["axa"][..., None])[h:-h, h:-h]
+    print("records -> operator -> analysis", _rel(g, r))
+    assert _rel(g, r) < TOL

@@ -183,6 +183,8 @@ def serving_leg(torch, dev, args, precision):
 
 BUCKET_NAMES = ["0", "1-256", "257-512", "513-1024", "1025-2048", "2049-4096", "4097-8192",
                 ">8192"]
+VARIANT_NAMES = {1: "dispatch order, 1 row group/step", 2: "dispatch order, 2 row groups/step",
+                 3: "persistent queue, 1 row group/step", 4: "persistent queue, 2 row groups/step"}
 
 
 def products_buckets(st, nob, products_ms):
@@ -198,19 +200,19 @@ def products_buckets(st, nob, products_ms):
             continue
         out.append({"p": nm, "analyses": st["bucket_analyses"][b],
                     "rows_share": round(st["bucket_rows"][b] / rows, 4),
-                    "row_groups_per_step": st["variant"][b],
+                    "variant": VARIANT_NAMES.get(st["variant"][b], st["variant"][b]),
                     "worker_ms": round(st["bucket_wave_ms"][b], 3),
                     "time_share": round(st["bucket_wave_ms"][b] / wave, 4) if wave else None})
     nob = np.asarray(nob, np.int64) if nob is not None else None
-    return {"schedule": ("length buckets, persistent work queue per run of buckets "
+    return {"schedule": ("length buckets, longest first, one launch per run of buckets "
                          "sharing a variant" if st["schedule"] else "column order"),
             "p_mean": round(float(nob.mean()), 1) if nob is not None else None,
             "p_max": int(nob.max()) if nob is not None else None,
             "buckets": out,
             "load_balance": round(st["products_balance"], 4),
-            "load_balance_note": ("summed busy time of the products' queue workers over "
-                                  "workers x launch span (1: no worker idle while another "
-                                  "still works)"),
+            "load_balance_note": ("summed busy time of the products' wavefront groups over "
+                                  "the groups the device holds at once x launch span (1: no "
+                                  "slot idle from the first start to the last end)"),
             "products_ms": round(products_ms, 3)}
 
 

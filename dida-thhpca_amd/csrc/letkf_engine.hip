@@ -515,6 +515,8 @@ struct dida_letkf_handle {
   DevBuf sp_xb, sp_xb32, sp_axb, sp_nan;  // device split of the full state
   int variant[kBuckets] = {0};        // products variant per bucket position
   std::vector<bool> split_timed;      // per batch: split eigensolver events recorded
+  std::vector<long long> run_capacity;  // per batch and bucket position: the wavefront
+                                        // groups the device held for the launch starting there
   std::string arch;                   // device architecture name (tuning cache key)
   double clock_khz = 100000.0;        // device wall clock (in-kernel timing)
 };
@@ -895,7 +897,7 @@ struct StateArgs {
 // ---------------------------------------------------------------------------
 // Tuning cache: the products variant (32-row groups per step) of each
 // (device, k, precision, length bucket), measured on a live call and kept in
-// a text file, one "arch k precision bucket groups microseconds" line each.
+// a text file, one "arch k precision bucket variant microseconds" line each.
 
 namespace {
 std::mutex g_tune_mu;
@@ -920,10 +922,10 @@ void tune_load_locked() {
   if (path.empty()) return;
   std::ifstream f(path);
   std::string arch;
-  int k, prec, bucket, groups;
+  int k, prec, bucket, variant;
   double us;
-  while (f >> arch >> k >> prec >> bucket >> groups >> us)
-    if (groups == 1 || groups == 2) g_tune[tune_key(arch, k, prec, bucket)] = {groups, us};
+  while (f >> arch >> k >> prec >> bucket >> variant >> us)
+    if (variant >= 1 && variant <= 4) g_tune[tune_key(arch, k, prec, bucket)] = {variant, us};
 }
 
 // rewrites the whole file (entries of every device and shape seen)
@@ -943,18 +945,22 @@ int tune_save_locked() {
   return DIDA_OK;
 }
 
-// products variants worth measuring: row groups per step, BF16 at k <= 64
+// Products variants of a bucket: 1, 2 = dispatch order with 1 or 2 row
+// groups of 32 per step, 3, 4 = the same on a persistent work queue.  Two row
+// groups exist for BF16 at k <= 64.  Every variant gives the same bits.
 std::vector<int> products_variants(int kp, int precision) {
-  if (kp <= 64 && precision == DIDA_PREC_BF16X3) return {1, 2};
-  return {1};
+  if (kp <= 64 && precision == DIDA_PREC_BF16X3) return {1, 2, 3, 4};
+  return {1, 3};
 }
+int variant_groups(int v) { return v == 2 || v == 4 ? 2 : 1; }
+bool variant_persistent(int v) { return v >= 3; }
 }  // namespace
 
-// variant per bucket position from the cache (default 1 row group);
-// DIDA_PRODUCTS_GROUPS = 1 or 2 sets every bucket's (a test hook)
+// variant per bucket position from the cache (default 1: dispatch order,
+// one row group); DIDA_PRODUCTS_VARIANT = 1..4 sets every bucket's (a test hook)
 static void resolve_variants(dida_letkf_handle* h) {
   const std::vector<int> vars = products_variants(h->kp, h->cfg.precision);
-  const int forced = env_int("DIDA_PRODUCTS_GROUPS");
+  const int forced = env_int("DIDA_PRODUCTS_VARIANT");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   tune_load_locked();
   for (int j = 0; j < kBuckets; ++j) {
@@ -1017,7 +1023,7 @@ static int prepare_call(dida_letkf_handle* h) {
     RC_TRY(h->b_order.ensure(sizeof(int) * mb * h->nzc));
     RC_TRY(h->b_blk.ensure(sizeof(long long) * std::max(bucket_blk_elems((int)mb), 1LL)));
     RC_TRY(h->b_seg.ensure(sizeof(long long) * (kBuckets + 1)));
-    RC_TRY(h->b_q.ensure(sizeof(int) * kBuckets));
+    RC_TRY(h->b_q.ensure(sizeof(int) * kBuckets * kQueueParts));
     RC_TRY(h->b_stats.ensure(sizeof(long long) * nb * 2 * kBuckets));
     RC_TRY(h->b_ticks.ensure(sizeof(unsigned long long) * nb * 5 * kBuckets));
   }
@@ -1055,9 +1061,9 @@ static int tune_products(dida_letkf_handle* h, hipStream_t st, ProdParams pp) {
         if (seg[j + 1] <= seg[j]) continue;
         pp.seg_lo = j;
         pp.seg_hi = j + 1;
-        pp.q_counter = h->b_q.as<int>() + j;
-        pp.groups = g;
-        HIP_TRY(launch_zero_ints(st, pp.q_counter, 1));
+        pp.q_counter = variant_persistent(g) ? h->b_q.as<int>() + j * kQueueParts : nullptr;
+        pp.groups = variant_groups(g);
+        if (pp.q_counter) HIP_TRY(launch_zero_ints(st, pp.q_counter, kQueueParts));
         HIP_TRY(hipEventRecord(e0, st));
         HIP_TRY(launch_products_queue(st, pp, c.precision));
         HIP_TRY(hipEventRecord(e1, st));
@@ -1071,7 +1077,7 @@ static int tune_products(dida_letkf_handle* h, hipStream_t st, ProdParams pp) {
       }
     }
   }
-  HIP_TRY(launch_zero_ints(st, h->b_q.as<int>(), kBuckets));
+  HIP_TRY(launch_zero_ints(st, h->b_q.as<int>(), kBuckets * kQueueParts));
   std::lock_guard<std::mutex> lk(g_tune_mu);
   tune_load_locked();
   for (int j = 0; j < kBuckets; ++j) {
@@ -1244,17 +1250,21 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
       pp.seg_off = bp.seg_off;
       if (tune && bi == 0 && products_variants(h->kp, c.precision).size() > 1)
         RC_TRY(tune_products(h, st, pp));
-      // runs of bucket positions that share a variant: one persistent launch each
+      // runs of bucket positions that share a variant: one launch each
       for (int j0 = 0; j0 < kBuckets;) {
         int j1 = j0 + 1;
         while (j1 < kBuckets && h->variant[j1] == h->variant[j0]) ++j1;
+        const int v = h->variant[j0];
         pp.seg_lo = j0;
         pp.seg_hi = j1;
-        pp.q_counter = bp.q_counters + j0;
-        pp.groups = h->variant[j0];
+        pp.q_counter = variant_persistent(v) ? bp.q_counters + j0 * kQueueParts : nullptr;
+        pp.groups = variant_groups(v);
         pp.bucket_ticks = bp.ticks;
         pp.launch_ticks = bp.ticks ? bp.ticks + kBuckets + 4 * j0 : nullptr;
-        HIP_TRY(launch_products_queue(st, pp, c.precision));
+        long long cap = 0;
+        HIP_TRY(launch_products_queue(st, pp, c.precision, &cap));
+        if (timed && (size_t)(bi * kBuckets + j0) < h->run_capacity.size())
+          h->run_capacity[bi * kBuckets + j0] = cap;
         j0 = j1;
       }
     } else {
@@ -1444,9 +1454,10 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms, bool sp
         stats->bucket_wave_ms[kBuckets - 1 - j] += (double)t[j] / h->clock_khz;
       for (int j = 0; j < kBuckets; ++j) {
         const unsigned long long* l = t + kBuckets + 4 * j;
-        if (l[3] == 0 || l[2] == 0) continue;
+        const long long slots = h->run_capacity[b * kBuckets + j];
+        if (l[3] == 0 || l[2] == 0 || slots <= 0) continue;
         busy += (double)l[2];
-        cap += (double)l[3] * (double)(l[1] - l[0]);
+        cap += (double)slots * (double)(l[1] - l[0]);
       }
     }
     stats->products_balance = cap > 0 ? busy / cap : 0.0;
@@ -1539,6 +1550,7 @@ static int run_analysis(dida_letkf_handle* h, const StateArgs& sa, dida_letkf_st
                      "overlapped cubes update an FP64 state without the fused epilogue");
   RC_TRY(prepare_call(h));
   h->split_timed.assign(n_batches_of(h), false);
+  h->run_capacity.assign(n_batches_of(h) * kBuckets, 0);
   bool graph = h->opt_graph != 0;
   bool reran = false;
   for (int attempt = 0;; ++attempt) {

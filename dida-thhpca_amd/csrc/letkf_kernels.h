@@ -123,7 +123,8 @@ struct ProdParams {
   int* nobsl;                // tile mode: local obs rows per analysis (batch-local)
   // queue schedule: the launch's wavefront groups take analyses one at a
   // time from positions [seg_off[seg_lo], seg_off[seg_hi]) of `order` (the
-  // length buckets, longest first) through the counter q_counter
+  // length buckets, longest first), split in kQueueParts contiguous parts
+  // with a counter each (q_counter[0 .. kQueueParts))
   const long long* seg_off;
   int seg_lo, seg_hi;
   int* q_counter;
@@ -139,6 +140,8 @@ struct ProdParams {
 // 1: 1..256, 2: 257..512, ... 6: 4097..8192, 7: more.  The order list holds
 // the buckets longest first: bucket b at position kBuckets - 1 - b.
 constexpr int kBuckets = 8;
+constexpr int kQueueParts = 8;   // queue counters per bucket position (one per XCD)
+constexpr bool kQueueSplit = false;  // split a segment among the XCDs
 __host__ __device__ inline int bucket_of(long long p) {
   if (p <= 0) return 0;
   if (p <= 256) return 1;
@@ -159,7 +162,7 @@ struct BucketParams {
   long long* seg_off;        // kBuckets + 1 positions (analyses) in the order list
   long long* bucket_stats;   // kBuckets analyses, kBuckets rows
   int* order;                // n_cols * nzc batch-local analysis offsets
-  int* q_counters;           // zeroed by the scan kernel (kBuckets entries)
+  int* q_counters;           // zeroed by the scan kernel (kBuckets x kQueueParts)
   unsigned long long* ticks; // optional: kBuckets bucket sums, then 4 per bucket
                              // position for launch spans (reset by the scan kernel)
 };
@@ -338,9 +341,12 @@ hipError_t launch_list_counts(hipStream_t st, const long long* off, int n_list, 
 hipError_t launch_zero_ll(hipStream_t st, long long* p, long long n);
 hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill);
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision);
-// queue schedule: a persistent launch over the bucket positions
-// [p.seg_lo, p.seg_hi) (p.order, p.seg_off and p.q_counter set)
-hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision);
+// bucket schedule over the positions [p.seg_lo, p.seg_hi) of p.order: in
+// dispatch order, or on a persistent work queue when p.q_counter is set
+// (kQueueParts zeroed counters).  capacity: wavefront groups the device holds
+// at once for this launch
+hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision,
+                                 long long* capacity = nullptr);
 // length buckets of a batch's columns (BucketParams.blk: bucket_blk_elems)
 long long bucket_blk_elems(int n_cols);
 hipError_t launch_buckets(hipStream_t st, const BucketParams& p);

@@ -400,66 +400,126 @@ __device__ __forceinline__ long long analysis_at(const ProdParams& p, long long 
   return (b * B + rem % Bb) * L + rem / Bb;
 }
 
-// The analyses a products launch visits.  Plain schedule: each group of WPA
-// wavefronts takes the analysis at its logical position.  Queue schedule (a
-// persistent launch): the groups take positions of the order list from a
-// counter until the launch's bucket segment is used up, so a long row list
-// never leaves the rest of a workgroup idle while other groups wait to start.
-// WPA is 1 (a wavefront per analysis) or 4 (the whole 256-lane workgroup).
-// body(li) runs the products of batch-local analysis li; with timing on, the
-// wall-clock ticks of each analysis go to its bucket position, and each
-// worker adds its first start, last end and busy ticks, and itself, to the
-// launch's four totals.
-template <int WPA, class Body>
+// The analyses a products launch visits.  WPA is 1 (a wavefront per
+// analysis) or 4 (the whole 256-lane workgroup); body(li) runs the products
+// of batch-local analysis li.  SCHED (separate instantiations, so that each
+// schedule keeps its own register budget and timing costs nothing when off):
+//   0  plain: the analysis at the group's logical position (column order,
+//      or bands of longitudes);
+//   1  dispatch order over a bucket segment: the launch's groups map to the
+//      positions [seg_off[seg_lo], seg_off[seg_hi]) of the order list
+//      (length buckets, longest first, grid order within a bucket), dealt to
+//      the 8 XCDs in contiguous eighths, and the hardware dispatcher hands
+//      them to CUs as CUs free up -- the longest analyses start first;
+//   3  persistent work queue over a bucket segment: as many workgroups as
+//      fit at once, the groups take positions from a counter per XCD eighth
+//      (then help the other eighths) until the segment is used up;
+//   2, 4  as 1, 3 with timing: each analysis adds its wall-clock ticks to its
+//      bucket position, and the launch's four totals get the earliest start,
+//      the latest end, the summed busy ticks and the analyses (1, 2) or
+//      workers (3, 4) that ran.
+// Within a bucket segment the analyses an XCD runs are neighbours in the
+// grid and share their obs rows in that XCD's L2.
+template <int WPA, int SCHED, class Body>
 __device__ __forceinline__ void for_analyses(const ProdParams& p, int lane, Body body) {
   static_assert(WPA == 1 || WPA == 4, "a wavefront or a workgroup per analysis");
   const int wave = threadIdx.x >> 6;
-  if (!p.q_counter) {
-    const int apb = (blockDim.x >> 6) / WPA;
+  const int apb = (blockDim.x >> 6) / WPA;
+  if constexpr (SCHED == 0) {
     const long long li = (long long)xcd_block(p.n_blocks) * apb + wave / WPA;
     if (li >= p.n_an) return;
     body(analysis_at(p, li));
     return;
   }
-  const long long lo = p.seg_off[p.seg_lo], hi = p.seg_off[p.seg_hi];
-  const bool timing = p.bucket_ticks != nullptr;
+  constexpr bool timing = SCHED == 2 || SCHED == 4;
+  const bool leader = lane == 0 && (WPA == 1 || wave == 0);
+  const long long lo = p.seg_off[p.seg_lo], hi = p.seg_off[p.seg_hi], len = hi - lo;
+  auto bucket_of_pos = [&](long long pos) {
+    int j = p.seg_lo;
+    while (j + 1 < p.seg_hi && pos >= p.seg_off[j + 1]) ++j;
+    return j;
+  };
+  if constexpr (SCHED == 1 || SCHED == 2) {
+    // Workgroups are dealt round-robin to the 8 XCDs by index and dispatched
+    // in index order.  XCD x = blockIdx % 8 runs the x-th eighth of every
+    // bucket of the segment, bucket after bucket: all XCDs work on the same
+    // bucket at the same time (longest first, balanced), and each on a
+    // contiguous piece of it, grid neighbours whose obs rows its L2 holds.
+    const int x = blockIdx.x & 7;
+    long long q = (long long)(blockIdx.x >> 3) * apb + wave / WPA;   // index in x's share
+    long long pos = -1;
+    for (int j = p.seg_lo; j < p.seg_hi; ++j) {
+      const long long b0 = p.seg_off[j], nb = p.seg_off[j + 1] - b0;
+      const long long e0 = b0 + nb * x / 8, e1 = b0 + nb * (x + 1) / 8;
+      if (q < e1 - e0) {
+        pos = e0 + q;
+        break;
+      }
+      q -= e1 - e0;
+    }
+    if (pos < 0) return;
+    unsigned long long t0 = 0;
+    if constexpr (timing) t0 = wall_clock64();
+    body((long long)__builtin_amdgcn_readfirstlane(p.order[pos]));
+    if constexpr (timing) {
+      const unsigned long long t1 = wall_clock64();
+      if (leader) {
+        atomicAdd(&p.bucket_ticks[bucket_of_pos(pos)], t1 - t0);
+        atomicMin(&p.launch_ticks[0], t0);
+        atomicMax(&p.launch_ticks[1], t1);
+        atomicAdd(&p.launch_ticks[2], t1 - t0);
+        atomicAdd(&p.launch_ticks[3], 1ull);
+      }
+    }
+    return;
+  }
   unsigned long long busy = 0, t_first = ~0ull, t_last = 0;
   __shared__ int s_next;
+  // kQueueSplit: one counter per XCD eighth of the segment (each XCD works
+  // on its own region); otherwise one counter, all XCDs at the queue's front
+  int vx = kQueueSplit ? (blockIdx.x & 7) : 0, tried = kQueueSplit ? 0 : 7;
   for (;;) {
     int v;
     if constexpr (WPA == 1) {
       v = 0;
-      if (lane == 0) v = atomicAdd(p.q_counter, 1);
-      v = __shfl(v, 0, 64);
+      if (lane == 0) v = atomicAdd(p.q_counter + vx, 1);
+      v = __builtin_amdgcn_readfirstlane(__shfl(v, 0, 64));
     } else {
-      if (threadIdx.x == 0) s_next = atomicAdd(p.q_counter, 1);
+      if (threadIdx.x == 0) s_next = atomicAdd(p.q_counter + vx, 1);
       __syncthreads();
-      v = s_next;
+      v = __builtin_amdgcn_readfirstlane(s_next);
       __syncthreads();
     }
-    const long long pos = lo + v;
-    if (pos >= hi) break;
-    const unsigned long long t0 = timing ? wall_clock64() : 0;
-    body((long long)p.order[pos]);
-    if (timing) {
+    const long long c0 = kQueueSplit ? lo + len * vx / 8 : lo;
+    const long long c1 = kQueueSplit ? lo + len * (vx + 1) / 8 : hi;
+    const long long pos = c0 + v;
+    if (pos >= c1) {
+      if (++tried == 8) break;
+      vx = (vx + 1) & 7;
+      continue;
+    }
+    unsigned long long t0 = 0;
+    if constexpr (timing) t0 = wall_clock64();
+    // the analysis is uniform over the wavefront: keep it (and every address
+    // derived from it) in scalar registers
+    body((long long)__builtin_amdgcn_readfirstlane(p.order[pos]));
+    if constexpr (timing) {
       const unsigned long long t1 = wall_clock64();
       busy += t1 - t0;
       t_first = t0 < t_first ? t0 : t_first;
       t_last = t1;
-      if (lane == 0 && (WPA == 1 || wave == 0)) {
-        int j = p.seg_lo;
-        while (j + 1 < p.seg_hi && pos >= p.seg_off[j + 1]) ++j;
-        atomicAdd(&p.bucket_ticks[j], t1 - t0);
-      }
+      if (leader) atomicAdd(&p.bucket_ticks[bucket_of_pos(pos)], t1 - t0);
     }
   }
-  if (timing && lane == 0 && (WPA == 1 || wave == 0)) {
-    if (busy > 0) {
-      atomicMin(&p.launch_ticks[0], t_first);
-      atomicMax(&p.launch_ticks[1], t_last);
-      atomicAdd(&p.launch_ticks[2], busy);
+  if constexpr (timing) {
+    if (leader) {
+      if (busy > 0) {
+        atomicMin(&p.launch_ticks[0], t_first);
+        atomicMax(&p.launch_ticks[1], t_last);
+        atomicAdd(&p.launch_ticks[2], busy);
+      }
+      atomicAdd(&p.launch_ticks[3], 1ull);
     }
-    atomicAdd(&p.launch_ticks[3], 1ull);
   }
 }
 
@@ -575,7 +635,7 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
   }
 }
 
-template <int KP>
+template <int KP, int SCHED>
 __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -583,7 +643,7 @@ __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wsub = wave % WPA;
-  for_analyses<WPA>(p, lane, [&](long long li) {
+  for_analyses<WPA, SCHED>(p, lane, [&](long long li) {
     const long long a = p.a0 + li;
     const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
     // row lists past the list buffer's capacity (the call then reruns with a
@@ -722,7 +782,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
   }
 }
 
-template <int KP>
+template <int KP, int SCHED>
 __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -730,7 +790,7 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   constexpr int PPW = (NPAIR + WPA - 1) / WPA;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wsub = wave % WPA;
-  for_analyses<WPA>(p, lane, [&](long long li) {
+  for_analyses<WPA, SCHED>(p, lane, [&](long long li) {
     const long long a = p.a0 + li;
     const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
     // row lists past the list buffer's capacity (the call then reruns with a
@@ -1009,7 +1069,7 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
   }
 }
 
-template <int KP, int G>
+template <int KP, int G, int SCHED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
@@ -1020,7 +1080,7 @@ void k_products_bf16(ProdParams p) {
   const int wsub = wave % WPA;
   if constexpr (WPA == 4) static_assert(KP % 64 == 0, "four wavefronts share an analysis");
   __shared__ bf8 sh[WPA == 1 ? 1 : 2 * 3 * (KP / 16) * 64];
-  for_analyses<WPA>(p, lane, [&](long long li) {
+  for_analyses<WPA, SCHED>(p, lane, [&](long long li) {
     const long long a = p.a0 + li;
     const int ah = (int)(a / p.nzc), kc = (int)(a % p.nzc);
     // row lists past the list buffer's capacity (the call then reruns with a
@@ -1556,7 +1616,7 @@ __global__ __launch_bounds__(64) void k_bucket_scan(BucketParams p, int n_blk) {
     s_tot[b] = t;
     p.bucket_stats[b] = t * p.nzc;
     p.bucket_stats[kBuckets + b] = r * p.nzc;
-    p.q_counters[b] = 0;
+    for (int x = 0; x < kQueueParts; ++x) p.q_counters[b * kQueueParts + x] = 0;
     if (p.ticks) {
       p.ticks[b] = 0;
       p.ticks[kBuckets + 4 * b] = ~0ull;
@@ -1847,46 +1907,81 @@ static int resident_blocks(const void* f, std::atomic<int>& cache) {
 
 template <class K>
 static hipError_t launch_prod(hipStream_t st, const ProdParams& p, K* kern, long long blocks,
-                              std::atomic<int>& cache) {
+                              std::atomic<int>& cache, int apb, long long* capacity) {
   ProdParams q = p;
   unsigned grid;
-  if (q.q_counter) {
-    const int r = resident_blocks(reinterpret_cast<const void*>(kern), cache);
-    if (r <= 0) return hipErrorInvalidDeviceFunction;
+  const int r = (q.order || capacity) ? resident_blocks(reinterpret_cast<const void*>(kern), cache)
+                                      : 1;
+  if (r <= 0) return hipErrorInvalidDeviceFunction;
+  if (q.q_counter && q.order) {
     grid = (unsigned)std::min<long long>(r, blocks);
+    q.n_blocks = (int)grid;
+  } else if (q.order) {
+    // an XCD's share is an eighth of each bucket, rounded: at most
+    // n / 8 + kBuckets analyses
+    const long long per_xcd = ((q.n_an + 7) / 8 + kBuckets + apb - 1) / apb;
+    grid = (unsigned)(8 * per_xcd);
     q.n_blocks = (int)grid;
   } else {
     q.n_blocks = (int)blocks;
     grid = (unsigned)(((blocks + 7) / 8) * 8);
   }
+  // wavefront groups the device holds at once (the load balance's capacity)
+  if (capacity) *capacity = (long long)std::min<long long>(r, grid) * apb;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, q);
   return hipGetLastError();
 }
 
+template <int KP, int SCHED>
+static hipError_t launch_products_sched(hipStream_t st, const ProdParams& p, int precision,
+                                        long long blocks, long long* capacity) {
+  constexpr int T = KP / 16;
+  constexpr int NPAIR = T * (T + 1) / 2;
+  constexpr int WPA = (NPAIR + 9) / 10;
+  constexpr int APB = 4 / WPA;
+  static std::atomic<int> c64{0}, c32{0}, c16a{0}, c16b{0};
+  if (precision == 0)
+    return launch_prod(st, p, k_products_f64<KP, SCHED>, blocks, c64, APB, capacity);
+  if (precision == 1)
+    return launch_prod(st, p, k_products_f32<KP, SCHED>, blocks, c32, APB, capacity);
+  if (WPA == 1 && p.groups == 2)
+    return launch_prod(st, p, k_products_bf16<KP, 2, SCHED>, blocks, c16b, APB, capacity);
+  return launch_prod(st, p, k_products_bf16<KP, 1, SCHED>, blocks, c16a, APB, capacity);
+}
+
 template <int KP>
-static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int precision) {
+static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int precision,
+                                     long long* capacity) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
   constexpr int WPA = (NPAIR + 9) / 10;
   constexpr int APB = 4 / WPA;
   const long long blocks = (p.n_an + APB - 1) / APB;
   if (blocks <= 0) return hipSuccess;
-  static std::atomic<int> c64{0}, c32{0}, c16a{0}, c16b{0};
-  if (precision == 0) return launch_prod(st, p, k_products_f64<KP>, blocks, c64);
-  if (precision == 1) return launch_prod(st, p, k_products_f32<KP>, blocks, c32);
-  if (WPA == 1 && p.groups == 2) return launch_prod(st, p, k_products_bf16<KP, 2>, blocks, c16b);
-  return launch_prod(st, p, k_products_bf16<KP, 1>, blocks, c16a);
+  // order set: a bucket segment, in dispatch order or (q_counter set) on a
+  // persistent queue; the ticks select the timed instantiation
+  const bool timed = p.bucket_ticks != nullptr;
+  if (!p.order) return launch_products_sched<KP, 0>(st, p, precision, blocks, capacity);
+  if (!p.q_counter)
+    return timed ? launch_products_sched<KP, 2>(st, p, precision, blocks, capacity)
+                 : launch_products_sched<KP, 1>(st, p, precision, blocks, capacity);
+  return timed ? launch_products_sched<KP, 4>(st, p, precision, blocks, capacity)
+               : launch_products_sched<KP, 3>(st, p, precision, blocks, capacity);
 }
 
-hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision) {
-  if (!p.q_counter || !p.order || !p.seg_off || p.seg_lo >= p.seg_hi) return hipErrorInvalidValue;
-  return launch_products(st, p, precision);
+hipError_t launch_products_queue(hipStream_t st, const ProdParams& p, int precision,
+                                 long long* capacity) {
+  if (!p.order || !p.seg_off || p.seg_lo >= p.seg_hi) return hipErrorInvalidValue;
+  if (p.kp == 32) return launch_products_kp<32>(st, p, precision, capacity);
+  if (p.kp == 64) return launch_products_kp<64>(st, p, precision, capacity);
+  if (p.kp == 128) return launch_products_kp<128>(st, p, precision, capacity);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_products(hipStream_t st, const ProdParams& p, int precision) {
-  if (p.kp == 32) return launch_products_kp<32>(st, p, precision);
-  if (p.kp == 64) return launch_products_kp<64>(st, p, precision);
-  if (p.kp == 128) return launch_products_kp<128>(st, p, precision);
+  if (p.kp == 32) return launch_products_kp<32>(st, p, precision, nullptr);
+  if (p.kp == 64) return launch_products_kp<64>(st, p, precision, nullptr);
+  if (p.kp == 128) return launch_products_kp<128>(st, p, precision, nullptr);
   return hipErrorInvalidValue;
 }
 

@@ -20,7 +20,7 @@ def test_line_has_contract_keys():
     bst = dict(schedule=1, bucket_analyses=[0, 10, 20, 30, 0, 0, 0, 0],
                bucket_rows=[0, 1000, 6000, 21000, 0, 0, 0, 0],
                bucket_wave_ms=[0.0, 1.0, 3.0, 6.0, 0.0, 0.0, 0.0, 0.0],
-               variant=[1, 1, 2, 2, 1, 1, 1, 1], products_balance=0.9)
+               variant=[1, 1, 2, 3, 1, 1, 1, 1], products_balance=0.9)
     res = {2: dict(value=1.18e6, ms_step=54.9, kernel_ms=km, stats=st, bucket_stats=bst,
                    nobsl=[100, 300, 700]),
            0: dict(value=1.0e6, ms_step=64.8, kernel_ms=dict(km, products=19.9), stats=st)}

@@ -1,9 +1,10 @@
 """GPU tests of the products schedule and the capture-ready call, through the
 C ABI:
 
-* length buckets + persistent work queue against the column-order launch,
-  bit for bit (FP64, FP32, BF16 at k <= 64 with one and two row groups per
-  step, BF16 at k = 128, the vertical loop, several batches);
+* length buckets (in dispatch order, or on persistent work queues) against
+  the column-order launch, bit for bit (FP64, FP32, BF16 at k <= 64 with one
+  and two row groups per step, BF16 at k = 128, the vertical loop, several
+  batches);
 * the bucket statistics against the local obs counts (same edges as
   ``engine.BUCKET_EDGES``) and the in-kernel timing's sanity;
 * a row-list capacity that is too small: the call runs again with the
@@ -70,7 +71,7 @@ class _Env:
                 os.environ[k] = v
 
 
-def _run(inst, precision=None, schedule=1, groups=None, f32=False, row_cap=0, max_batch=0,
+def _run(inst, precision=None, schedule=1, variant=None, f32=False, row_cap=0, max_batch=0,
          timing=0, keep=False):
     """One analysis on a fresh handle; returns (xa, axa, stats dict, nobsl)."""
     import copy
@@ -80,7 +81,7 @@ def _run(inst, precision=None, schedule=1, groups=None, f32=False, row_cap=0, ma
     if precision is not None:
         cfg = copy.copy(cfg)
         cfg.precision = precision
-    env = {} if groups is None else {"DIDA_PRODUCTS_GROUPS": groups}
+    env = {} if variant is None else {"DIDA_PRODUCTS_VARIANT": variant}
     with _Env(**env):
         eng = engine.LetkfEngine(cfg, keep_transforms=keep, max_batch_columns=max_batch)
     with eng:
@@ -103,22 +104,26 @@ def _run(inst, precision=None, schedule=1, groups=None, f32=False, row_cap=0, ma
         return xa.cpu().numpy(), axa.cpu().numpy(), eng.stats.as_dict(), nob
 
 
-CASES = [("c2_k64", 2, None, False), ("c2_k64", 2, 2, False), ("c2_k64", 0, None, False),
-         ("c2_k64", 1, None, False), ("c2_k64", 2, 2, True), ("c1_k32_vertical", 2, 2, False),
-         ("c1_k32_vertical", 0, None, False), ("c3_k128", 2, None, False),
-         ("c3_k128", 0, None, False)]
+# variants: 1, 2 = dispatch order with 1 or 2 row groups per step, 3, 4 = the
+# same on persistent work queues (two row groups: BF16 at k <= 64 only)
+CASES = [("c2_k64", 2, None, False), ("c2_k64", 2, 2, False), ("c2_k64", 2, 3, False),
+         ("c2_k64", 2, 4, False), ("c2_k64", 0, None, False), ("c2_k64", 0, 3, False),
+         ("c2_k64", 1, 3, False), ("c2_k64", 2, 2, True), ("c1_k32_vertical", 2, 2, False),
+         ("c1_k32_vertical", 0, 3, False), ("c3_k128", 2, None, False), ("c3_k128", 2, 3, False),
+         ("c3_k128", 0, 3, False)]
 
 
-@pytest.mark.parametrize("name,prec,groups,f32", CASES)
-def test_queue_equals_column_order(name, prec, groups, f32):
+@pytest.mark.parametrize("name,prec,variant,f32", CASES)
+def test_queue_equals_column_order(name, prec, variant, f32):
     inst = _instances()[name]
-    print(name, "precision", prec, "groups", groups, "f32 state", f32, flush=True)
+    print(name, "precision", prec, "variant", variant, "f32 state", f32, flush=True)
     xa0, axa0, s0, _ = _run(inst, prec, schedule=0, f32=f32)
-    xa1, axa1, s1, _ = _run(inst, prec, schedule=1, groups=groups, f32=f32)
+    xa1, axa1, s1, _ = _run(inst, prec, schedule=1, variant=variant, f32=f32)
     assert s0["schedule"] == 0 and s1["schedule"] == 1
-    if groups:
+    if variant:
+        valid = (1, 2, 3, 4) if inst.cfg.k <= 64 and prec == 2 else (1, 3)
         used = [v for v, n in zip(s1["variant"], s1["bucket_analyses"]) if n]
-        assert set(used) == ({groups} if inst.cfg.k <= 64 and prec == 2 else {1})
+        assert set(used) == ({variant} if variant in valid else {1})
     assert np.array_equal(xa0, xa1) and np.array_equal(axa0, axa1)
     assert s0["total_rows"] == s1["total_rows"]
 
@@ -128,6 +133,9 @@ def test_queue_several_batches_and_bucket_stats():
     inst = _instances()["c2_k64"]
     xa0, axa0, s0, nob = _run(inst, 2, schedule=0, keep=True)
     xa1, axa1, s1, nob1 = _run(inst, 2, schedule=1, max_batch=200, timing=1, keep=True)
+    xa2, axa2, s2, _ = _run(inst, 2, schedule=1, max_batch=200, timing=1, variant=3)
+    assert np.array_equal(xa0, xa2) and np.array_equal(axa0, axa2)
+    assert 0.0 < s2["products_balance"] <= 1.0 + 1e-9
     assert np.array_equal(xa0, xa1) and np.array_equal(axa0, axa1)
     assert np.array_equal(nob, nob1)
     edges = np.array(engine.BUCKET_EDGES)
@@ -263,6 +271,6 @@ def test_tuner_writes_cache_and_keeps_results(tmp_path):
     nonempty = [b for b in range(8) if st["bucket_analyses"][b]]
     got = {int(r[3]): int(r[4]) for r in rows if int(r[1]) == inst.cfg.k and int(r[2]) == 2}
     assert set(nonempty) <= set(got)
-    assert all(g in (1, 2) for g in got.values())
+    assert all(g in (1, 2, 3, 4) for g in got.values())
     for b in nonempty:
         assert st["variant"][b] == got[b]

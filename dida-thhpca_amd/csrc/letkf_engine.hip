@@ -157,9 +157,10 @@ __global__ void k_f64_to_f32(long long n, const double* __restrict__ in, float* 
   if (i < n) out[i] = (float)in[i];
 }
 
-__global__ void k_iota(long long n, int* __restrict__ out) {
+// out[i] = i for i < n, then 0 for the list padding up to n + pad
+__global__ void k_iota(long long n, long long pad, int* __restrict__ out) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (int)i;
+  if (i < n + pad) out[i] = i < n ? (int)i : 0;
 }
 
 int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
@@ -792,7 +793,11 @@ static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
                        (long long)n_rows, rinv, h->rinv32.as<float>());
   if (c.precision != DIDA_PREC_FP64) {
     const long long ny = (long long)n_rows * c.k;
-    RC_TRY(h->yb32.ensure(sizeof(float) * std::max(ny, 1LL)));
+    // the reduced-precision products address the FP32 Yb with 32-bit offsets
+    if (ny + (long long)kListPad * c.k >= (1LL << 32))
+      return set_error(DIDA_ERR_UNSUPPORTED, "%d obs rows x k=%d exceed the 2^32 values the "
+                       "reduced-precision products address", n_rows, c.k);
+    RC_TRY(h->yb32.ensure(sizeof(float) * (std::max(ny, 1LL) + (long long)kListPad * c.k)));
     if (ny > 0)
       hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, st, ny,
                          yb, h->yb32.as<float>());
@@ -1003,12 +1008,19 @@ static int prepare_call(dida_letkf_handle* h) {
   RC_TRY(h->status.ensure(sizeof(int) * h->n_an));
   RC_TRY(h->nobsl_dev.ensure(sizeof(int) * std::max(h->n_h, 1LL)));
   RC_TRY(h->batch_tot.ensure(sizeof(long long) * nb));
-  RC_TRY(h->row_idx.ensure(sizeof(int) * rc));
+  {
+    // the lists' padding holds valid row indices: a new buffer is zeroed
+    // (once, before any call's launches)
+    const void* before = h->row_idx.p;
+    RC_TRY(h->row_idx.ensure(sizeof(int) * (rc + kListPad)));
+    if (h->row_idx.p != before)
+      HIP_TRY(launch_zero_ints(h->stream, h->row_idx.as<int>(), (int)(rc + kListPad)));
+  }
   if (h->tiled) {
     RC_TRY(h->row_site.ensure(sizeof(int) * rc));
   } else {
-    RC_TRY(h->row_gc.ensure(sizeof(double) * rc));
-    if (h->nzc == 1) RC_TRY(h->row_wd.ensure(sizeof(double) * rc));
+    RC_TRY(h->row_gc.ensure(sizeof(double) * (rc + kListPad)));
+    if (h->nzc == 1) RC_TRY(h->row_wd.ensure(sizeof(double) * (rc + kListPad)));
   }
   if (c.keep_transforms || c.overlap > 0)
     RC_TRY(h->trans.ensure(sizeof(double) * h->n_an * c.k * c.k));
@@ -1773,15 +1785,15 @@ int core_layout(CoreBufs& cb, EigScratch& es, int k, long long n_an, long long n
   add(cb.y, 8 * nrr * k);
   add(cb.r, 4 * nrr);
   add(cb.d, 8 * nrr);
-  add(cb.g, 8 * nrr);
-  add(cb.idx, 4 * nrr);
+  add(cb.g, 8 * (nrr + kListPad));
+  add(cb.idx, 4 * (nrr + kListPad));
   add(cb.off, 8 * (n + 1));
   add(cb.A, 8 * n * kp * kp);
   add(cb.b, 8 * n * kp);
   add(cb.t, 8 * n * k * k);
   add(cb.w, 8 * n * k);
   add(cb.s, 4 * n);
-  if (precision != DIDA_PREC_FP64) add(cb.y32, 4 * nrr * k);
+  if (precision != DIDA_PREC_FP64) add(cb.y32, 4 * (nrr + kListPad) * k);
   add(es.refl, 8 * n * kp * kp);
   add(es.tau, 8 * n * kp);
   add(es.d, 8 * n * kp);
@@ -1838,8 +1850,8 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
     HIP_TRY(hipMemcpyAsync(cb.r.p, rdiaginv, sizeof(float) * nr, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(cb.d.p, dep, sizeof(double) * nr, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(cb.g.p, rloc, sizeof(double) * nr, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_iota, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, nr,
-                       cb.idx.as<int>());
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((nr + kListPad + 255) / 256)), dim3(256), 0, st,
+                       nr, (long long)kListPad, cb.idx.as<int>());
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipMemcpyAsync(cb.off.p, row_ptr, sizeof(long long) * (n_an + 1), hipMemcpyHostToDevice, st));

@@ -139,6 +139,11 @@ struct ProdParams {
 // Length buckets over the local obs count p of a column: 0: p = 0,
 // 1: 1..256, 2: 257..512, ... 6: 4097..8192, 7: more.  The order list holds
 // the buckets longest first: bucket b at position kBuckets - 1 - b.
+// Row lists and FP32 Yb are allocated kListPad entries (Yb: kListPad rows'
+// worth of values) past their end, and the list tail holds valid row
+// indices (0), so the BF16 products may read a group's positions past the end
+// of an analysis' list (with weight 0) without clamping each index.
+constexpr int kListPad = 128;
 constexpr int kBuckets = 8;
 constexpr int kQueueParts = 8;   // queue counters per bucket position (one per XCD)
 constexpr bool kQueueSplit = false;  // split a segment among the XCDs

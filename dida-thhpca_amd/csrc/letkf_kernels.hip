@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <type_traits>
 #include "letkf_device.h"
 
@@ -539,7 +540,7 @@ struct PairTable {
   __device__ static constexpr int J(int pid) { return pid - I(pid) * (I(pid) + 1) / 2; }
 };
 
-template <int KP, int WS, int PPW>
+template <int KP, int WS, int PPW, bool PW>
 __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long a, int ah,
                                                   double vc, long long r0, long long r1,
                                                   int lane) {
@@ -566,7 +567,7 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
     const int rc = rl < n ? rl : n - 1;
     const int row = p.row_idx[r0 + rc];
     double wv, wdv;
-    if (p.pre_weighted) {
+    if constexpr (PW) {
       wv = p.row_gc[r0 + rc];
       wdv = p.row_wd[r0 + rc];
     } else {
@@ -580,7 +581,9 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       const int m = 16 * t + i16;
-      y[t] = (m < p.k) ? yr[m < p.k ? m : 0] : 0.0;
+      // an unconditional load of an in-row member and a select: no branch
+      const double yl = yr[m < p.k ? m : p.k - 1];
+      y[t] = (m < p.k) ? yl : 0.0;
     }
   };
   double yn[G][T], wn[G], wdn[G];
@@ -635,7 +638,7 @@ __device__ __forceinline__ void products_f64_wave(const ProdParams& p, long long
   }
 }
 
-template <int KP, int SCHED>
+template <int KP, int SCHED, bool PW>
 __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -651,16 +654,16 @@ __global__ __launch_bounds__(256) void k_products_f64(ProdParams p) {
     const long long r0 = min(p.row_off[ah - p.h0], p.row_cap);
     const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
     const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-    if (wsub == 0) products_f64_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 0) products_f64_wave<KP, 0, PPW, PW>(p, a, ah, vc, r0, r1, lane);
     if constexpr (WPA > 1) {
-      if (wsub == 1) products_f64_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 2) products_f64_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 3) products_f64_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 1) products_f64_wave<KP, 1, PPW, PW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 2) products_f64_wave<KP, 2, PPW, PW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 3) products_f64_wave<KP, 3, PPW, PW>(p, a, ah, vc, r0, r1, lane);
     }
   });
 }
 
-template <int KP, int WS, int PPW>
+template <int KP, int WS, int PPW, bool PW>
 __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long a, int ah,
                                                   double vc, long long r0, long long r1,
                                                   int lane) {
@@ -683,7 +686,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
     const int rc = rl < n ? rl : n - 1;
     const int row = p.row_idx[r0 + rc];
     double w, wd;
-    if (p.pre_weighted) {
+    if constexpr (PW) {
       w = p.row_gc[r0 + rc];
       wd = p.row_wd[r0 + rc];
     } else {
@@ -697,7 +700,8 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       const int m = 16 * t + i16;
-      y[t] = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
+      const float yl = yr[m < p.k ? m : p.k - 1];
+      y[t] = (m < p.k) ? yl : 0.f;
     }
   };
   // G groups of 4 rows per step (lane q takes rows 4g + q), loaded one step
@@ -782,7 +786,7 @@ __device__ __forceinline__ void products_f32_wave(const ProdParams& p, long long
   }
 }
 
-template <int KP, int SCHED>
+template <int KP, int SCHED, bool PW>
 __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -798,11 +802,11 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
     const long long r0 = min(p.row_off[ah - p.h0], p.row_cap);
     const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
     const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
-    if (wsub == 0) products_f32_wave<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane);
+    if (wsub == 0) products_f32_wave<KP, 0, PPW, PW>(p, a, ah, vc, r0, r1, lane);
     if constexpr (WPA > 1) {
-      if (wsub == 1) products_f32_wave<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 2) products_f32_wave<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane);
-      if (wsub == 3) products_f32_wave<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 1) products_f32_wave<KP, 1, PPW, PW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 2) products_f32_wave<KP, 2, PPW, PW>(p, a, ah, vc, r0, r1, lane);
+      if (wsub == 3) products_f32_wave<KP, 3, PPW, PW>(p, a, ah, vc, r0, r1, lane);
     }
   });
 }
@@ -816,13 +820,16 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
 // One 32-row group position of an analysis' row list (rl, with n rows):
 // sqrt(w) and w d in FP32, zero past the end of the list (whose last row is
 // read in its place, so every load stays in bounds without a branch).
+template <bool PW>
 __device__ __forceinline__ void group_row(const ProdParams& p, long long r0, int n, int rl,
                                           int row, double vc, float& s, float& wdf) {
   const int rc = rl < n ? rl : n - 1;
   double w, wd;
-  if (p.pre_weighted) {
-    w = p.row_gc[r0 + rc];
-    wd = p.row_wd[r0 + rc];
+  if constexpr (PW) {
+    // the lists are padded (kListPad): read in place, the select below
+    // drops positions past the end
+    w = p.row_gc[r0 + rl];
+    wd = p.row_wd[r0 + rl];
   } else {
     w = row_weight(p, row, p.row_gc[r0 + rc], vc);
     wd = w * p.dep[row];
@@ -833,7 +840,7 @@ __device__ __forceinline__ void group_row(const ProdParams& p, long long r0, int
   wdf = in ? (float)wd : 0.f;
 }
 
-template <int KP, int WS, int PPW, int G = 1>
+template <int KP, int WS, int PPW, int G, bool PW>
 __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long long a, int ah,
                                                    double vc, long long r0, long long r1,
                                                    int lane) {
@@ -856,13 +863,15 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   // bits for every G).  The row indices of the next step are loaded one step
   // ahead, so only the Yb gathers wait on memory; positions past the end
   // read the list's last entry and enter with weight 0.
+  // The lists are padded past their end with valid row indices (kListPad),
+  // so no index is clamped; Yb rows are read as 32-bit offsets from the
+  // array with the member offsets folded into the loads.
   int idx_n[G][8];
-  if (n > 0) {
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg)
+  for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-      for (int u = 0; u < 8; ++u) idx_n[gg][u] = ridx[min(32 * gg + 8 * q + u, n - 1)];
-  }
+    for (int u = 0; u < 8; ++u) idx_n[gg][u] = ridx[32 * gg + 8 * q + u];
+  const unsigned kk = (unsigned)p.k;
   for (int g0 = 0; g0 < n; g0 += 32 * G) {
     bf8 hi[G][T], lo[G][T];
     int idx[G][8];
@@ -874,18 +883,21 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
     for (int gg = 0; gg < G; ++gg)
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        idx_n[gg][u] = ridx[min(g0 + 32 * G + 32 * gg + 8 * q + u, n - 1)];
+        idx_n[gg][u] = ridx[g0 + 32 * G + 32 * gg + 8 * q + u];
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         float s, wdf;
-        group_row(p, r0, n, g0 + 32 * gg + 8 * q + u, idx[gg][u], vc, s, wdf);
-        const float* yr = p.yb32 + (long long)idx[gg][u] * p.k;
+        group_row<PW>(p, r0, n, g0 + 32 * gg + 8 * q + u, idx[gg][u], vc, s, wdf);
+        const unsigned base = (unsigned)idx[gg][u] * kk + (unsigned)i16;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
           const int m = 16 * t + i16;
-          const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
+          // members past k read the next row's values (or the padding):
+          // a select drops them, no branch
+          const float yl = p.yb32[base + 16u * t];
+          const float yv = (m < p.k) ? yl : 0.f;
           const float yh = s * yv;
           const __bf16 h = (__bf16)yh;
           hi[gg][t][u] = h;
@@ -934,7 +946,7 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
 // double-buffered LDS stage with one barrier per group, and each wavefront
 // then runs the MFMAs of its tile pairs from LDS.  Same values and
 // accumulation order as products_bf16_wave.
-template <int KP, int WS, int PPW>
+template <int KP, int WS, int PPW, bool PW>
 __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long long a, int ah,
                                                      double vc, long long r0, long long r1,
                                                      int lane, bf8* __restrict__ sh) {
@@ -995,12 +1007,13 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       float s, wdf;
-      group_row(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
+      group_row<PW>(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
       const float* yr = p.yb32 + (long long)idx[u] * p.k;
 #pragma unroll
       for (int tt = 0; tt < TW; ++tt) {
         const int m = 16 * (WS * TW + tt) + i16;
-        const float yv = (m < p.k) ? yr[m < p.k ? m : 0] : 0.f;
+        const float yl = yr[m < p.k ? m : p.k - 1];     // a select, not a branch
+        const float yv = (m < p.k) ? yl : 0.f;
         // yh = hi + mid + lo: 24 bits in three BF16 terms (each difference
         // is exact in FP32)
         const float yh = s * yv;
@@ -1069,8 +1082,10 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
   }
 }
 
-template <int KP, int G, int SCHED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// Three wavefronts per SIMD, and no more: given room for a fourth, the
+// compiler serialises the row gathers to fit its registers.
+template <int KP, int G, int SCHED, bool PW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
@@ -1089,12 +1104,12 @@ void k_products_bf16(ProdParams p) {
     const long long r1 = min(p.row_off[ah - p.h0 + 1], p.row_cap);
     const double vc = p.vertical ? chunk_vcoord(p, ah, kc) : 0.0;
     if constexpr (WPA == 1) {
-      products_bf16_wave<KP, 0, PPW, G>(p, a, ah, vc, r0, r1, lane);
+      products_bf16_wave<KP, 0, PPW, G, PW>(p, a, ah, vc, r0, r1, lane);
     } else {
-      if (wsub == 0) products_bf16_shared<KP, 0, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-      if (wsub == 1) products_bf16_shared<KP, 1, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-      if (wsub == 2) products_bf16_shared<KP, 2, PPW>(p, a, ah, vc, r0, r1, lane, sh);
-      if (wsub == 3) products_bf16_shared<KP, 3, PPW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 0) products_bf16_shared<KP, 0, PPW, PW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 1) products_bf16_shared<KP, 1, PPW, PW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 2) products_bf16_shared<KP, 2, PPW, PW>(p, a, ah, vc, r0, r1, lane, sh);
+      if (wsub == 3) products_bf16_shared<KP, 3, PPW, PW>(p, a, ah, vc, r0, r1, lane, sh);
       // the next analysis' first stage must not overwrite operands another
       // wavefront still reads
       __syncthreads();
@@ -1928,7 +1943,11 @@ static hipError_t launch_prod(hipStream_t st, const ProdParams& p, K* kern, long
   }
   // wavefront groups the device holds at once (the load balance's capacity)
   if (capacity) *capacity = (long long)std::min<long long>(r, grid) * apb;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, q);
+  static const int lds = [] {
+    const char* e = std::getenv("DIDA_PROD_LDS");
+    return e ? std::atoi(e) : 0;
+  }();
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, st, q);
   return hipGetLastError();
 }
 
@@ -1939,14 +1958,26 @@ static hipError_t launch_products_sched(hipStream_t st, const ProdParams& p, int
   constexpr int NPAIR = T * (T + 1) / 2;
   constexpr int WPA = (NPAIR + 9) / 10;
   constexpr int APB = 4 / WPA;
-  static std::atomic<int> c64{0}, c32{0}, c16a{0}, c16b{0};
+  // PW: the row lists carry the weights w and w d (horizontal loop), so the
+  // products read them instead of forming them per row; straight-line code
+  // lets a group's loads be in flight together
+  static std::atomic<int> c64[2], c32[2], c16a[2], c16b[2];
+  const int w = p.pre_weighted ? 1 : 0;
   if (precision == 0)
-    return launch_prod(st, p, k_products_f64<KP, SCHED>, blocks, c64, APB, capacity);
+    return w ? launch_prod(st, p, k_products_f64<KP, SCHED, true>, blocks, c64[1], APB, capacity)
+             : launch_prod(st, p, k_products_f64<KP, SCHED, false>, blocks, c64[0], APB, capacity);
   if (precision == 1)
-    return launch_prod(st, p, k_products_f32<KP, SCHED>, blocks, c32, APB, capacity);
+    return w ? launch_prod(st, p, k_products_f32<KP, SCHED, true>, blocks, c32[1], APB, capacity)
+             : launch_prod(st, p, k_products_f32<KP, SCHED, false>, blocks, c32[0], APB, capacity);
   if (WPA == 1 && p.groups == 2)
-    return launch_prod(st, p, k_products_bf16<KP, 2, SCHED>, blocks, c16b, APB, capacity);
-  return launch_prod(st, p, k_products_bf16<KP, 1, SCHED>, blocks, c16a, APB, capacity);
+    return w ? launch_prod(st, p, k_products_bf16<KP, 2, SCHED, true>, blocks, c16b[1], APB,
+                           capacity)
+             : launch_prod(st, p, k_products_bf16<KP, 2, SCHED, false>, blocks, c16b[0], APB,
+                           capacity);
+  return w ? launch_prod(st, p, k_products_bf16<KP, 1, SCHED, true>, blocks, c16a[1], APB,
+                         capacity)
+           : launch_prod(st, p, k_products_bf16<KP, 1, SCHED, false>, blocks, c16a[0], APB,
+                         capacity);
 }
 
 template <int KP>

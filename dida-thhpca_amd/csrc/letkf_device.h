@@ -8,6 +8,19 @@ namespace dida {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer loads of 8 and 16 bytes (buffer_load_dwordx2 / dwordx4 offen;
+// an offset past the resource's num_records bytes reads 0), bound to the
+// LLVM intrinsics: this compiler lowers __builtin_amdgcn_raw_buffer_load_b128
+// to a single-dword load.
+__device__ f2 raw_buffer_load_f2(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ f4 raw_buffer_load_f4(__amdgpu_buffer_rsrc_t rsrc, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+
 
 // pro_info.F90:148-161: pi = atan(1.0)*4 evaluated in real(4), and the
 // real(4) literal a = 6371.393, both promoted to real(8).

@@ -115,9 +115,22 @@ void k_tridiag(EigParams p) {
   const bool in_r = lane < k;
 
   double arow[KMAX];
+  if (p.a_f32) {
+    // FP32 in tile order (a_entry; kp = KMAX): one base address per lane and
+    // a constant offset per column, every load unconditional and in bounds
+    // (lanes past kp read position 0), then a select
+    const float* __restrict__ Af =
+        reinterpret_cast<const float*>(Ag) + a_tile_pos(lane < KMAX ? lane : 0, KMAX);
+    float av[KMAX];
 #pragma unroll
-  for (int c = 0; c < KMAX; ++c)
-    arow[c] = (in_r && c < k) ? Ag[(long long)c * kp + lane] : 0.0;
+    for (int c = 0; c < KMAX; ++c) av[c] = Af[a_tile_pos(c, KMAX) * KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) arow[c] = (in_r && c < k) ? (double)av[c] : 0.0;
+  } else {
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c)
+      arow[c] = (in_r && c < k) ? Ag[(long long)c * kp + lane] : 0.0;
+  }
   double xcol = arow[0];
 
   for (int j = 0; j + 2 < k; ++j) {
@@ -1629,7 +1642,7 @@ __global__ __launch_bounds__(KMAX < 64 ? 64 : KMAX) void k_tridiag_lds(EigParams
   const bool in_r = r < k;
   if (row)
     for (int c = 0; c < KMAX; ++c)
-      sA[c * LD + r] = (in_r && c < k) ? Ag[(long long)c * kp + r] : 0.0;
+      sA[c * LD + r] = (in_r && c < k) ? a_entry(Ag, p.a_f32, kp, r, c) : 0.0;
   int slot = 0;
   __syncthreads();
   for (int j = 0; j + 2 < k; ++j) {

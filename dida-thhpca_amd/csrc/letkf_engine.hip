@@ -314,6 +314,7 @@ int prepare_eig(EigScratch& s, const XformParams& xp, EigParams& ep) {
   ep.a0 = xp.a0;
   ep.n_an = n;
   ep.A = xp.A;
+  ep.a_f32 = xp.a_f32;
   ep.b = xp.b;
   ep.refl = s.refl.as<double>();
   ep.tau = s.tau.as<double>();
@@ -793,10 +794,11 @@ static int build_obs_index(dida_letkf_handle* h, int n_sites, const double* lat,
                        (long long)n_rows, rinv, h->rinv32.as<float>());
   if (c.precision != DIDA_PREC_FP64) {
     const long long ny = (long long)n_rows * c.k;
-    // the reduced-precision products address the FP32 Yb with 32-bit offsets
-    if (ny + (long long)kListPad * c.k >= (1LL << 32))
-      return set_error(DIDA_ERR_UNSUPPORTED, "%d obs rows x k=%d exceed the 2^32 values the "
-                       "reduced-precision products address", n_rows, c.k);
+    // the reduced-precision products address the FP32 Yb with 32-bit byte
+    // offsets below kYbOffsetLimit
+    if (4 * (ny + (long long)kListPad * c.k) >= kYbOffsetLimit)
+      return set_error(DIDA_ERR_UNSUPPORTED, "%d obs rows x k=%d exceed the 2 GiB of FP32 Yb "
+                       "the reduced-precision products address", n_rows, c.k);
     RC_TRY(h->yb32.ensure(sizeof(float) * (std::max(ny, 1LL) + (long long)kListPad * c.k)));
     if (ny > 0)
       hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, st, ny,
@@ -1171,8 +1173,10 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
     sp.cnt = h->cnt.as<long long>();
     sp.off = h->off.as<long long>();
     sp.tile = tiled ? kTile : 0;
-    // horizontal loop: weights precomputed by the fill pass
-    sp.weights = h->nzc == 1 ? 1 : 0;
+    // horizontal loop: weights precomputed by the fill pass (for the BF16
+    // products as the FP32 pair {sqrt(w), w dep})
+    sp.weights = h->nzc != 1 ? 0 : c.precision == DIDA_PREC_BF16X3 ? 2 : 1;
+    sp.k = c.k;
     sp.rinv32 = h->rinv32.as<float>();
     sp.dep = h->o_dep.as<double>();
     // count pass -> offsets -> fill pass into the fixed-capacity lists; per
@@ -1227,6 +1231,7 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
     pp.row_cap = h->row_cap;
     pp.yb64 = h->o_yb64.as<double>();
     pp.yb32 = h->yb32.as<float>();
+    pp.yb32_bytes = (unsigned)std::min<size_t>(h->yb32.bytes, kYbOffsetLimit);
     pp.dep = h->o_dep.as<double>();
     pp.rinv32 = h->rinv32.as<float>();
     pp.vco = h->o_vco.as<double>();
@@ -1293,6 +1298,8 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
     xp.g = g;
     xp.k = c.k;
     xp.kp = h->kp;
+    // the BF16 products of k <= 64 store A in FP32 tile order
+    xp.a_f32 = !tiled && c.precision == DIDA_PREC_BF16X3 && h->kp <= 64;
     xp.inflation = c.inflation_factor;
     xp.guard = c.precision != DIDA_PREC_FP64;
     xp.A = pp.A;
@@ -1838,6 +1845,9 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
              double inflation, int precision, int* status) {
   const long long nr = row_ptr[n_an];
   const int kp = kp_of(k);
+  if (precision != DIDA_PREC_FP64 && 4 * (nr + (long long)kListPad) * k >= kYbOffsetLimit)
+    return set_error(DIDA_ERR_UNSUPPORTED, "%lld obs rows x k=%d exceed the 2 GiB of FP32 Yb "
+                     "the reduced-precision products address", nr, k);
   DevBuf* bufs[kCoreMaxBufs];
   size_t sizes[kCoreMaxBufs];
   const int nb = core_layout(cb, es, k, n_an, nr, precision, bufs, sizes);
@@ -1875,6 +1885,7 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
   pp.groups = 1;
   pp.yb64 = cb.y.as<double>();
   pp.yb32 = precision != DIDA_PREC_FP64 ? cb.y32.as<float>() : nullptr;
+  pp.yb32_bytes = (unsigned)std::min<size_t>(cb.y32.bytes, kYbOffsetLimit);
   pp.dep = cb.d.as<double>();
   pp.rinv32 = cb.r.as<float>();
   pp.a0 = 0;
@@ -1887,6 +1898,7 @@ int core_run(hipStream_t st, CoreBufs& cb, EigScratch& es, int mode, int k, long
   xp.g = g;
   xp.k = k;
   xp.kp = kp;
+  xp.a_f32 = precision == DIDA_PREC_BF16X3 && kp <= 64;
   xp.inflation = inflation;
   xp.guard = precision != DIDA_PREC_FP64;
   xp.A = pp.A;

@@ -71,8 +71,12 @@ struct SelectParams {
   double* row_gc;
   long long cap;          // fill pass: capacity of row_idx / row_gc
   // horizontal loop: the fill pass stores w = rinv * rho in row_gc and
-  // w * dep in row_wd, so the products read them with the row index
+  // w * dep in row_wd (weights = 1), or, for the BF16 products, the FP32
+  // pair {sqrt(w), w * dep} in row_gc's 8 bytes and the row's offset in Yb,
+  // row * k, in row_idx (weights = 2), so the products read them with the
+  // list position
   int weights;
+  int k;
   const float* rinv32;
   const double* dep;
   double* row_wd;
@@ -92,14 +96,18 @@ struct ProdParams {
   const double* hybi;
   double p0, ptop;
   const double* axb;
-  const int* row_idx;
+  const int* row_idx;        // obs rows (Yb offsets row * k for the BF16
+                             // products when pre_weighted)
   const double* row_gc;      // GC weight, or w = rinv * rho when pre_weighted
+                             // ({sqrt(w), w dep} in FP32 for the BF16 products)
   const double* row_wd;      // w * dep when pre_weighted
   int pre_weighted;
   const long long* row_off;  // per batch column, n_cols + 1 entries
   long long row_cap;         // entries of row_idx / row_gc / row_wd
   const double* yb64;
   const float* yb32;
+  unsigned yb32_bytes;       // bytes of yb32's allocation (padding included),
+                             // at most kYbOffsetLimit
   const double* dep;
   const float* rinv32;
   const double* vco;
@@ -143,6 +151,10 @@ struct ProdParams {
 // worth of values) past their end, and the list tail holds valid row
 // indices (0), so the BF16 products may read a group's positions past the end
 // of an analysis' list (with weight 0) without clamping each index.
+// The reduced-precision products read the FP32 Yb at 32-bit byte offsets
+// below this limit; a position past the end of a row list reads at the limit
+// itself, which lies past the buffer resource and returns 0.
+constexpr unsigned kYbOffsetLimit = 0x7fff0000u;
 constexpr int kListPad = 128;
 constexpr int kBuckets = 8;
 constexpr int kQueueParts = 8;   // queue counters per bucket position (one per XCD)
@@ -178,6 +190,7 @@ struct XformParams {
   int k, kp;
   double inflation;
   int guard;
+  int a_f32;                 // A in the BF16 products' FP32 tile order (a_entry)
   const double* A;
   const double* b;
   long long a0, n_an;
@@ -224,7 +237,8 @@ struct EigParams {
   const int* list;
   const int* list_n;
   int list_cap;
-  const double* A;          // batch-local, full symmetric kp x kp
+  const double* A;          // batch-local, full symmetric kp x kp (a_entry)
+  int a_f32;                // A in the BF16 products' FP32 tile order
   const double* b;          // batch-local, kp
   double* refl;             // batch-local, kp x kp reflectors (column j = v_j)
   double* tau;              // batch-local, kp
@@ -377,5 +391,19 @@ hipError_t launch_zero_ints(hipStream_t st, int* p, int n);
 constexpr long long kEigPipeMinHalf = 1024;
 hipError_t launch_eig_pipelined(hipStream_t st, hipStream_t st2, hipEvent_t e_first,
                                 hipEvent_t e_join, const EigParams& p, hipEvent_t* ev = nullptr);
+
+// Entry (r, c) of one analysis' A, whose slot holds kp * kp doubles.  FP64:
+// column-major.  a_f32 (the BF16 products, kp = 32 or 64): the FP32 sums in
+// the products' tile order, column-major in the first half of the slot,
+// where member m = T i + t (T = kp / 16, i < 16) sits at position 16 t + i.
+__device__ __forceinline__ int a_tile_pos(int m, int kp) {
+  const int sh = kp == 64 ? 2 : 1;
+  return ((m & ((1 << sh) - 1)) << 4) + (m >> sh);
+}
+__device__ __forceinline__ double a_entry(const double* Ag, int a_f32, int kp, int r, int c) {
+  if (a_f32)
+    return (double)reinterpret_cast<const float*>(Ag)[a_tile_pos(c, kp) * kp + a_tile_pos(r, kp)];
+  return Ag[(long long)c * kp + r];
+}
 
 }  // namespace dida

@@ -208,8 +208,14 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
       const int row = s_r0[wv][lo] + (e - s_pre[wv][lo]);
       const double g = s_gc[wv][lo];
       if (pos < p.cap) {
-        p.row_idx[pos] = row;
-        if (p.weights) {
+        p.row_idx[pos] = p.weights == 2 ? row * p.k : row;
+        if (p.weights == 2) {
+#pragma clang fp contract(off)
+          const double w = (double)p.rinv32[row] * g;
+          // w > 0 is a normal number: the hardware square root (1 ulp) suffices
+          const f2 sw = {__builtin_amdgcn_sqrtf((float)w), (float)(w * p.dep[row])};
+          reinterpret_cast<f2*>(p.row_gc)[pos] = sw;
+        } else if (p.weights) {
 #pragma clang fp contract(off)
           const double w = (double)p.rinv32[row] * g;
           p.row_gc[pos] = w;
@@ -820,24 +826,25 @@ __global__ __launch_bounds__(256) void k_products_f32(ProdParams p) {
 // One 32-row group position of an analysis' row list (rl, with n rows):
 // sqrt(w) and w d in FP32, zero past the end of the list (whose last row is
 // read in its place, so every load stays in bounds without a branch).
-template <bool PW>
+// TAIL false: the caller knows rl < n.
+template <bool PW, bool TAIL = true>
 __device__ __forceinline__ void group_row(const ProdParams& p, long long r0, int n, int rl,
                                           int row, double vc, float& s, float& wdf) {
-  const int rc = rl < n ? rl : n - 1;
-  double w, wd;
+  const bool in = !TAIL || rl < n;
   if constexpr (PW) {
-    // the lists are padded (kListPad): read in place, the select below
-    // drops positions past the end
-    w = p.row_gc[r0 + rl];
-    wd = p.row_wd[r0 + rl];
+    // the fill pass stored {sqrt(w), w d} in FP32; the lists are padded
+    // (kListPad): read in place, the select drops positions past the end
+    const f2 sw = reinterpret_cast<const f2*>(p.row_gc)[r0 + rl];
+    s = in ? sw.x : 0.f;
+    wdf = in ? sw.y : 0.f;
   } else {
-    w = row_weight(p, row, p.row_gc[r0 + rc], vc);
-    wd = w * p.dep[row];
+    const int rc = in ? rl : n - 1;
+    const double w = row_weight(p, row, p.row_gc[r0 + rc], vc);
+    const double wd = w * p.dep[row];
+    // w > 0 is a normal number: the hardware square root (1 ulp) suffices
+    s = in ? __builtin_amdgcn_sqrtf((float)w) : 0.f;
+    wdf = in ? (float)wd : 0.f;
   }
-  const bool in = rl < n;
-  // w > 0 is a normal number: the hardware square root (1 ulp) suffices
-  s = in ? __builtin_amdgcn_sqrtf((float)w) : 0.f;
-  wdf = in ? (float)wd : 0.f;
 }
 
 template <int KP, int WS, int PPW, int G, bool PW>
@@ -861,48 +868,80 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
   // G groups of 32 rows per step (more gathers in flight for G = 2; the
   // MFMAs and the b sums run in the same order, so the sums are the same
   // bits for every G).  The row indices of the next step are loaded one step
-  // ahead, so only the Yb gathers wait on memory; positions past the end
-  // read the list's last entry and enter with weight 0.
-  // The lists are padded past their end with valid row indices (kListPad),
-  // so no index is clamped; Yb rows are read as 32-bit offsets from the
-  // array with the member offsets folded into the loads.
+  // ahead, so only the Yb gathers wait on memory.  The lists are padded past
+  // their end with valid row indices (kListPad), so no index is clamped.
+  // Yb is read through a buffer resource at a 32-bit byte offset per row
+  // (the list holds row * k when pre-weighted), one 4T-byte load per lane and
+  // row: lane i16 holds members T i16 .. T i16 + T - 1, so tile t of the
+  // MFMAs is the members m = T i + t (i = 0..15), and A and b are stored
+  // with the members put back in order.  Members m >= k read the next row
+  // (or 0 past the resource) and are dropped when A and b are stored.
+  // Positions past the end of the list (last step only) load from an offset
+  // past the resource, which reads 0, and enter with weight 0: they add
+  // exact zeros whatever the rows that follow in the list hold.
   int idx_n[G][8];
 #pragma unroll
   for (int gg = 0; gg < G; ++gg)
 #pragma unroll
     for (int u = 0; u < 8; ++u) idx_n[gg][u] = ridx[32 * gg + 8 * q + u];
-  const unsigned kk = (unsigned)p.k;
-  for (int g0 = 0; g0 < n; g0 += 32 * G) {
+  const __amdgpu_buffer_rsrc_t yrs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)p.yb32, 0, (int)p.yb32_bytes, 0x00020000);
+  static_assert(T == 2 || T == 4, "one 8- or 16-byte load per lane and row");
+  const int voff = 4 * T * i16;
+  const int rowb = PW ? 4 : 4 * p.k;           // bytes per list entry unit
+  // one step of G groups from position g0; TAIL: some positions may lie past
+  // the end of the list (the last step only)
+  auto step = [&](int g0, auto tail) {
+    constexpr bool TAIL = decltype(tail)::value;
     bf8 hi[G][T], lo[G][T];
     int idx[G][8];
 #pragma unroll
     for (int gg = 0; gg < G; ++gg)
 #pragma unroll
       for (int u = 0; u < 8; ++u) idx[gg][u] = idx_n[gg][u];
+    if constexpr (!TAIL) {
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg)
+      for (int gg = 0; gg < G; ++gg)
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        idx_n[gg][u] = ridx[g0 + 32 * G + 32 * gg + 8 * q + u];
+        for (int u = 0; u < 8; ++u)
+          idx_n[gg][u] = ridx[g0 + 32 * G + 32 * gg + 8 * q + u];
+    }
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
+      float s[8], wdf[8], yv[8][T];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        float s, wdf;
-        group_row<PW>(p, r0, n, g0 + 32 * gg + 8 * q + u, idx[gg][u], vc, s, wdf);
-        const unsigned base = (unsigned)idx[gg][u] * kk + (unsigned)i16;
+        group_row<PW, TAIL>(p, r0, n, g0 + 32 * gg + 8 * q + u, idx[gg][u], vc, s[u], wdf[u]);
+        const int rl = g0 + 32 * gg + 8 * q + u;
+        const int yo = (!TAIL || rl < n) ? idx[gg][u] * rowb + voff : (int)kYbOffsetLimit;
+        if constexpr (T == 4) {
+          const f4 v = raw_buffer_load_f4(yrs, yo, 0, 0);
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const int m = 16 * t + i16;
-          // members past k read the next row's values (or the padding):
-          // a select drops them, no branch
-          const float yl = p.yb32[base + 16u * t];
-          const float yv = (m < p.k) ? yl : 0.f;
-          const float yh = s * yv;
-          const __bf16 h = (__bf16)yh;
-          hi[gg][t][u] = h;
-          lo[gg][t][u] = (__bf16)(yh - (float)h);
-          if (WS == 0) bacc[t] = fmaf(wdf, yv, bacc[t]);
+          for (int t = 0; t < T; ++t) yv[u][t] = v[t];
+        } else {
+          const f2 v = raw_buffer_load_f2(yrs, yo, 0, 0);
+#pragma unroll
+          for (int t = 0; t < T; ++t) yv[u][t] = v[t];
+        }
+      }
+      // rows u and u + 1 convert as one packed pair: hi = bf16(s y) and
+      // lo = bf16(s y - hi), both rounded to nearest even
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        u4 hw, lw;
+#pragma unroll
+        for (int up = 0; up < 4; ++up) {
+          const f2 yh = {s[2 * up] * yv[2 * up][t], s[2 * up + 1] * yv[2 * up + 1][t]};
+          const bf2 h2 = __builtin_convertvector(yh, bf2);
+          const bf2 l2 = __builtin_convertvector(yh - __builtin_convertvector(h2, f2), bf2);
+          hw[up] = __builtin_bit_cast(unsigned, h2);
+          lw[up] = __builtin_bit_cast(unsigned, l2);
+        }
+        hi[gg][t] = __builtin_bit_cast(bf8, hw);
+        lo[gg][t] = __builtin_bit_cast(bf8, lw);
+        if (WS == 0) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) bacc[t] = fmaf(wdf[u], yv[u][t], bacc[t]);
         }
       }
     }
@@ -916,16 +955,25 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
         acc[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[gg][I], hi[gg][J], acc[x], 0, 0, 0);
       }
     }
-  }
-  double* A = p.A + (a - p.a0) * (long long)KP * KP;
+  };
+  const int n_full = n / (32 * G) * (32 * G);
+  for (int g0 = 0; g0 < n_full; g0 += 32 * G) step(g0, std::false_type{});
+  if (n_full < n) step(n_full, std::true_type{});
+  // A in FP32 tile order (a_entry): tile entry (4q + r, i16) of tiles (I, J)
+  // is members T (4q + r) + I and T i16 + J, stored at positions
+  // 16 I + 4q + r and 16 J + i16, so the sixteen lanes of a quarter write
+  // consecutive words
+  float* A = reinterpret_cast<float*>(p.A + (a - p.a0) * (long long)KP * KP);
 #pragma unroll
   for (int x = 0; x < NP; ++x) {
     const int I = PT::I(P0 + x), J = PT::J(P0 + x);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = 16 * I + 4 * q + r, col = 16 * J + i16;
-      A[(long long)col * KP + row] = (double)acc[x][r];
-      if (I != J) A[(long long)row * KP + col] = (double)acc[x][r];
+      const int mr = T * (4 * q + r) + I, mc = T * i16 + J;
+      const int pr = 16 * I + 4 * q + r, pc = 16 * J + i16;
+      const float v = (mr < p.k && mc < p.k) ? acc[x][r] : 0.f;
+      A[pc * KP + pr] = v;
+      if (I != J) A[pr * KP + pc] = v;
     }
   }
   if (WS == 0) {
@@ -935,7 +983,7 @@ __device__ __forceinline__ void products_bf16_wave(const ProdParams& p, long lon
       float v = bacc[t];
       v += __shfl_xor(v, 16, 64);
       v += __shfl_xor(v, 32, 64);
-      if (q == 0) b[16 * t + i16] = (double)v;
+      if (q == 0) b[T * i16 + t] = (T * i16 + t < p.k) ? (double)v : 0.0;
     }
   }
 }
@@ -1008,7 +1056,8 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
     for (int u = 0; u < 8; ++u) {
       float s, wdf;
       group_row<PW>(p, r0, n, g0 + 8 * q + u, idx[u], vc, s, wdf);
-      const float* yr = p.yb32 + (long long)idx[u] * p.k;
+      // pre-weighted lists hold the row's offset row * k
+      const float* yr = p.yb32 + (PW ? (long long)idx[u] : (long long)idx[u] * p.k);
 #pragma unroll
       for (int tt = 0; tt < TW; ++tt) {
         const int m = 16 * (WS * TW + tt) + i16;
@@ -1151,7 +1200,8 @@ __global__ __launch_bounds__(64) void k_transform(XformParams p) {
   for (int c = 0; c < KMAX; ++c) {
     double v = 0.0;
     if (lane < KMAX) {
-      if (act_r && c < k) v = (lane >= c) ? Ag[(long long)c * p.kp + lane] : Ag[(long long)lane * p.kp + c];
+      if (act_r && c < k)
+        v = (lane >= c) ? a_entry(Ag, p.a_f32, p.kp, lane, c) : a_entry(Ag, p.a_f32, p.kp, c, lane);
       sA[c * LD + lane] = v;
     }
   }
@@ -1907,11 +1957,11 @@ hipError_t launch_products_tile(hipStream_t st, const ProdParams& p, int precisi
 
 // Workgroups of kernel f resident on the whole device at once (the grid of a
 // queue-schedule launch), computed once per kernel.
-static int resident_blocks(const void* f, std::atomic<int>& cache) {
+static int resident_blocks(const void* f, int threads, std::atomic<int>& cache) {
   int v = cache.load(std::memory_order_relaxed);
   if (v > 0) return v;
   int per_cu = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, 0) != hipSuccess ||
       hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
@@ -1920,13 +1970,19 @@ static int resident_blocks(const void* f, std::atomic<int>& cache) {
   return v;
 }
 
+// One workgroup per analysis: its wavefronts (one, or four sharing the
+// analysis) start and retire together, so no wavefront waits on another
+// analysis of its workgroup.
 template <class K>
 static hipError_t launch_prod(hipStream_t st, const ProdParams& p, K* kern, long long blocks,
-                              std::atomic<int>& cache, int apb, long long* capacity) {
+                              std::atomic<int>& cache, int wpa, long long* capacity) {
   ProdParams q = p;
   unsigned grid;
-  const int r = (q.order || capacity) ? resident_blocks(reinterpret_cast<const void*>(kern), cache)
-                                      : 1;
+  constexpr int apb = 1;
+  const int threads = 64 * wpa;
+  const int r = (q.order || capacity)
+                    ? resident_blocks(reinterpret_cast<const void*>(kern), threads, cache)
+                    : 1;
   if (r <= 0) return hipErrorInvalidDeviceFunction;
   if (q.q_counter && q.order) {
     grid = (unsigned)std::min<long long>(r, blocks);
@@ -1943,11 +1999,7 @@ static hipError_t launch_prod(hipStream_t st, const ProdParams& p, K* kern, long
   }
   // wavefront groups the device holds at once (the load balance's capacity)
   if (capacity) *capacity = (long long)std::min<long long>(r, grid) * apb;
-  static const int lds = [] {
-    const char* e = std::getenv("DIDA_PROD_LDS");
-    return e ? std::atoi(e) : 0;
-  }();
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, st, q);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, st, q);
   return hipGetLastError();
 }
 
@@ -1957,37 +2009,32 @@ static hipError_t launch_products_sched(hipStream_t st, const ProdParams& p, int
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;
   constexpr int WPA = (NPAIR + 9) / 10;
-  constexpr int APB = 4 / WPA;
   // PW: the row lists carry the weights w and w d (horizontal loop), so the
   // products read them instead of forming them per row; straight-line code
   // lets a group's loads be in flight together
   static std::atomic<int> c64[2], c32[2], c16a[2], c16b[2];
   const int w = p.pre_weighted ? 1 : 0;
   if (precision == 0)
-    return w ? launch_prod(st, p, k_products_f64<KP, SCHED, true>, blocks, c64[1], APB, capacity)
-             : launch_prod(st, p, k_products_f64<KP, SCHED, false>, blocks, c64[0], APB, capacity);
+    return w ? launch_prod(st, p, k_products_f64<KP, SCHED, true>, blocks, c64[1], WPA, capacity)
+             : launch_prod(st, p, k_products_f64<KP, SCHED, false>, blocks, c64[0], WPA, capacity);
   if (precision == 1)
-    return w ? launch_prod(st, p, k_products_f32<KP, SCHED, true>, blocks, c32[1], APB, capacity)
-             : launch_prod(st, p, k_products_f32<KP, SCHED, false>, blocks, c32[0], APB, capacity);
+    return w ? launch_prod(st, p, k_products_f32<KP, SCHED, true>, blocks, c32[1], WPA, capacity)
+             : launch_prod(st, p, k_products_f32<KP, SCHED, false>, blocks, c32[0], WPA, capacity);
   if (WPA == 1 && p.groups == 2)
-    return w ? launch_prod(st, p, k_products_bf16<KP, 2, SCHED, true>, blocks, c16b[1], APB,
+    return w ? launch_prod(st, p, k_products_bf16<KP, 2, SCHED, true>, blocks, c16b[1], WPA,
                            capacity)
-             : launch_prod(st, p, k_products_bf16<KP, 2, SCHED, false>, blocks, c16b[0], APB,
+             : launch_prod(st, p, k_products_bf16<KP, 2, SCHED, false>, blocks, c16b[0], WPA,
                            capacity);
-  return w ? launch_prod(st, p, k_products_bf16<KP, 1, SCHED, true>, blocks, c16a[1], APB,
+  return w ? launch_prod(st, p, k_products_bf16<KP, 1, SCHED, true>, blocks, c16a[1], WPA,
                          capacity)
-           : launch_prod(st, p, k_products_bf16<KP, 1, SCHED, false>, blocks, c16a[0], APB,
+           : launch_prod(st, p, k_products_bf16<KP, 1, SCHED, false>, blocks, c16a[0], WPA,
                          capacity);
 }
 
 template <int KP>
 static hipError_t launch_products_kp(hipStream_t st, const ProdParams& p, int precision,
                                      long long* capacity) {
-  constexpr int T = KP / 16;
-  constexpr int NPAIR = T * (T + 1) / 2;
-  constexpr int WPA = (NPAIR + 9) / 10;
-  constexpr int APB = 4 / WPA;
-  const long long blocks = (p.n_an + APB - 1) / APB;
+  const long long blocks = p.n_an;
   if (blocks <= 0) return hipSuccess;
   // order set: a bucket segment, in dispatch order or (q_counter set) on a
   // persistent queue; the ticks select the timed instantiation

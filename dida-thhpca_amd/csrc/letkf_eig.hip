@@ -404,6 +404,14 @@ __device__ __forceinline__ void ring_block(const double2* sring, int base, int B
   for (int t = 0; t < 8; ++t) cs[t] = e[7 - t];
 }
 
+// Read-only global memory addressed uniformly across the wavefront: loads
+// through these pointer types go to the scalar unit (s_load).  Only for data
+// that no kernel writes while the reader runs.
+typedef const __attribute__((address_space(4))) int* uniform_int_cptr;
+__device__ __forceinline__ int4 load_uniform_int4(uniform_int_cptr q, int i) {
+  return make_int4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
+}
+
 template <int KMAX>
 struct VecLds {
   static constexpr int LDV = KMAX + 1;
@@ -446,22 +454,26 @@ void k_vec(EigParams p) {
   const int nref = k > 2 ? k - 2 : 0;
 
   // ---- A. Z <- Z H_j, reflectors staged 16 at a time ----
+  // The next 16 reflectors are loaded into registers while the current 16
+  // are applied, so their memory latency is hidden behind the work.
   if (lane < nref) stau[lane] = tg[lane];
   double z[KMAX];
 #pragma unroll
   for (int c = 0; c < KMAX; ++c) z[c] = (c == lane) ? 1.0 : 0.0;
+  double vnext[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+    vnext[t] = (t < nref && lane < KMAX) ? refl[(long long)t * kp + lane] : 0.0;
   for (int j0 = 0; j0 < nref; j0 += 16) {
     __syncthreads();
-    {
-      double v[16];
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
-        v[t] = (j0 + t < nref && lane < KMAX) ? refl[(long long)(j0 + t) * kp + lane] : 0.0;
-#pragma unroll
-      for (int t = 0; t < 16; ++t)
-        if (lane < KMAX) sbuf[t * LDV + lane] = v[t];
-    }
+    for (int t = 0; t < 16; ++t)
+      if (lane < KMAX) sbuf[t * LDV + lane] = vnext[t];
     __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      vnext[t] = (j0 + 16 + t < nref && lane < KMAX) ? refl[(long long)(j0 + 16 + t) * kp + lane]
+                                                     : 0.0;
     const int j1 = min(j0 + 16, nref);
     for (int j = j0; j < j1; ++j) {
       const double tau = uniform(stau[j]);
@@ -495,38 +507,48 @@ void k_vec(EigParams p) {
   {
     double2* sring = reinterpret_cast<double2*>(sbuf);
     const double2* __restrict__ rot = p.rot + ar * (long long)p.rot_cap;
-    const int4* __restrict__ hdr = p.hdr + ar * (long long)p.hdr_cap;
+    // the chase headers are read as scalars (s_load), so waiting for one does
+    // not wait for the vector loads of the next refill issued before it
+    const uniform_int_cptr hdr =
+        (uniform_int_cptr)(p.hdr + (long long)uniform((int)ar) * p.hdr_cap);
     const int nch = uniform(min(p.nchase[ar], p.hdr_cap));
     int nrot = 0;
     if (nch > 0) {
-      const int4 hl = hdr[nch - 1];
+      const int4 hl = load_uniform_int4(hdr, nch - 1);
       nrot = uniform(hl.z + hl.w);
     }
     int committed = 0;
-    int4 hn = nch > 0 ? hdr[0] : make_int4(0, 0, 0, 0);
+    int4 hn = nch > 0 ? load_uniform_int4(hdr, 0) : make_int4(0, 0, 0, 0);
+    // the entries of the next refill, loaded one refill ahead so that the
+    // log's memory latency is hidden behind the chases in between
+    double2 vnext[REFILL / 64];
+#pragma unroll
+    for (int t = 0; t < REFILL / 64; ++t) {
+      const int e = 64 * t + lane;
+      vnext[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
+    }
     for (int ch = 0; ch < nch; ++ch) {
       int4 h = hn;
       h.x = uniform(h.x);
       h.y = uniform(h.y);
       h.z = uniform(h.z);
       h.w = uniform(h.w);
-      if (ch + 1 < nch) hn = hdr[ch + 1];
+      if (ch + 1 < nch) hn = load_uniform_int4(hdr, ch + 1);
       const int lo = h.x, hi = h.y;
       if (h.z + h.w > committed) {
-        double2 v[REFILL / 64];
-#pragma unroll
-        for (int t = 0; t < REFILL / 64; ++t) {
-          const int e = committed + 64 * t + lane;
-          v[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
-        }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < REFILL / 64; ++t) {
           const int slot = (committed + 64 * t + lane) & (RING - 1);
-          sring[slot] = v[t];
-          if (slot < 8) sring[RING + slot] = v[t];
+          sring[slot] = vnext[t];
+          if (slot < 8) sring[RING + slot] = vnext[t];
         }
         committed += REFILL;
+#pragma unroll
+        for (int t = 0; t < REFILL / 64; ++t) {
+          const int e = committed + 64 * t + lane;
+          vnext[t] = (e < nrot) ? rot[e] : make_double2(1.0, 0.0);
+        }
         __syncthreads();
       }
       const int bhi = (hi - 1) >> 3, blo = lo >> 3;

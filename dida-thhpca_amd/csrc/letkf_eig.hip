@@ -439,7 +439,17 @@ void k_vec(EigParams p) {
   constexpr int LDV = L::LDV, RING = L::RING, REFILL = L::REFILL;
   static_assert(REFILL >= KMAX + 16 && REFILL % 64 == 0, "a refill covers any chase");
   __shared__ __align__(16) double sbuf[L::N];
-  __shared__ double sf[KMAX], sg[KMAX], sb[KMAX], st[KMAX], sw[KMAX], stau[KMAX];
+  // stage C's vectors share one array with stage A's Gram matrices (16 x 16),
+  // which are dead by then: k_vec<64> then needs 11.4 KB of LDS, so eight of
+  // its workgroups and one of k_ql_log's (66.5 KB) fit a CU together
+  constexpr int NSC = 5 * KMAX > 256 ? 5 * KMAX : 256;
+  __shared__ double ssc[NSC], stau[KMAX];
+  double* const sf = ssc;
+  double* const sg = ssc + KMAX;
+  double* const sb = ssc + 2 * KMAX;
+  double* const st = ssc + 3 * KMAX;
+  double* const sw = ssc + 4 * KMAX;
+  double* const sgram = ssc;
   const int lane = threadIdx.x;
   long long a, ai, ar;
   if (!eig_slot(p, blockIdx.x, a, ai, ar)) return;
@@ -464,40 +474,74 @@ void k_vec(EigParams p) {
 #pragma unroll
   for (int t = 0; t < 16; ++t)
     vnext[t] = (t < nref && lane < KMAX) ? refl[(long long)t * kp + lane] : 0.0;
+  // Reflectors apply in groups of four: the four dots from the same z, each
+  // corrected by the earlier ones' scales times the pairwise products
+  // v_s . v_t (the dot with z as the earlier reflectors of the group left
+  // it), then one pass z -= sum_t alpha_t v_t.  A group is one dependent
+  // step instead of four.  The products v_s . v_t of a staged block are its
+  // Gram matrix V^T V, one f64 MFMA 16x16x4 per four columns.
   for (int j0 = 0; j0 < nref; j0 += 16) {
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < 16; ++t)
       if (lane < KMAX) sbuf[t * LDV + lane] = vnext[t];
     __syncthreads();
+    {
+      const int i16 = lane & 15, q = lane >> 4;
+      d4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s4 = 0; s4 < KMAX / 4; ++s4) {
+        const double v = sbuf[i16 * LDV + 4 * s4 + q];      // V[4 s4 + q][i16]
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(v, v, g, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sgram[(q + 4 * r) * 16 + i16] = g[r];
+    }
 #pragma unroll
     for (int t = 0; t < 16; ++t)
       vnext[t] = (j0 + 16 + t < nref && lane < KMAX) ? refl[(long long)(j0 + 16 + t) * kp + lane]
                                                      : 0.0;
+    __syncthreads();
     const int j1 = min(j0 + 16, nref);
-    for (int j = j0; j < j1; ++j) {
-      const double tau = uniform(stau[j]);
-      if (tau == 0.0) continue;
-      const double* q = sbuf + (j - j0) * LDV;
-      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    for (int j = j0; j < j1; j += 4) {
+      // past nref the staged columns are zero (tau 0), so a short last group
+      // runs with zero reflectors
+      const int t0 = j - j0;
+      double tau[4], al[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) tau[u] = j + u < j1 ? uniform(stau[j + u]) : 0.0;
+      const double* q0 = sbuf + t0 * LDV;
+      double acc[4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = 0.0;
 #pragma unroll
       for (int B = 0; B < NB; ++B) {
         if (8 * B + 7 <= j) continue;
-        d0 = fma(z[8 * B + 0], q[8 * B + 0], d0);
-        d1 = fma(z[8 * B + 1], q[8 * B + 1], d1);
-        d2 = fma(z[8 * B + 2], q[8 * B + 2], d2);
-        d3 = fma(z[8 * B + 3], q[8 * B + 3], d3);
-        d0 = fma(z[8 * B + 4], q[8 * B + 4], d0);
-        d1 = fma(z[8 * B + 5], q[8 * B + 5], d1);
-        d2 = fma(z[8 * B + 6], q[8 * B + 6], d2);
-        d3 = fma(z[8 * B + 7], q[8 * B + 7], d3);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int c = 8 * B + t;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[u][t & 1] = fma(z[c], q0[u * LDV + c], acc[u][t & 1]);
+        }
       }
-      const double dot = ((d0 + d1) + (d2 + d3)) * tau;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double d = acc[u][0] + acc[u][1];
+#pragma unroll
+        for (int v = 0; v < u; ++v) d = fma(-al[v], uniform(sgram[(t0 + v) * 16 + t0 + u]), d);
+        al[u] = d * tau[u];
+      }
 #pragma unroll
       for (int B = 0; B < NB; ++B) {
         if (8 * B + 7 <= j) continue;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) z[8 * B + t] = fma(-dot, q[8 * B + t], z[8 * B + t]);
+        for (int t = 0; t < 8; ++t) {
+          const int c = 8 * B + t;
+          double zc = z[c];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) zc = fma(-al[u], q0[u * LDV + c], zc);
+          z[c] = zc;
+        }
       }
     }
   }

@@ -1262,6 +1262,10 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
       bp.order = h->b_order.as<int>();
       bp.q_counters = h->b_q.as<int>();
       bp.ticks = h->opt_timing ? h->b_ticks.as<unsigned long long>() + bi * 5 * kBuckets : nullptr;
+      if (products_band() > 0 && h0 % g.nlat_c == 0 && (h1 - h0) % g.nlat_c == 0) {
+        bp.band_w = products_band();
+        bp.band_l = g.nlat_c;
+      }
       HIP_TRY(launch_buckets(st, bp));
       pp.order = bp.order;
       pp.seg_off = bp.seg_off;

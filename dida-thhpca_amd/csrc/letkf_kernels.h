@@ -182,6 +182,8 @@ struct BucketParams {
   int* q_counters;           // zeroed by the scan kernel (kBuckets x kQueueParts)
   unsigned long long* ticks; // optional: kBuckets bucket sums, then 4 per bucket
                              // position for launch spans (reset by the scan kernel)
+  int band_w, band_l;        // > 0: columns enter the sort in bands of band_w
+                             // longitudes, latitude by latitude (band_index)
 };
 constexpr int kBucketBlock = 1024;
 

@@ -397,14 +397,18 @@ __device__ __forceinline__ int xcd_block(int n_blocks) {
 // XCD form a compact patch and share more of their gathered obs rows in its
 // L2 than a run along one meridian does.  A bijection of [0, n_an): the last
 // band may be narrower.
-__device__ __forceinline__ long long analysis_at(const ProdParams& p, long long li) {
-  if (p.order) return p.order[li];
-  if (p.band_w <= 0) return li;
-  const long long L = p.band_l, B = p.band_w;
-  const long long NL = p.n_an / L;
+__device__ __forceinline__ long long band_index(long long li, long long n, long long B,
+                                                long long L) {
+  const long long NL = n / L;
   const long long b = li / (B * L), rem = li - b * B * L;
   const long long Bb = min(B, NL - b * B);
   return (b * B + rem % Bb) * L + rem / Bb;
+}
+
+__device__ __forceinline__ long long analysis_at(const ProdParams& p, long long li) {
+  if (p.order) return p.order[li];
+  if (p.band_w <= 0) return li;
+  return band_index(li, p.n_an, p.band_w, p.band_l);
 }
 
 // The analyses a products launch visits.  WPA is 1 (a wavefront per
@@ -1631,8 +1635,15 @@ hipError_t launch_zero_ll(hipStream_t st, long long* p, long long n) {
 // Length buckets of a batch's columns over their local obs count p: a stable
 // counting sort (columns keep their order within a bucket, so neighbouring
 // columns, which share obs rows in L2, stay close in the queue), buckets
-// longest first.  Three launches: per-block counts, one workgroup scanning the
-// block counts per bucket, and the scatter.
+// longest first.  With band_w > 0 the columns enter the sort in bands of
+// band_w longitudes, latitude by latitude, so the analyses that one XCD runs
+// at a time form a compact patch rather than a run along meridians.  Three
+// launches: per-block counts, one workgroup scanning the block counts per
+// bucket, and the scatter.
+
+__device__ __forceinline__ int bucket_column(const BucketParams& p, int e) {
+  return p.band_w > 0 ? (int)band_index(e, p.n_cols, p.band_w, p.band_l) : e;
+}
 
 long long bucket_blk_elems(int n_cols) {
   return (long long)((n_cols + kBucketBlock - 1) / kBucketBlock) * 3 * kBuckets;
@@ -1649,9 +1660,8 @@ __global__ __launch_bounds__(256) void k_bucket_blocks(BucketParams p) {
   const int base = blockIdx.x * kBucketBlock + 4 * threadIdx.x;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int c = base + u;
-    if (c < p.n_cols) {
-      const int r = p.counts[c];
+    if (base + u < p.n_cols) {
+      const int r = p.counts[bucket_column(p, base + u)];
       const int b = bucket_of(r);
       atomicAdd(&s_cnt[b], 1);
       atomicAdd(&s_rows[b], (unsigned long long)r);
@@ -1717,7 +1727,8 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(BucketParams p) {
   const int base = blockIdx.x * kBucketBlock + 4 * threadIdx.x;
   int bk[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) bk[u] = base + u < p.n_cols ? bucket_of(p.counts[base + u]) : -1;
+  for (int u = 0; u < 4; ++u)
+    bk[u] = base + u < p.n_cols ? bucket_of(p.counts[bucket_column(p, base + u)]) : -1;
   const long long* blk = p.blk + (long long)blockIdx.x * 3 * kBuckets;
   for (int b = 0; b < kBuckets; ++b) {
     long long mine = 0;
@@ -1731,7 +1742,7 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(BucketParams p) {
     for (int u = 0; u < 4; ++u) {
       if (bk[u] != b) continue;
       const long long pos = first + rank++;
-      const int c = base + u;
+      const int c = bucket_column(p, base + u);
       for (int kc = 0; kc < p.nzc; ++kc) p.order[pos * p.nzc + kc] = c * p.nzc + kc;
     }
   }

@@ -1136,9 +1136,11 @@ __device__ __forceinline__ void products_bf16_shared(const ProdParams& p, long l
 }
 
 // Three wavefronts per SIMD, and no more: given room for a fourth, the
-// compiler serialises the row gathers to fit its registers.
+// compiler serialises the row gathers to fit its registers.  The persistent
+// queue loops (SCHED 3, 4) keep their state across analyses and get two.
 template <int KP, int G, int SCHED, bool PW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(SCHED >= 3 ? 2 : 3, SCHED >= 3 ? 2 : 3)))
 void k_products_bf16(ProdParams p) {
   constexpr int T = KP / 16;
   constexpr int NPAIR = T * (T + 1) / 2;

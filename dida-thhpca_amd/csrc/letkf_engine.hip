@@ -1173,6 +1173,10 @@ static int enqueue_call(dida_letkf_handle* h, const StateArgs& sa, bool timed, i
     sp.cnt = h->cnt.as<long long>();
     sp.off = h->off.as<long long>();
     sp.tile = tiled ? kTile : 0;
+    if (products_band() > 0 && h0 % g.nlat_c == 0 && (h1 - h0) % g.nlat_c == 0) {
+      sp.band_w = products_band();
+      sp.band_l = g.nlat_c;
+    }
     // horizontal loop: weights precomputed by the fill pass (for the BF16
     // products as the FP32 pair {sqrt(w), w dep})
     sp.weights = h->nzc != 1 ? 0 : c.precision == DIDA_PREC_BF16X3 ? 2 : 1;

@@ -84,6 +84,9 @@ struct SelectParams {
   // [h0, h1); the union of their candidate rows, with the sorted site index
   int tile;
   int* row_site;
+  // column mode: > 0 visits the columns in bands of band_w longitudes,
+  // latitude by latitude (band_l latitudes per longitude)
+  int band_w, band_l;
 };
 
 struct ProdParams {

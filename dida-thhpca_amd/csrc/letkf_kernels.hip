@@ -148,6 +148,17 @@ __device__ __forceinline__ double fast_v(const SelectParams& p, int s, double c2
   return p.s_cos[s] * c2 * (p.s_cosl[s] * cl2 + p.s_sinl[s] * sl2) + p.s_sin[s] * s2;
 }
 
+// Position li of n columns (latitude fastest, L latitudes per longitude) in
+// bands of B longitudes visited latitude by latitude: a bijection of [0, n)
+// when n is a multiple of L (the last band may be narrower).
+__device__ __forceinline__ long long band_index(long long li, long long n, long long B,
+                                                long long L) {
+  const long long NL = n / L;
+  const long long b = li / (B * L), rem = li - b * B * L;
+  const long long Bb = min(B, NL - b * B);
+  return (b * B + rem % Bb) * L + rem / Bb;
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   __shared__ int s_pre[4][64], s_r0[4][64];
@@ -155,9 +166,17 @@ __global__ __launch_bounds__(256) void k_select(SelectParams p) {
   __shared__ int s_q[4][128];            // fill pass: queued candidates (FIFO, site order)
   const int lane = threadIdx.x & 63;
   const int wv = (threadIdx.x >> 6) & 3;
-  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // Workgroups are dealt round-robin to the 8 XCDs; each XCD walks a
+  // contiguous range of visiting positions (the grid is padded to a multiple
+  // of 8), and with band_w > 0 the positions go over the columns in bands of
+  // longitudes, so the columns in flight on one XCD form a compact patch
+  // whose candidate sites share its L2.
+  const int ncol = p.h1 - p.h0;
+  const int per = (int)gridDim.x / 8;
+  const int li = ((int)(blockIdx.x % 8) * per + (int)blockIdx.x / 8) * 4 + wv;
+  if (li >= ncol) return;
+  const int wid = p.band_w > 0 ? (int)band_index(li, ncol, p.band_w, p.band_l) : li;
   const int ah = p.h0 + wid;
-  if (ah >= p.h1) return;
   int sj, ej, si, ei;
   column_center(p.g, ah, &sj, &ej, &si, &ei);
   const double clat = (double)p.g.glat[(sj + ej) / 2 - 1];
@@ -397,14 +416,6 @@ __device__ __forceinline__ int xcd_block(int n_blocks) {
 // XCD form a compact patch and share more of their gathered obs rows in its
 // L2 than a run along one meridian does.  A bijection of [0, n_an): the last
 // band may be narrower.
-__device__ __forceinline__ long long band_index(long long li, long long n, long long B,
-                                                long long L) {
-  const long long NL = n / L;
-  const long long b = li / (B * L), rem = li - b * B * L;
-  const long long Bb = min(B, NL - b * B);
-  return (b * B + rem % Bb) * L + rem / Bb;
-}
-
 __device__ __forceinline__ long long analysis_at(const ProdParams& p, long long li) {
   if (p.order) return p.order[li];
   if (p.band_w <= 0) return li;
@@ -1521,7 +1532,7 @@ hipError_t launch_select_tile(hipStream_t st, const SelectParams& p, bool fill) 
 hipError_t launch_select(hipStream_t st, const SelectParams& p, bool fill) {
   const int ncol = p.h1 - p.h0;
   if (ncol <= 0) return hipSuccess;
-  const int blocks = (ncol + 3) / 4;
+  const int blocks = ((ncol + 3) / 4 + 7) / 8 * 8;     // a multiple of 8 (XCDs)
   if (fill) hipLaunchKernelGGL(k_select<true>, dim3(blocks), dim3(256), 0, st, p);
   else hipLaunchKernelGGL(k_select<false>, dim3(blocks), dim3(256), 0, st, p);
   return hipGetLastError();

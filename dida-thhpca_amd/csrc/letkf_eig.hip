@@ -701,13 +701,26 @@ void k_vec(EigParams p) {
           }
     }
     if (p.xb || p.xb32) {
-      // the B operand of the update, over the (consumed) reflector block
+      // the B operand of the update, over the (consumed) reflector block; for
+      // the FP32 state in FP32 (the update rounds it to FP32 anyway), half
+      // the bytes to write here and to read there
+      if (p.state_f32) {
+        float* const rf = reinterpret_cast<float*>(refl);
 #pragma unroll
-      for (int I = 0; I < IPP; ++I)
+        for (int I = 0; I < IPP; ++I)
 #pragma unroll
-        for (int J = 0; J < TT; ++J)
+          for (int J = 0; J < TT; ++J)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) refl[tile_off<KMAX>(pass * IPP + I, J, r, lane)] = acc[I][J][r];
+            for (int r = 0; r < 4; ++r)
+              rf[tile_off<KMAX>(pass * IPP + I, J, r, lane)] = (float)acc[I][J][r];
+      } else {
+#pragma unroll
+        for (int I = 0; I < IPP; ++I)
+#pragma unroll
+          for (int J = 0; J < TT; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) refl[tile_off<KMAX>(pass * IPP + I, J, r, lane)] = acc[I][J][r];
+      }
     }
   }
   if (__builtin_amdgcn_ballot_w64(in_r && !__builtin_isfinite(wbar))) {
@@ -846,11 +859,13 @@ void k_update(EigParams p) {
   constexpr bool F32 = sizeof(TS) == 4;
   auto out_row = [q](int r) { return F32 ? 4 * q + r : q + 4 * r; };
   const TS* __restrict__ xb = ST::xb(p);
+  // trans tiles from k_vec: FP32 for the FP32 state, else FP64
   const double* __restrict__ tb = p.refl + ai * (long long)p.kp * p.kp;
+  const TS* __restrict__ tbs = reinterpret_cast<const TS*>(tb);
   for (int idx = tid; idx < TT * TT * 4 * 64; idx += 256) {
     const int l = idx & 63, c = idx >> 6, r = c & 3, IJ = c >> 2;
     const int row = 16 * (IJ / TT) + 4 * r + (l >> 4), col = 16 * (IJ % TT) + (l & 15);
-    sT[row * LDT + col] = (row < k && col < k) ? (TS)tb[idx] : (TS)0;
+    sT[row * LDT + col] = (row < k && col < k) ? tbs[idx] : (TS)0;
   }
 
   const AnRows R = an_rows(p, a);

@@ -10,7 +10,7 @@ versions of the oracle are checked against fixed data.
   letkf_core.npz      12 letkf_core cases, k in {2,3,8,20} x p in {0,1,17,300}
                       (one near-degenerate spectrum among them), inputs and
                       (trans, overline_w)
-  c0_decimated.npz    configs[0] generator on a 8x4-decimated grid
+  c0_decimated.npz    the c0 workload's generator (72x36 grid, k=20) on a 8x4-decimated grid
                       (9 x 9 columns, 250 obs): inputs and (xa, axa,
                       overline_w, nobsl)
 
@@ -72,7 +72,7 @@ def main():
     out["inflation_factor"] = np.array(1.05)
     np.savez_compressed(os.path.join(HERE, "letkf_core.npz"), **out)
 
-    # configs[0] decimated by 8 in lon and 4 in lat: 9 x 9 columns
+    # the c0 workload decimated by 8 in lon and 4 in lat: 9 x 9 columns
     inst = synth.generate("c0", seed=3, nlat=9, nlon=9, n_obs=250)
     ref = orc.letkf_compute(inst, want_trans=True, nthreads=1)
     o = inst.obs

@@ -95,6 +95,30 @@ __device__ void nan_transform(const EigParams& p, long long a) {
 // ---------------------------------------------------------------------------
 // k_tridiag
 // ---------------------------------------------------------------------------
+// Writes arow[c] for the uniform column c = j1 to dst: a switch on a
+// wavefront-uniform value, lowered to a short tree of scalar branches (no
+// per-column selects or dynamic register indexing), with one LDS store.
+template <int KMAX>
+__device__ __forceinline__ void store_column(const double (&arow)[KMAX], int j1, double* dst) {
+  double x = 0.0;
+  switch (j1) {
+#define DIDA_COL_CASE(c) \
+  case c:                \
+    if constexpr (c < KMAX) x = arow[c]; \
+    break;
+#define DIDA_COL_CASE8(b) \
+  DIDA_COL_CASE(b) DIDA_COL_CASE(b + 1) DIDA_COL_CASE(b + 2) DIDA_COL_CASE(b + 3) \
+  DIDA_COL_CASE(b + 4) DIDA_COL_CASE(b + 5) DIDA_COL_CASE(b + 6) DIDA_COL_CASE(b + 7)
+    DIDA_COL_CASE8(0) DIDA_COL_CASE8(8) DIDA_COL_CASE8(16) DIDA_COL_CASE8(24)
+    DIDA_COL_CASE8(32) DIDA_COL_CASE8(40) DIDA_COL_CASE8(48) DIDA_COL_CASE8(56)
+#undef DIDA_COL_CASE8
+#undef DIDA_COL_CASE
+    default:
+      break;
+  }
+  *dst = x;
+}
+
 // k = 64 is capped at 168 registers (three wavefronts per SIMD: the kernel is
 // bound by the latency of its per-step reductions and LDS round trips, and a
 // few spilled scalars cost less than the third wavefront gains)
@@ -102,7 +126,8 @@ template <int KMAX>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX == 64 ? 3 : 2)))
 void k_tridiag(EigParams p) {
   constexpr int NB = KMAX / 8;
-  __shared__ double sv[KMAX], sw[KMAX];
+  __shared__ __align__(16) double sv[KMAX];
+  __shared__ __align__(16) double sw[KMAX];
   __shared__ double se_s[KMAX + 1], st_s[KMAX + 1];
   __shared__ double sxn[64];             // column j+1 after step j's update
   const int lane = threadIdx.x;
@@ -140,11 +165,8 @@ void k_tridiag(EigParams p) {
     const double rest = wave_sum_full((act && lane > j + 1) ? x * x : 0.0);
     __builtin_amdgcn_sched_barrier(0);
     if (rest == 0.0) {
-      // column already in tridiagonal form; column j+1 goes through LDS,
-      // written under a uniform branch (no per-slot selects)
-#pragma unroll
-      for (int c = 0; c < KMAX; ++c)
-        if (c == j + 1) sxn[lane] = arow[c];
+      // column already in tridiagonal form; column j+1 goes through LDS
+      store_column(arow, j + 1, &sxn[lane]);
       if (lane < KMAX) refl[(long long)j * kp + lane] = 0.0;
       if (lane == 0) { se_s[j] = x0; st_s[j] = 0.0; }
       __syncthreads();
@@ -181,17 +203,20 @@ void k_tridiag(EigParams p) {
     const double wr = act ? pr - K * v : 0.0;
     if (lane < KMAX) sw[lane] = wr;
     __syncthreads();
+    // rank-2 update A -= v w^T + w v^T, a block of 8 columns per LDS round
+    // trip (sv and sw are zero for c <= j); column j+1 goes through LDS after
+    // the whole update
 #pragma unroll
     for (int B = 0; B < NB; ++B) {
       if (8 * B + 7 <= j) continue;
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int c = 8 * B + t;
-        arow[c] -= v * sw[c] + wr * sv[c];
-        if (c == j + 1) sxn[lane] = arow[c];
+        arow[c] = fma(-wr, sv[c], fma(-v, sw[c], arow[c]));
       }
       __builtin_amdgcn_sched_barrier(0);
     }
+    store_column(arow, j + 1, &sxn[lane]);
     if (lane < KMAX) refl[(long long)j * kp + lane] = v;
     if (lane == 0) { se_s[j] = alpha; st_s[j] = tau; }
     __syncthreads();

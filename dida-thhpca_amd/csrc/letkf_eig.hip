@@ -95,16 +95,15 @@ __device__ void nan_transform(const EigParams& p, long long a) {
 // ---------------------------------------------------------------------------
 // k_tridiag
 // ---------------------------------------------------------------------------
-// Writes arow[c] for the uniform column c = j1 to dst: a switch on a
-// wavefront-uniform value, lowered to a short tree of scalar branches (no
-// per-column selects or dynamic register indexing), with one LDS store.
-template <int KMAX>
-__device__ __forceinline__ void store_column(const double (&arow)[KMAX], int j1, double* dst) {
+// arow[c] for a wavefront-uniform column c: a switch lowered to a short tree
+// of scalar branches (no per-column selects or dynamic register indexing)
+template <int N>
+__device__ __forceinline__ double pick_column(const double (&arow)[N], int c) {
   double x = 0.0;
-  switch (j1) {
-#define DIDA_COL_CASE(c) \
-  case c:                \
-    if constexpr (c < KMAX) x = arow[c]; \
+  switch (c) {
+#define DIDA_COL_CASE(i) \
+  case i:                \
+    if constexpr (i < N) x = arow[i]; \
     break;
 #define DIDA_COL_CASE8(b) \
   DIDA_COL_CASE(b) DIDA_COL_CASE(b + 1) DIDA_COL_CASE(b + 2) DIDA_COL_CASE(b + 3) \
@@ -116,7 +115,7 @@ __device__ __forceinline__ void store_column(const double (&arow)[KMAX], int j1,
     default:
       break;
   }
-  *dst = x;
+  return x;
 }
 
 // k = 64 is capped at 168 registers (three wavefronts per SIMD: the kernel is
@@ -166,7 +165,7 @@ void k_tridiag(EigParams p) {
     __builtin_amdgcn_sched_barrier(0);
     if (rest == 0.0) {
       // column already in tridiagonal form; column j+1 goes through LDS
-      store_column(arow, j + 1, &sxn[lane]);
+      sxn[lane] = pick_column(arow, j + 1);
       if (lane < KMAX) refl[(long long)j * kp + lane] = 0.0;
       if (lane == 0) { se_s[j] = x0; st_s[j] = 0.0; }
       __syncthreads();
@@ -216,7 +215,7 @@ void k_tridiag(EigParams p) {
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-    store_column(arow, j + 1, &sxn[lane]);
+    sxn[lane] = pick_column(arow, j + 1);
     if (lane < KMAX) refl[(long long)j * kp + lane] = v;
     if (lane == 0) { se_s[j] = alpha; st_s[j] = tau; }
     __syncthreads();
@@ -1142,13 +1141,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   auto pass_column = [&](int j, int nxt) {
     const int hj1 = (j + 1) & 1;
     if (h == hj1) {
-#pragma unroll
-      for (int c = 0; c < 64; ++c) {
-        if (2 * c + h == j + 1) {
-          sx[nxt][r] = (r > j + 1) ? arow[c] : 0.0;
-          if (r == j + 2) sx0[nxt] = arow[c];
-        }
-      }
+      const double xc = pick_column(arow, (j + 1) >> 1);
+      sx[nxt][r] = (r > j + 1) ? xc : 0.0;
+      if (r == j + 2) sx0[nxt] = xc;
     }
     const double xn = (h == hj1 && r > j + 2) ? sx[nxt][r] : 0.0;
     const double sq = wave_sum_full(xn * xn);
@@ -1200,7 +1195,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int c = 8 * B; c < 8 * B + 8; ++c) {
         const int cg = 2 * c + h;
-        arow[c] -= fma(vr, sw[cg], wr * col[cg]);
+        arow[c] = fma(-wr, col[cg], fma(-vr, sw[cg], arow[c]));
       }
       __builtin_amdgcn_sched_barrier(0);
     }

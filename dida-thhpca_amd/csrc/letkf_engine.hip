@@ -165,6 +165,83 @@ __global__ void k_iota(long long n, long long pad, int* __restrict__ out) {
 
 int kp_of(int k) { return k <= 32 ? 32 : (k <= 64 ? 64 : 128); }
 
+// Layout of the end-of-call summary, in 8-byte words: the flagged analyses,
+// the local obs rows (all levels), the largest column count, the columns over
+// 10,000 rows, the two eigensolver flag counters, the two split NaN counters,
+// then the per-batch row totals, bucket statistics and bucket ticks.
+constexpr int kSumHead = 8;
+
+struct SummaryParams {
+  const int* status;
+  long long n_an;
+  const int* counts;
+  long long n_h;
+  int nzc;
+  const int* flags;
+  const unsigned long long* nan;      // null without the device split
+  const long long* tot;
+  int n_tot;
+  const long long* bst;
+  int n_bst;
+  const unsigned long long* tk;
+  int n_tk;
+  long long* out;
+};
+
+// One workgroup of 1024 threads gathers a call's results into one small
+// array, so that the host reads them back with one copy.
+__global__ __launch_bounds__(1024) void k_call_summary(SummaryParams p) {
+  __shared__ long long red[3][16];
+  __shared__ int red_max[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  long long flagged = 0, rows = 0, over = 0;
+  int mx = 0;
+  for (long long i = t; i < p.n_an; i += 1024) flagged += p.status[i] != 0;
+  for (long long i = t; i < p.n_h; i += 1024) {
+    const int r = p.counts[i];
+    rows += (long long)r * p.nzc;
+    mx = max(mx, r);
+    over += r > 10000;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    flagged += __shfl_xor(flagged, o, 64);
+    rows += __shfl_xor(rows, o, 64);
+    over += __shfl_xor(over, o, 64);
+    mx = max(mx, __shfl_xor(mx, o, 64));
+  }
+  if (lane == 0) {
+    red[0][w] = flagged;
+    red[1][w] = rows;
+    red[2][w] = over;
+    red_max[w] = mx;
+  }
+  __syncthreads();
+  if (t == 0) {
+    long long a = 0, b = 0, c = 0;
+    int m = 0;
+    for (int i = 0; i < 16; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+      c += red[2][i];
+      m = max(m, red_max[i]);
+    }
+    p.out[0] = a;
+    p.out[1] = b;
+    p.out[2] = m;
+    p.out[3] = c;
+    p.out[4] = p.flags[0];
+    p.out[5] = p.flags[1];
+    p.out[6] = p.nan ? (long long)p.nan[0] : 0;
+    p.out[7] = p.nan ? (long long)p.nan[1] : 0;
+  }
+  long long* o = p.out + kSumHead;
+  for (int i = t; i < p.n_tot; i += 1024) o[i] = p.tot[i];
+  o += p.n_tot;
+  for (int i = t; i < p.n_bst; i += 1024) o[i] = p.bst[i];
+  o += p.n_bst;
+  for (int i = t; i < p.n_tk; i += 1024) o[i] = (long long)p.tk[i];
+}
+
 }  // namespace
 
 // Scratch of the split eigensolver for one batch of analyses.
@@ -501,6 +578,7 @@ struct dida_letkf_handle {
   // then reruns), per-batch totals and per-column counts read after a call
   long long row_cap = 0;
   DevBuf batch_tot;
+  DevBuf call_sum;                    // end-of-call summary (k_call_summary)
   // per-batch timing events (9 per batch), created once
   std::vector<hipEvent_t> ev;
   // hipGraph of a call's launches, replayed while every pointer and size it
@@ -670,6 +748,7 @@ int dida_letkf_destroy(dida_letkf_handle* h) {
                     &h->eig.rr_lam};
   for (DevBuf* b : bufs) b->release();
   h->batch_tot.release();
+  h->call_sum.release();
   for (DevBuf* b : {&h->b_order, &h->b_blk, &h->b_seg, &h->b_q, &h->b_stats, &h->b_ticks,
                     &h->sp_xb, &h->sp_xb32, &h->sp_axb, &h->sp_nan})
     b->release();
@@ -1397,41 +1476,51 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms, bool sp
   hipStream_t st = h->stream;
   const long long nb = n_batches_of(h);
   const bool queue = h->opt_schedule == 1 && !h->tiled;
-  std::vector<long long> tot(nb);
-  std::vector<int> counts(h->n_h), stt(h->n_an);
-  std::vector<long long> bst(queue ? nb * 2 * kBuckets : 0);
-  std::vector<unsigned long long> tk(queue && h->opt_timing ? nb * 5 * kBuckets : 0);
-  int fl[2] = {0, 0};
-  unsigned long long nan[2] = {0, 0};
-  if (split)
-    HIP_TRY(hipMemcpyAsync(nan, h->sp_nan.p, sizeof(nan), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(tot.data(), h->batch_tot.p, sizeof(long long) * nb, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(counts.data(), h->nobsl_dev.p, sizeof(int) * h->n_h, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(stt.data(), h->status.p, sizeof(int) * h->n_an, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(fl, h->eig.flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
-  if (!bst.empty())
-    HIP_TRY(hipMemcpyAsync(bst.data(), h->b_stats.p, sizeof(long long) * bst.size(),
+  const int n_bst = queue ? (int)(nb * 2 * kBuckets) : 0;
+  const int n_tk = queue && h->opt_timing ? (int)(nb * 5 * kBuckets) : 0;
+  const size_t words = kSumHead + (size_t)nb + n_bst + n_tk;
+  RC_TRY(h->call_sum.ensure(sizeof(long long) * words));
+  SummaryParams sp{};
+  sp.status = h->status.as<int>();
+  sp.n_an = h->n_an;
+  sp.counts = h->nobsl_dev.as<int>();
+  sp.n_h = h->n_h;
+  sp.nzc = h->nzc;
+  sp.flags = h->eig.flags.as<int>();
+  sp.nan = split ? h->sp_nan.as<unsigned long long>() : nullptr;
+  sp.tot = h->batch_tot.as<long long>();
+  sp.n_tot = (int)nb;
+  sp.bst = n_bst ? h->b_stats.as<long long>() : nullptr;
+  sp.n_bst = n_bst;
+  sp.tk = n_tk ? h->b_ticks.as<unsigned long long>() : nullptr;
+  sp.n_tk = n_tk;
+  sp.out = h->call_sum.as<long long>();
+  hipLaunchKernelGGL(k_call_summary, dim3(1), dim3(1024), 0, st, sp);
+  HIP_TRY(hipGetLastError());
+  std::vector<long long> sum(words);
+  HIP_TRY(hipMemcpyAsync(sum.data(), h->call_sum.p, sizeof(long long) * words,
+                         hipMemcpyDeviceToHost, st));
+  std::vector<int> counts;
+  if (c.keep_transforms) {
+    counts.resize(h->n_h);
+    HIP_TRY(hipMemcpyAsync(counts.data(), h->nobsl_dev.p, sizeof(int) * h->n_h,
                            hipMemcpyDeviceToHost, st));
-  if (!tk.empty())
-    HIP_TRY(hipMemcpyAsync(tk.data(), h->b_ticks.p, sizeof(unsigned long long) * tk.size(),
-                           hipMemcpyDeviceToHost, st));
+  }
   HIP_TRY(hipStreamSynchronize(st));
+  unsigned long long nan[2] = {(unsigned long long)sum[6], (unsigned long long)sum[7]};
   if (nan[0]) return nan_error(h, nan);
+  const long long* tot = sum.data() + kSumHead;
+  const long long* bst = tot + nb;
+  const long long* tk = bst + n_bst;
   need = 0;
-  for (long long t : tot) need = std::max(need, t);
+  for (long long b = 0; b < nb; ++b) need = std::max(need, tot[b]);
   if (need > h->row_cap) return 0;
-  int flagged = 0;
-  for (int s : stt) flagged += (s != 0);
+  const int flagged = (int)sum[0];
   if (c.keep_transforms)
     for (long long a = 0; a < h->n_an; ++a) h->nobsl_host[a] = counts[a / h->nzc];
   if (!stats) return flagged;
-  long long total_rows = 0;
-  int max_rows = 0, over_cap = 0;
-  for (int r : counts) {
-    total_rows += (long long)r * h->nzc;
-    max_rows = std::max(max_rows, r);
-    over_cap += r > 10000;
-  }
+  const long long total_rows = sum[1];
+  const int max_rows = (int)sum[2], over_cap = (int)sum[3];
   float ms_sel = 0, ms_prod = 0, ms_xf = 0, ms_k[4] = {0, 0, 0, 0};
   if (timed) {
     for (long long b = 0; b < nb; ++b) {
@@ -1466,7 +1555,7 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms, bool sp
   stats->ms_ql = ms_k[1];
   stats->ms_apply = ms_k[2];
   stats->ms_update = ms_k[3];
-  stats->n_rerun = fl[1];
+  stats->n_rerun = (int)sum[5];
   stats->schedule = queue ? 1 : 0;
   if (queue) {
     double busy = 0, cap = 0;
@@ -1475,12 +1564,12 @@ static int finish_call(dida_letkf_handle* h, bool timed, float graph_ms, bool sp
         stats->bucket_analyses[i] += bst[b * 2 * kBuckets + i];
         stats->bucket_rows[i] += bst[b * 2 * kBuckets + kBuckets + i];
       }
-      if (tk.empty()) continue;
-      const unsigned long long* t = tk.data() + b * 5 * kBuckets;
+      if (n_tk == 0) continue;
+      const long long* t = tk + b * 5 * kBuckets;
       for (int j = 0; j < kBuckets; ++j)
         stats->bucket_wave_ms[kBuckets - 1 - j] += (double)t[j] / h->clock_khz;
       for (int j = 0; j < kBuckets; ++j) {
-        const unsigned long long* l = t + kBuckets + 4 * j;
+        const long long* l = t + kBuckets + 4 * j;
         const long long slots = h->run_capacity[b * kBuckets + j];
         if (l[3] == 0 || l[2] == 0 || slots <= 0) continue;
         busy += (double)l[2];

@@ -432,6 +432,10 @@ __device__ __forceinline__ void ring_block(const double2* sring, int base, int B
 // through these pointer types go to the scalar unit (s_load).  Only for data
 // that no kernel writes while the reader runs.
 typedef const __attribute__((address_space(4))) int* uniform_int_cptr;
+typedef const __attribute__((address_space(4))) double* uniform_dbl_cptr;
+// a row of zeros in read-only global memory, for scalar reads past the last
+// reflector
+__device__ const double kZeroRow[128] = {};
 __device__ __forceinline__ int4 load_uniform_int4(uniform_int_cptr q, int i) {
   return make_int4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
 }
@@ -534,7 +538,15 @@ void k_vec(EigParams p) {
       double tau[4], al[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) tau[u] = j + u < j1 ? uniform(stau[j + u]) : 0.0;
-      const double* q0 = sbuf + t0 * LDV;
+      // the group's reflector entries are the same for every lane: they are
+      // read as scalars (s_load from the reflector block, rows past the last
+      // reflector from a zero row) and enter the FMAs as scalar operands, so
+      // the group's two passes take no LDS bandwidth
+      uniform_dbl_cptr qr[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        qr[u] = j + u < j1 ? (uniform_dbl_cptr)(refl + (long long)(j + u) * kp)
+                           : (uniform_dbl_cptr)kZeroRow;
       double acc[4][2];
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = 0.0;
@@ -545,7 +557,7 @@ void k_vec(EigParams p) {
         for (int t = 0; t < 8; ++t) {
           const int c = 8 * B + t;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) acc[u][t & 1] = fma(z[c], q0[u * LDV + c], acc[u][t & 1]);
+          for (int u = 0; u < 4; ++u) acc[u][t & 1] = fma(z[c], qr[u][c], acc[u][t & 1]);
         }
       }
 #pragma unroll
@@ -563,7 +575,7 @@ void k_vec(EigParams p) {
           const int c = 8 * B + t;
           double zc = z[c];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) zc = fma(-al[u], q0[u * LDV + c], zc);
+          for (int u = 0; u < 4; ++u) zc = fma(-al[u], qr[u][c], zc);
           z[c] = zc;
         }
       }

@@ -169,6 +169,26 @@ def test_vertical_level_units_k32():
     _assert_parity(gpu, ref, what="c1-reduced")
 
 
+def test_call_statistics_match_counts():
+    """The end-of-call summary (one workgroup reducing the per-analysis status
+    and the per-column row counts on the device) against the per-analysis
+    counts read back with the transforms and checked against the oracle's
+    nobsl: row total, largest count, columns over 10,000 rows, flagged."""
+    from dida_letkf import synth
+    for inst in (synth.generate("c2", seed=21, k=32, nlat=18, nlon=36, nz=4, n_obs=6000),
+                 synth.generate("c1", seed=22, nlat=12, nlon=24, nz=6, n_obs=3000)):
+        gpu, ref = _run_both(inst)
+        _assert_parity(gpu, ref, what="stats")
+        st, nob = gpu["stats"], np.asarray(gpu["nobsl"], np.int64)
+        nzc = nob.size // (inst.cfg.nlat * inst.cfg.nlon)
+        print("nzc", nzc, "rows", st["total_rows"], "max", st["max_rows"])
+        assert st["n_analyses"] == nob.size
+        assert st["total_rows"] == int(nob.sum()) > 0
+        assert st["max_rows"] == int(nob.max())
+        assert st["rows_over_10000"] == int((nob > 10000).sum()) // nzc
+        assert st["n_flagged"] == gpu["n_flagged"] == 0
+
+
 def test_vertical_lnp_hybrid():
     """ln p vertical coordinate with hybrid p(ps) (ps2alnp, obs_make.F90:418)."""
     from dida_letkf import synth, config
